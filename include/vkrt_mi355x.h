@@ -1,0 +1,118 @@
+/*
+ * Entry points that the MI355X build adds beside the reference API.
+ * None of these has a counterpart in src/core/api/vkrt.h; each comment names
+ * the reference code whose behaviour it exposes or extends.
+ */
+#ifndef VKRT_MI355X_H
+#define VKRT_MI355X_H
+
+#include "vkrt_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Random-number policy for the per-path streams.
+ * VKRT_RNG_HASH reproduces src/shaders/sampling/random.slang:7-27 (the default).
+ * VKRT_RNG_PCG32 uses a PCG32 stream per path, seeded by `seed` and selected by
+ * (pixel, frame, sample); each draw keeps the top 24 bits like random.slang:16-19. */
+#define VKRT_RNG_HASH 0u
+#define VKRT_RNG_PCG32 1u
+VKRT_Result VKRT_setRandomPolicy(VKRT* vkrt, uint32_t policy, uint64_t seed);
+
+/* Environment-map next-event estimation with an alias-table sampler and MIS
+ * (power heuristic) against BSDF sampling. The reference only gathers the
+ * environment when a BSDF ray misses (src/shaders/light/environment.slang:16-27);
+ * enabling this keeps the estimator unbiased but changes the random sequence,
+ * so only converged images compare with the reference. Default off. */
+VKRT_Result VKRT_setEnvironmentImportanceSampling(VKRT* vkrt, uint8_t enabled);
+
+/* Selects an already-added linear texture as the environment map (the
+ * reference's internal replaceEnvironmentTexture, src/core/scene/environment.c:11-41,
+ * reachable there only through a file path). */
+VKRT_Result VKRT_setEnvironmentTexture(VKRT* vkrt, uint32_t textureIndex);
+
+/* Reads back the accumulation buffer (RGBA32F, .w = sample count, as
+ * writeback.slang:110): `outRGBA` holds width*height*4 floats. In spectral
+ * modes .xyz is CIE XYZ (equal-energy white), as the reference stores it. */
+VKRT_Result VKRT_readAccumulation(const VKRT* vkrt, float* outRGBA, size_t floatCount);
+
+/* Reads back the display image (RGBA16 UNORM, writeback.slang:113). */
+VKRT_Result VKRT_readOutputImage(const VKRT* vkrt, uint16_t* outRGBA, size_t count);
+
+typedef struct VKRT_TraceStats {
+    uint64_t extensionRays; /* one per path per depth iteration = "bounces" */
+    uint64_t shadowRays;
+    uint64_t paths;
+    float traceMs;          /* wall time of the last VKRT_trace, device-synchronised */
+    float extendMs;         /* summed closest-hit traversal kernel time of the last trace */
+    float shadowMs;         /* summed shadow traversal kernel time */
+    float shadeMs;          /* summed shading kernel time */
+    uint32_t extendLaunches;
+    float bvhBuildMs;       /* last BVH build, device time */
+    uint32_t bvhNodeCount;
+    uint32_t bvhLeafTriangles;
+    uint32_t bvhMaxDepth;
+    uint32_t triangleCount;
+} VKRT_TraceStats;
+VKRT_Result VKRT_getTraceStats(const VKRT* vkrt, VKRT_TraceStats* outStats);
+
+/* Host copies of the packed scene buffers exactly as the kernels consume them
+ * (the reference's descriptor bindings 10-24, src/shaders/scene/resources.slang:7-59).
+ * Pointers stay valid until the next scene change or VKRT_destroy. */
+typedef struct VKRT_SceneBuffers {
+    const SceneData* sceneData;
+    const ShaderVertex* vertices;
+    uint32_t vertexCount;
+    const uint32_t* indices;
+    uint32_t indexCount;
+    const MeshInfo* meshInfos;
+    uint32_t meshCount;
+    const Material* materials;
+    uint32_t materialCount;
+    const EmissiveMesh* emissiveMeshes;
+    const EmissiveTriangle* emissiveTriangles;
+    const float* meshAliasQ;
+    const uint32_t* meshAliasIdx;
+    const float* triAliasQ;
+    const uint32_t* triAliasIdx;
+    const float* rgb2specTable; /* res scale floats then 3*res^3*3 coefficients */
+    uint32_t rgb2specFloatCount;
+    const float* environmentPixels; /* RGBA32F, NULL if none */
+    uint32_t environmentWidth;
+    uint32_t environmentHeight;
+    const float* worldTriangles;    /* 9 floats per triangle: world-space v0,v1,v2 in global order */
+    uint32_t triangleCount;
+    const uint32_t* triangleMesh;   /* global triangle -> mesh index */
+    uint32_t rngPolicy;
+    uint64_t rngSeed;
+    uint32_t environmentImportanceSampling;
+} VKRT_SceneBuffers;
+VKRT_Result VKRT_exportSceneBuffers(VKRT* vkrt, VKRT_SceneBuffers* out);
+
+/* Closest-hit queries through the GPU traversal for caller-supplied rays.
+ * rays: count * 8 floats (ox,oy,oz,tmin, dx,dy,dz,tmax).
+ * Outputs per ray: mesh (instance) index, primitive index within the mesh,
+ * hit distance and barycentrics (u,v) -- the payload of the reference's
+ * closest-hit shader (src/shaders/entry/path/closest_hit.slang:3-5);
+ * a miss reports VKRT_INVALID_INDEX. */
+VKRT_Result VKRT_traceRays(VKRT* vkrt, const float* rays, size_t count, uint32_t* outMesh, uint32_t* outPrimitive,
+                           float* outT, float* outBarycentrics);
+
+/* Primary-hit query for every pixel of the current viewport at a given sample
+ * index: the first TraceRay of the reference's path loop
+ * (src/shaders/integrator/path/state.slang:140-146, loop.slang:8-17), camera
+ * jitter drawn from the configured RNG policy. */
+VKRT_Result VKRT_tracePrimaryHits(VKRT* vkrt, uint32_t sampleIndex, uint32_t* outMesh, uint32_t* outPrimitive);
+
+/* Occlusion queries (any hit) for caller-supplied rays, classified like the
+ * reference's shadow closest-hit / miss shaders
+ * (src/shaders/entry/shadow/closest_hit.slang:4-8): 0 occluded, 1 visible,
+ * 2 occluded by a transmissive material. */
+VKRT_Result VKRT_traceShadowRays(VKRT* vkrt, const float* rays, size_t count, uint32_t* outVisibility);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

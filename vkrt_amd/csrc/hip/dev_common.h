@@ -1,0 +1,139 @@
+// Device-side common definitions for the gfx950 path tracer: small vector
+// math, the view of the scene the kernels read, and the wavefront queues.
+//
+// Floating-point policy: shading code is compiled without FMA contraction and
+// uses correctly rounded division / sqrt, so that with the same random stream
+// it follows the reference's arithmetic (src/shaders/**) closely enough for
+// per-sample comparisons; rsqrt(x) is spelled 1/sqrt(x) for the same reason.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "vkrt_shared.h"
+
+#define DEV __device__ __forceinline__
+
+namespace mrt {
+
+struct F2 {
+    float x, y;
+};
+struct F3 {
+    float x, y, z;
+};
+struct F4 {
+    float x, y, z, w;
+};
+
+DEV F3 f3(float x, float y, float z) { return F3{x, y, z}; }
+DEV F3 f3(float s) { return F3{s, s, s}; }
+DEV F4 f4(float x, float y, float z, float w) { return F4{x, y, z, w}; }
+DEV F4 f4(float s) { return F4{s, s, s, s}; }
+DEV F3 operator+(F3 a, F3 b) { return F3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+DEV F3 operator-(F3 a, F3 b) { return F3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+DEV F3 operator*(F3 a, F3 b) { return F3{a.x * b.x, a.y * b.y, a.z * b.z}; }
+DEV F3 operator/(F3 a, F3 b) { return F3{a.x / b.x, a.y / b.y, a.z / b.z}; }
+DEV F3 operator*(F3 a, float s) { return F3{a.x * s, a.y * s, a.z * s}; }
+DEV F3 operator/(F3 a, float s) { return F3{a.x / s, a.y / s, a.z / s}; }
+DEV F3 operator-(F3 a) { return F3{-a.x, -a.y, -a.z}; }
+DEV F4 operator+(F4 a, F4 b) { return F4{a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w}; }
+DEV F4 operator*(F4 a, F4 b) { return F4{a.x * b.x, a.y * b.y, a.z * b.z, a.w * b.w}; }
+DEV F4 operator*(F4 a, float s) { return F4{a.x * s, a.y * s, a.z * s, a.w * s}; }
+DEV F4 operator/(F4 a, float s) { return F4{a.x / s, a.y / s, a.z / s, a.w / s}; }
+
+DEV float dot(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+DEV F3 cross(F3 a, F3 b) { return F3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+DEV float rsq(float x) { return 1.0f / sqrtf(x); }
+DEV F3 normalize(F3 a) { return a * rsq(dot(a, a)); }
+DEV float sat(float x) { return fminf(fmaxf(x, 0.0f), 1.0f); }
+DEV F3 sat(F3 a) { return F3{sat(a.x), sat(a.y), sat(a.z)}; }
+DEV F4 sat(F4 a) { return F4{sat(a.x), sat(a.y), sat(a.z), sat(a.w)}; }
+DEV float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
+DEV float fracf(float x) { return x - floorf(x); }
+DEV float lerpf(float a, float b, float t) { return a + (b - a) * t; }
+DEV F3 lerp(F3 a, F3 b, float t) { return F3{lerpf(a.x, b.x, t), lerpf(a.y, b.y, t), lerpf(a.z, b.z, t)}; }
+DEV float maxc(F3 a) { return fmaxf(a.x, fmaxf(a.y, a.z)); }
+DEV float maxc(F4 a) { return fmaxf(fmaxf(a.x, a.y), fmaxf(a.z, a.w)); }
+DEV bool anyPos(F3 a) { return a.x > 0.0f || a.y > 0.0f || a.z > 0.0f; }
+DEV bool anyPos(F4 a) { return a.x > 0.0f || a.y > 0.0f || a.z > 0.0f || a.w > 0.0f; }
+DEV float sum(F4 a) { return a.x + a.y + a.z + a.w; }
+DEV F3 maxs(F3 a, float s) { return F3{fmaxf(a.x, s), fmaxf(a.y, s), fmaxf(a.z, s)}; }
+DEV F3 xyz(float4 v) { return F3{v.x, v.y, v.z}; }
+DEV F4 ld4(float4 v) { return F4{v.x, v.y, v.z, v.w}; }
+DEV float4 st4(F4 v) { return make_float4(v.x, v.y, v.z, v.w); }
+DEV float4 st4(F3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
+
+constexpr float kPi = 3.14159265358979323846f;
+constexpr float kInvPi = 0.31830988618379067154f;
+// src/shaders/scene/constants.slang:6-15
+constexpr float kRayTMin = 0.001f;
+constexpr float kRayTMax = 10000.0f;
+constexpr float kShadowOriginOffset = 0.001f;
+constexpr float kShadowDistanceOffset = 0.002f;
+constexpr float kRRMin = 0.05f;
+constexpr float kRRMax = 0.95f;
+constexpr float kSafeNormalizeEps = 1e-12f;
+
+// ---------------------------------------------------------------- BVH --
+// 4-wide node, 128 bytes = one cache line pair on gfx950; child boxes stored
+// per axis so a lane loads the node with eight 16-byte loads.
+struct alignas(16) BvhNode {
+    float4 loX, hiX, loY, hiY, loZ, hiZ;
+    uint4 child;  // node reference per slot (see below)
+    uint4 pad;
+};
+static_assert(sizeof(BvhNode) == 128, "BvhNode layout");
+
+// node reference: bit31 = leaf; leaf: bits 0..26 first triangle (BVH order),
+// bits 27..30 triangle count - 1; internal: node index. kEmptyRef = empty slot.
+constexpr uint32_t kLeafBit = 0x80000000u;
+constexpr uint32_t kEmptyRef = 0xFFFFFFFFu;
+constexpr uint32_t kLeafMaxTris = 4u;
+DEV uint32_t leafFirst(uint32_t ref) { return ref & 0x07FFFFFFu; }
+DEV uint32_t leafCount(uint32_t ref) { return ((ref >> 27) & 0xFu) + 1u; }
+
+// Triangles in BVH order: three float4 (v0.xyz + global triangle id bits, v1, v2).
+struct alignas(16) TriRecord {
+    float4 v0;
+    float4 v1;
+    float4 v2;
+};
+
+// What the kernels read of the scene. Passed by value as a kernel argument.
+struct SceneView {
+    const BvhNode* nodes;
+    const TriRecord* tris;
+    uint32_t triCount;
+    uint32_t nodeCount;
+    const ShaderVertex* vertices;
+    const uint32_t* indices;
+    const uint32_t* triMesh;      // global triangle -> mesh
+    const uint32_t* meshTriBase;  // mesh -> first global triangle
+    const MeshInfo* meshes;
+    const Material* materials;
+    uint32_t meshCount;
+    uint32_t materialCount;
+    const EmissiveMesh* emissiveMeshes;
+    const EmissiveTriangle* emissiveTriangles;
+    const float* meshAliasQ;
+    const uint32_t* meshAliasIdx;
+    const float* triAliasQ;
+    const uint32_t* triAliasIdx;
+    const float4* env;
+    uint32_t envWidth;
+    uint32_t envHeight;
+    const float* envAliasQ;
+    const uint32_t* envAliasIdx;
+    const float* envPdf;
+    const float* rgb2spec;
+};
+
+struct Hit {
+    float t;
+    uint32_t tri;  // global triangle id, kNoHit for a miss
+    float u, v;
+};
+constexpr uint32_t kNoHit = 0xFFFFFFFFu;
+
+}  // namespace mrt

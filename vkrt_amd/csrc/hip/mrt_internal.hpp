@@ -1,0 +1,136 @@
+// Internal state of one GPU context (one HIP device, one stream).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../mrt.h"
+#include "dev_common.h"
+
+namespace mrt {
+
+void setError(const std::string& msg);
+bool check(hipError_t e, const char* what);
+
+template <typename T>
+struct DeviceArray {
+    T* ptr = nullptr;
+    size_t count = 0;     // elements in use
+    size_t capacity = 0;  // elements allocated
+    bool reserve(size_t n) {
+        if (n <= capacity) return true;
+        if (ptr) hipFree(ptr);
+        ptr = nullptr;
+        capacity = 0;
+        size_t bytes = (n ? n : 1) * sizeof(T);
+        if (!check(hipMalloc(&ptr, bytes), "hipMalloc")) return false;
+        capacity = n;
+        return true;
+    }
+    bool upload(const T* host, size_t n, hipStream_t s) {
+        if (!reserve(n)) return false;
+        count = n;
+        if (n == 0) return true;
+        return check(hipMemcpyAsync(ptr, host, n * sizeof(T), hipMemcpyHostToDevice, s), "hipMemcpyAsync H2D");
+    }
+    void release() {
+        if (ptr) hipFree(ptr);
+        ptr = nullptr;
+        count = capacity = 0;
+    }
+};
+
+// Wavefront buffers, sized for the largest chunk of paths seen so far.
+struct PathBuffers {
+    DeviceArray<uint4> rng;
+    DeviceArray<uint32_t> flags;
+    DeviceArray<float4> thr, rad, radXYZ, tpp, ptpp, pbtp, lam, misc, sigma;
+    DeviceArray<float4> rays[2][2];  // [queue][origin+path | direction+tmax]
+    DeviceArray<float4> hits;
+    DeviceArray<float4> shadow[4];   // origin+path, direction+tmax, contribution if visible, otherwise
+    DeviceArray<float4> sampleRad;
+    size_t capacity = 0;
+};
+
+}  // namespace mrt
+
+struct mrt_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t evBegin = nullptr, evEnd = nullptr;
+    hipEvent_t evStage[8] = {};
+    int cuCount = 0;
+    std::string name;
+    uint32_t vendor = 0, driver = 0;
+
+    // scene
+    mrt::DeviceArray<ShaderVertex> vertices;
+    mrt::DeviceArray<uint32_t> indices;
+    mrt::DeviceArray<float> worldTris;
+    mrt::DeviceArray<uint32_t> triMesh;
+    mrt::DeviceArray<uint32_t> meshTriBase;
+    mrt::DeviceArray<mrt::TriRecord> tris;
+    mrt::DeviceArray<mrt::BvhNode> nodes;
+    mrt::DeviceArray<MeshInfo> meshes;
+    mrt::DeviceArray<Material> materials;
+    mrt::DeviceArray<EmissiveMesh> emissiveMeshes;
+    mrt::DeviceArray<EmissiveTriangle> emissiveTriangles;
+    mrt::DeviceArray<float> meshAliasQ, triAliasQ;
+    mrt::DeviceArray<uint32_t> meshAliasIdx, triAliasIdx;
+    mrt::DeviceArray<float4> env;
+    uint32_t envWidth = 0, envHeight = 0;
+    mrt::DeviceArray<float> envAliasQ, envPdf;
+    mrt::DeviceArray<uint32_t> envAliasIdx;
+    mrt::DeviceArray<float> rgb2spec;
+    uint32_t triCount = 0;
+    uint32_t bvhMaxDepth = 0;
+
+    // accumulation
+    mrt::DeviceArray<float4> accum;
+    mrt::DeviceArray<ushort4> output;
+    mrt::DeviceArray<float4> frameSum;
+    uint32_t width = 0, height = 0;
+
+    // wavefront + traversal scratch
+    mrt::PathBuffers paths;
+    mrt::DeviceArray<uint32_t> counters;
+    mrt::DeviceArray<uint32_t> traversalStack;
+    uint32_t traceGrid = 0;
+
+    mrt::SceneView view() const {
+        mrt::SceneView v{};
+        v.nodes = nodes.ptr;
+        v.tris = tris.ptr;
+        v.triCount = triCount;
+        v.nodeCount = triCount ? (uint32_t)nodes.count : 0u;
+        v.vertices = vertices.ptr;
+        v.indices = indices.ptr;
+        v.triMesh = triMesh.ptr;
+        v.meshTriBase = meshTriBase.ptr;
+        v.meshes = meshes.ptr;
+        v.materials = materials.ptr;
+        v.meshCount = (uint32_t)meshes.count;
+        v.materialCount = (uint32_t)materials.count;
+        v.emissiveMeshes = emissiveMeshes.ptr;
+        v.emissiveTriangles = emissiveTriangles.ptr;
+        v.meshAliasQ = meshAliasQ.ptr;
+        v.meshAliasIdx = meshAliasIdx.ptr;
+        v.triAliasQ = triAliasQ.ptr;
+        v.triAliasIdx = triAliasIdx.ptr;
+        v.env = envWidth ? env.ptr : nullptr;
+        v.envWidth = envWidth;
+        v.envHeight = envHeight;
+        v.envAliasQ = envAliasQ.ptr;
+        v.envAliasIdx = envAliasIdx.ptr;
+        v.envPdf = envPdf.ptr;
+        v.rgb2spec = rgb2spec.ptr;
+        return v;
+    }
+};
+
+// traversal launch helpers shared by the integrator and the query entry points
+namespace mrt {
+bool ensureTraversalScratch(mrt_context* ctx);
+}

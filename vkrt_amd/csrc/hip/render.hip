@@ -1,0 +1,1152 @@
+// Wavefront path tracer for gfx950 -- the MI355X replacement for the
+// reference's raygen megakernels (src/shaders/entry/path/raygen_*.slang) and
+// their per-pixel loop (src/shaders/integrator/path/{rgb,spectral_single,
+// spectral_hero}/integrator.slang).
+//
+// One frame = samplesPerPixel paths per viewport pixel, processed in chunks of
+// at most kMaxChunk paths. Path p of the frame is (sample s, pixel k) with
+// p = s * pixels + k, so a chunk of whole samples touches every pixel once per
+// sample and consecutive lanes shade neighbouring pixels. Per chunk:
+//   generate -> [ extend (closest hit) -> shade -> shadow (any hit) ] x depth
+//   -> finalize (per-path XYZ / RGB) -> reduce (per-pixel sum in sample order)
+// and once per frame: accumulate (running mean, display encode).
+// Queues live in HBM and are compacted with one atomic per wavefront.
+// Shadow rays carry two contributions (visible / not visible) so the
+// radiance update is the reference's `radiance += throughput * contribution`
+// evaluated with the same operands once visibility is known.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+
+#include "dev_shading.h"
+#include "dev_traverse.h"
+#include "mrt_internal.hpp"
+
+namespace mrt {
+
+enum : int { kModeRgb = 0, kModeSingle = 1, kModeHero = 2 };
+
+constexpr uint32_t kFPrevNee = 1u;
+constexpr uint32_t kFHeroActive = 2u;
+constexpr uint32_t kFRefractive = 4u;
+constexpr uint32_t kFAbsorption = 8u;
+constexpr uint32_t kFPendingCollapse = 16u;
+
+constexpr uint32_t kVisOccluded = 0u, kVisVisible = 1u, kVisUnsupported = 2u;
+
+constexpr uint64_t kMaxChunk = 1ull << 23;
+constexpr int kBlock = 256;
+
+// counters[] layout
+constexpr int kCntQueue0 = 0, kCntQueue1 = 1, kCntShadow = 2, kCntFetchExt = 3, kCntFetchShadow = 4,
+              kCntDropped = 5, kCntStats = 8;  // 64-bit stats at counters+8: [0] ext rays, [1] shadow rays
+
+struct FrameParams {
+    SceneData sd;
+    uint64_t pathBegin;
+    uint32_t count;
+    uint32_t numPixels;
+    uint32_t vpW;
+    uint32_t accumWidth;
+    uint32_t rngPolicy;
+    uint64_t rngSeed;
+    uint32_t neeEnabled, bsdfOnly, neeOnly;
+    uint32_t depth;
+    uint32_t traceShadowAlways;
+    Spec spec;
+};
+
+struct PathView {
+    uint4* rng;
+    uint32_t* flags;
+    float4 *thr, *rad, *radXYZ, *tpp, *ptpp, *pbtp, *lam, *misc, *sigma;
+};
+
+struct QueueView {
+    float4* o;
+    float4* d;
+    uint32_t* count;
+};
+
+struct ShadowView {
+    float4 *o, *d, *a, *b;
+    uint32_t* count;
+};
+
+DEV Rng loadRng(uint4 v, uint32_t policy) {
+    Rng r;
+    r.policy = policy;
+    r.state = ((uint64_t)v.y << 32) | v.x;
+    r.inc = ((uint64_t)v.w << 32) | v.z;
+    return r;
+}
+DEV uint4 storeRng(const Rng& r) {
+    return make_uint4((uint32_t)r.state, (uint32_t)(r.state >> 32), (uint32_t)r.inc, (uint32_t)(r.inc >> 32));
+}
+
+// Appends this lane's item if pred; one atomic per wavefront. Every lane of
+// the wavefront must call it. Returns the slot, or 0xFFFFFFFF.
+DEV uint32_t waveAppend(bool pred, uint32_t* counter) {
+    unsigned long long mask = __ballot(pred);
+    if (mask == 0ull) return 0xFFFFFFFFu;
+    uint32_t lane = __lane_id();
+    int leader = __ffsll(mask) - 1;
+    uint32_t base = 0;
+    if ((int)lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+    base = (uint32_t)__shfl((int)base, leader);
+    return pred ? base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull)) : 0xFFFFFFFFu;
+}
+
+// ------------------------------------------------------------ lights --
+// src/shaders/sampling/discrete.slang:8-18
+DEV float powerHeuristic(float a, float b) {
+    float a2 = a * a;
+    return a2 / (a2 + b * b);
+}
+DEV uint32_t sampleAlias(float u, uint32_t count, uint32_t offset, const float* q, const uint32_t* idx) {
+    float scaled = u * (float)count;
+    uint32_t i = min((uint32_t)scaled, count - 1u);
+    float rem = scaled - (float)i;
+    return rem < q[offset + i] ? i : idx[offset + i];
+}
+
+struct LightSurfaceSample {
+    F3 wi;
+    F3 emission;
+    float pdfSolidAngle, shadowDistance;
+    bool valid;
+};
+
+// src/shaders/light/direct/light_sampling.slang:9-63
+DEV LightSurfaceSample sampleLightSurface(const SceneView& sv, const SceneData& sd, F3 hitPoint, Rng& rng) {
+    LightSurfaceSample s;
+    s.valid = false;
+    uint32_t meshCount = sd.emissiveMeshCount;
+    if (meshCount == 0u) return s;
+    uint32_t mi = sampleAlias(rnd(rng), meshCount, 0u, sv.meshAliasQ, sv.meshAliasIdx);
+    const EmissiveMesh em = sv.emissiveMeshes[mi];
+    if (em.triCount == 0u) return s;
+    uint32_t lt = sampleAlias(rnd(rng), em.triCount, em.triOffset, sv.triAliasQ, sv.triAliasIdx);
+    const EmissiveTriangle tri = sv.emissiveTriangles[em.triOffset + lt];
+    float u1 = rnd(rng);
+    float u2 = rnd(rng);
+    float su1 = sqrtf(u1);
+    float b1 = u2 * su1;
+    float b2 = (1.0f - u2) * su1;
+    F3 v0 = F3{tri.v0Area[0], tri.v0Area[1], tri.v0Area[2]};
+    F3 e1 = F3{tri.e1Pad[0], tri.e1Pad[1], tri.e1Pad[2]};
+    F3 e2 = F3{tri.e2Pad[0], tri.e2Pad[1], tri.e2Pad[2]};
+    F3 pos = v0 + e1 * b1 + e2 * b2;
+    F3 nrm = safeNormalize(cross(e1, e2));
+    s.emission = F3{em.emission[0], em.emission[1], em.emission[2]};
+    float pdf = em.pmfMesh * em.invTotalArea;
+    if (!(pdf > 0.0f)) return s;
+    F3 toL = pos - hitPoint;
+    float d2 = dot(toL, toL);
+    if (d2 <= 0.0f) return s;
+    float invD = rsq(d2);
+    float dist = d2 * invD;
+    s.wi = toL * invD;
+    float cosL = fabsf(dot(s.wi, nrm));
+    if (cosL <= 0.0f) return s;
+    s.pdfSolidAngle = pdf * d2 / cosL;
+    if (s.pdfSolidAngle <= 0.0f) return s;
+    s.shadowDistance = dist - kShadowDistanceOffset;
+    if (s.shadowDistance <= 0.0f) return s;
+    s.valid = true;
+    return s;
+}
+
+// src/shaders/light/environment.slang:9-27 with the reference's environment
+// sampler (linear filter, repeat U, clamp-to-edge V, level 0)
+DEV F3 environmentRadiance(const SceneView& sv, const SceneData& sd, F3 dirIn) {
+    if (sd.environmentTextureIndex == VKRT_INVALID_INDEX || !sv.env)
+        return F3{sd.environmentLight[0], sd.environmentLight[1], sd.environmentLight[2]};
+    F3 d = normalize(dirIn);
+    float phi = atan2f(d.y, d.x) + sd.environmentRotation * (kPi / 180.0f);
+    float theta = acosf(clampf(d.z, -1.0f, 1.0f));
+    float u = fracf(phi * (0.5f * kInvPi) + 0.5f);
+    float v = theta * kInvPi;
+    int W = (int)sv.envWidth, H = (int)sv.envHeight;
+    float fx = u * (float)W - 0.5f, fy = v * (float)H - 0.5f;
+    float x0f = floorf(fx), y0f = floorf(fy);
+    float tx = fx - x0f, ty = fy - y0f;
+    int x0 = (int)x0f, y0 = (int)y0f;
+    int x1 = x0 + 1, y1 = y0 + 1;
+    x0 = ((x0 % W) + W) % W;
+    x1 = ((x1 % W) + W) % W;
+    y0 = min(max(y0, 0), H - 1);
+    y1 = min(max(y1, 0), H - 1);
+    F3 a00 = xyz(sv.env[(size_t)y0 * W + x0]), a10 = xyz(sv.env[(size_t)y0 * W + x1]);
+    F3 a01 = xyz(sv.env[(size_t)y1 * W + x0]), a11 = xyz(sv.env[(size_t)y1 * W + x1]);
+    F3 a = lerp(a00, a10, tx), b = lerp(a01, a11, tx);
+    return lerp(a, b, ty) * sd.environmentLight[3];
+}
+
+// src/shaders/light/direct/mis_weights.slang:6-42
+DEV float areaToSolidAngle(float pdfArea, F3 gn, F3 d, float t) {
+    if (pdfArea <= 0.0f) return 0.0f;
+    float d2 = t * t;
+    if (d2 <= 0.0f) return 0.0f;
+    float cosL = fabsf(dot(d, gn));
+    if (cosL <= 0.0f) return 0.0f;
+    return pdfArea * d2 / cosL;
+}
+DEV float spectralMis(F4 a, F4 b) {
+    float den = sum(a) + sum(b);
+    return den > 0.0f ? a.x / den : 0.0f;
+}
+
+DEV F3 transmittance3(F4 sigma, bool absorbing, float d) {
+    if (!absorbing) return f3(1.0f);
+    return F3{expf(-sigma.x * d), expf(-sigma.y * d), expf(-sigma.z * d)};
+}
+DEV F4 transmittance4(F4 sigma, bool absorbing, float d) {
+    if (!absorbing) return f4(1.0f);
+    return F4{expf(-sigma.x * d), expf(-sigma.y * d), expf(-sigma.z * d), expf(-sigma.w * d)};
+}
+
+// src/shaders/bsdf/base/medium.slang:75-113
+DEV void updateMedium(const Spec& sp, const BMat& m, uint32_t ff, uint32_t isT, float l, uint32_t spectral,
+                      uint32_t& flags, F4& sigma) {
+    if (!isT) return;
+    bool entering = ff != 0u;
+    bool refr = entering && isRefractive(m);
+    bool absb = entering && hasAbsorption(m);
+    flags = (flags & ~(kFRefractive | kFAbsorption)) | (refr ? kFRefractive : 0u) | (absb ? kFAbsorption : 0u);
+    F3 s3v = f3(0.0f);
+    if (absb) {
+        if (!spectral) {
+            F3 t = maxs(sat(m.attenuationColor), 1e-6f);
+            s3v = F3{-logf(t.x), -logf(t.y), -logf(t.z)} * m.absorptionCoefficient;
+        } else {
+            float t = fmaxf(spec1(sp, sat(m.attenuationColor), l), 1e-6f);
+            s3v = f3(-logf(t) * m.absorptionCoefficient);
+        }
+    }
+    sigma = spectral ? F4{s3v.x, s3v.x, s3v.x, s3v.x} : F4{s3v.x, s3v.y, s3v.z, s3v.x};
+}
+DEV void updateMedium4(const Spec& sp, const BMat& m, uint32_t ff, uint32_t isT, F4 l, uint32_t& flags, F4& sigma) {
+    if (!isT) return;
+    bool entering = ff != 0u;
+    bool refr = entering && isRefractive(m);
+    bool absb = entering && hasAbsorption(m);
+    flags = (flags & ~(kFRefractive | kFAbsorption)) | (refr ? kFRefractive : 0u) | (absb ? kFAbsorption : 0u);
+    if (!absb) {
+        sigma = f4(0.0f);
+        return;
+    }
+    F4 t = spec4(sp, sat(m.attenuationColor), l);
+    t = F4{fmaxf(t.x, 1e-6f), fmaxf(t.y, 1e-6f), fmaxf(t.z, 1e-6f), fmaxf(t.w, 1e-6f)};
+    sigma = F4{-logf(t.x), -logf(t.y), -logf(t.z), -logf(t.w)} * m.absorptionCoefficient;
+}
+
+// ------------------------------------------------------------ kernels --
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void kGenerate(FrameParams fp, PathView pv, QueueView q, const float4* accum) {
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= fp.count) return;
+    uint64_t p = fp.pathBegin + i;
+    uint32_t s = (uint32_t)(p / fp.numPixels);
+    uint32_t k = (uint32_t)(p % fp.numPixels);
+    int px = (int)(fp.sd.viewportRect[0] + k % fp.vpW);
+    int py = (int)(fp.sd.viewportRect[1] + k / fp.vpW);
+    float4 prev = accum[(size_t)py * fp.accumWidth + px];
+    uint32_t sampleAbs = (uint32_t)(prev.w + 0.5f) + s;
+    Rng rng = rngInit(fp.rngPolicy, fp.rngSeed, px, py, fp.sd.frameNumber, sampleAbs);
+    float jx = rnd(rng);
+    float jy = rnd(rng);
+    F3 o, d;
+    primaryRay(fp.sd, px, py, jx - 0.5f, jy - 0.5f, o, d);
+    uint32_t flags = 0u;
+    if (MODE == kModeSingle) {
+        float u = wavelengthUnit(rng, sampleAbs);
+        pv.lam[i] = make_float4(360.0f + sat(u) * 470.0f, 470.0f, 0.0f, 0.0f);
+    } else if (MODE == kModeHero) {
+        float u = wavelengthUnit(rng, sampleAbs);
+        pv.lam[i] = make_float4(360.0f + fracf(u + 0.0f) * 470.0f, 360.0f + fracf(u + 0.25f) * 470.0f,
+                                360.0f + fracf(u + 0.5f) * 470.0f, 360.0f + fracf(u + 0.75f) * 470.0f);
+        pv.radXYZ[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        pv.tpp[i] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+        pv.ptpp[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        pv.pbtp[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        flags |= kFHeroActive;
+    }
+    pv.rng[i] = storeRng(rng);
+    pv.flags[i] = flags;
+    pv.thr[i] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+    pv.rad[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    pv.misc[i] = make_float4(0.0f, 1.0f, 0.0f, 0.0f);
+    pv.sigma[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    q.o[i] = make_float4(o.x, o.y, o.z, __uint_as_float(i));
+    q.d[i] = make_float4(d.x, d.y, d.z, kRayTMax);
+}
+
+__global__ void kPrepDepth(uint32_t* counters, int cur, int nxt) {
+    unsigned long long* stats = reinterpret_cast<unsigned long long*>(counters + kCntStats);
+    stats[0] += counters[cur];
+    counters[nxt] = 0u;
+    counters[kCntShadow] = 0u;
+    counters[kCntFetchExt] = 0u;
+    counters[kCntFetchShadow] = 0u;
+}
+__global__ void kPrepShadow(uint32_t* counters) {
+    unsigned long long* stats = reinterpret_cast<unsigned long long*>(counters + kCntStats);
+    stats[1] += counters[kCntShadow];
+}
+__global__ void kSetCount(uint32_t* counters, int slot, uint32_t value) { counters[slot] = value; }
+
+// Closest-hit traversal, persistent: each wavefront claims 64 consecutive rays at a time.
+__global__ __launch_bounds__(kTraceBlock) void kExtend(SceneView sv, const float4* ro, const float4* rd, float4* hits,
+                                                      uint32_t* counters, int countSlot, uint32_t* gstack) {
+    __shared__ uint32_t lds[kLdsStack * kTraceBlock];
+    TraversalStack st;
+    st.lds = lds + threadIdx.x;
+    st.glob = gstack + (size_t)(blockIdx.x * kTraceBlock + threadIdx.x) * kGlobalStack;
+    st.dropped = 0;
+    const uint32_t n = counters[countSlot];
+    const uint32_t lane = threadIdx.x & 63u;
+    while (true) {
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(&counters[kCntFetchExt], 64u);
+        base = (uint32_t)__shfl((int)base, 0);
+        if (base >= n) break;
+        uint32_t i = base + lane;
+        if (i < n) {
+            float4 a = ro[i], b = rd[i];
+            TraversalRay r = makeTraversalRay(xyz(a), xyz(b));
+            Hit h = traceClosest(sv, r, kRayTMin, b.w, st);
+            hits[i] = make_float4(h.t, __uint_as_float(h.tri), h.u, h.v);
+        }
+    }
+    if (st.dropped) atomicAdd(&counters[kCntDropped], st.dropped);
+}
+
+DEV uint32_t classifyOccluder(const SceneView& sv, uint32_t tri) {
+    if (tri == kNoHit) return kVisVisible;
+    const Material& m = sv.materials[sv.meshes[sv.triMesh[tri]].materialIndex];
+    return m.transmission > 0.0f ? kVisUnsupported : kVisOccluded;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kTraceBlock) void kShadow(SceneView sv, PathView pv, ShadowView sh, uint32_t* counters,
+                                                      uint32_t* gstack) {
+    __shared__ uint32_t lds[kLdsStack * kTraceBlock];
+    TraversalStack st;
+    st.lds = lds + threadIdx.x;
+    st.glob = gstack + (size_t)(blockIdx.x * kTraceBlock + threadIdx.x) * kGlobalStack;
+    st.dropped = 0;
+    const uint32_t n = counters[kCntShadow];
+    const uint32_t lane = threadIdx.x & 63u;
+    while (true) {
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(&counters[kCntFetchShadow], 64u);
+        base = (uint32_t)__shfl((int)base, 0);
+        if (base >= n) break;
+        uint32_t i = base + lane;
+        if (i >= n) continue;
+        float4 o = sh.o[i], d = sh.d[i];
+        uint32_t q = __float_as_uint(o.w);
+        TraversalRay r = makeTraversalRay(xyz(o), xyz(d));
+        uint32_t vis = classifyOccluder(sv, traceAny(sv, r, kRayTMin, d.w, st));
+        F4 c = ld4(vis == kVisVisible ? sh.a[i] : sh.b[i]);
+        uint32_t flags = pv.flags[q];
+        uint32_t newFlags = flags;
+        if (vis == kVisUnsupported) newFlags &= ~kFPrevNee;
+        if (MODE == kModeRgb) {
+            F4 rad = ld4(pv.rad[q]);
+            pv.rad[q] = st4(F3{rad.x, rad.y, rad.z} + F3{c.x, c.y, c.z}, 0.0f);
+        } else if (MODE == kModeSingle) {
+            F4 rad = ld4(pv.rad[q]);
+            rad.x += c.x;
+            pv.rad[q] = st4(rad);
+        } else {
+            if (flags & kFPendingCollapse) {
+                // src/shaders/integrator/path/spectral_hero/transport.slang:77-87, deferred until this
+                // vertex's contribution is known
+                F4 rad = ld4(pv.rad[q]) + c;
+                F4 rx = ld4(pv.radXYZ[q]);
+                F3 xyzv = F3{rx.x, rx.y, rx.z} + toXYZ4(rad, ld4(pv.lam[q]), f4(470.0f));
+                pv.radXYZ[q] = make_float4(xyzv.x, xyzv.y, xyzv.z, rx.w);
+                pv.rad[q] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                newFlags &= ~kFPendingCollapse;
+            } else if (flags & kFHeroActive) {
+                pv.rad[q] = st4(ld4(pv.rad[q]) + c);
+            } else {
+                F4 rx = ld4(pv.radXYZ[q]);
+                rx.w += c.x;
+                pv.radXYZ[q] = st4(rx);
+            }
+        }
+        if (newFlags != flags) pv.flags[q] = newFlags;
+    }
+    if (st.dropped) atomicAdd(&counters[kCntDropped], st.dropped);
+}
+
+// One path vertex for every ray of the current queue.
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void kShade(FrameParams fp, SceneView sv, PathView pv, QueueView in,
+                                                 const float4* hits, QueueView out, ShadowView sh, uint32_t* counters,
+                                                 int inSlot, int outSlot) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t n = counters[inSlot];
+    bool cont = false, shadowEmit = false;
+    F3 nextO = f3(0.0f), nextD = f3(0.0f), shO = f3(0.0f), shD = f3(0.0f);
+    float shT = 0.0f;
+    F4 shA = f4(0.0f), shB = f4(0.0f);
+    uint32_t q = 0;
+    if (i < n) do {
+        const SceneData& sd = fp.sd;
+        float4 ro = in.o[i], rdir = in.d[i];
+        q = __float_as_uint(ro.w);
+        F3 o = xyz(ro), d = xyz(rdir);
+        float4 hh = hits[i];
+        Hit h{hh.x, __float_as_uint(hh.y), hh.z, hh.w};
+        uint32_t flags = pv.flags[q];
+        const bool absorbing = (flags & kFAbsorption) != 0u;
+        const bool refractive = (flags & kFRefractive) != 0u;
+        const bool heroActive = MODE == kModeHero && (flags & kFHeroActive) != 0u;
+        F4 thr = ld4(pv.thr[q]);
+        F4 misc = ld4(pv.misc[q]);  // prevBsdfPdf, scalar throughput, scalar wavelength, scalar 1/pdf
+        F4 lam = (MODE != kModeRgb) ? ld4(pv.lam[q]) : f4(0.0f);
+
+        if (h.tri == kNoHit) {
+            if (!fp.neeOnly && !(refractive || absorbing)) {
+                F3 env = environmentRadiance(sv, sd, d);
+                if (MODE == kModeRgb) {
+                    F4 rad = ld4(pv.rad[q]);
+                    pv.rad[q] = st4(F3{rad.x, rad.y, rad.z} + F3{thr.x, thr.y, thr.z} * env, 0.0f);
+                } else if (MODE == kModeSingle) {
+                    F4 rad = ld4(pv.rad[q]);
+                    rad.x += thr.x * spec1(fp.spec, env, lam.x);
+                    pv.rad[q] = st4(rad);
+                } else if (heroActive) {
+                    F4 tpp = ld4(pv.tpp[q]);
+                    pv.rad[q] = st4(ld4(pv.rad[q]) + thr * heroBalance(tpp) * spec4(fp.spec, env, lam));
+                } else {
+                    F4 rx = ld4(pv.radXYZ[q]);
+                    rx.w += misc.y * spec1(fp.spec, env, misc.z);
+                    pv.radXYZ[q] = st4(rx);
+                }
+            }
+            break;
+        }
+
+        F4 sigma = ld4(pv.sigma[q]);
+        if (absorbing) {
+            if (MODE == kModeRgb) {
+                F3 t = transmittance3(sigma, true, h.t);
+                thr = F4{thr.x * t.x, thr.y * t.y, thr.z * t.z, thr.w};
+            } else if (MODE == kModeSingle) {
+                thr.x *= transmittance3(sigma, true, h.t).x;
+            } else if (heroActive) {
+                thr = thr * transmittance4(sigma, true, h.t);
+            } else {
+                misc.y *= transmittance3(sigma, true, h.t).x;
+            }
+        }
+        if (MODE == kModeRgb ? !anyPos(F3{thr.x, thr.y, thr.z})
+                             : (MODE == kModeSingle ? !(thr.x > 0.0f) : (heroActive ? !anyPos(thr) : !(misc.y > 0.0f))))
+            break;
+
+        Surface sf = reconstruct(sv, o, d, h);
+        BMat mat = loadMaterial(sv.materials, sf.materialIndex, sf.vertexColor);
+        F3 wo = toLocal(-d, sf.basis);
+        const float stateLambda = MODE == kModeRgb ? 0.0f : (MODE == kModeSingle ? lam.x : (heroActive ? lam.x : misc.z));
+        BState st = makeState(mat, wo, sf.frontFace, stateLambda, MODE == kModeRgb ? 0u : 1u);
+        const bool curNee = !refractive;
+        Rng rng = loadRng(pv.rng[q], fp.rngPolicy);
+        F4 tpp = heroActive ? ld4(pv.tpp[q]) : f4(0.0f);
+
+        // next-event estimation (src/shaders/light/direct/*_lighting.slang): the contribution is
+        // evaluated here assuming visibility; the shadow kernel picks it or drops it
+        F4 nee = f4(0.0f);
+        bool neeTried = false;
+        if (!fp.bsdfOnly && fp.neeEnabled && curNee && wo.z > 0.0f) {
+            LightSurfaceSample ls = sampleLightSurface(sv, sd, sf.p, rng);
+            if (ls.valid) {
+                neeTried = true;
+                F3 off = dot(ls.wi, sf.ng) >= 0.0f ? sf.ng : -sf.ng;
+                shO = sf.p + off * kShadowOriginOffset;
+                shD = ls.wi;
+                shT = ls.shadowDistance;
+                F3 wl = toLocal(ls.wi, sf.basis);
+                if (!(isRefractive(mat) && wl.z <= 0.0f)) {
+                    if (MODE == kModeRgb) {
+                        Eval e = evalScalar(fp.spec, st, wl);
+                        if (e.pdf > 0.0f) {
+                            float mis = powerHeuristic(ls.pdfSolidAngle, e.pdf);
+                            F3 tr = transmittance3(sigma, absorbing, ls.shadowDistance);
+                            F3 fc = e.value * fabsf(wl.z);
+                            F3 r = tr * mis * fc * ls.emission / ls.pdfSolidAngle;
+                            nee = F4{r.x, r.y, r.z, 0.0f};
+                        }
+                    } else if (heroActive) {
+                        F4 btp;
+                        F4 fc = evalHero(fp.spec, st, wl, lam, btp) * fabsf(wl.z);
+                        float mis = spectralMis(tpp * ls.pdfSolidAngle, tpp * btp);
+                        if (mis > 0.0f) {
+                            F4 tr = transmittance4(sigma, absorbing, ls.shadowDistance);
+                            nee = tr * mis * fc * spec4(fp.spec, ls.emission, lam) / ls.pdfSolidAngle;
+                        }
+                    } else {
+                        Eval e = evalScalar(fp.spec, st, wl);
+                        if (e.pdf > 0.0f) {
+                            float sv1 = e.value.x * fabsf(wl.z) * spec1(fp.spec, ls.emission, stateLambda);
+                            float mis = powerHeuristic(ls.pdfSolidAngle, e.pdf);
+                            nee.x = mis * transmittance3(sigma, absorbing, ls.shadowDistance).x * sv1 / ls.pdfSolidAngle;
+                        }
+                    }
+                }
+            }
+        }
+
+        // emission with MIS (src/shaders/integrator/path/loop.slang:62-81, mis_weights.slang)
+        F4 em = f4(0.0f);
+        bool hasEm = false;
+        if (!fp.neeOnly && anyPos(mat.emission)) {
+            hasEm = true;
+            const bool prevNee = (flags & kFPrevNee) != 0u;
+            if (MODE == kModeHero && heroActive) {
+                float w = heroBalance(tpp);
+                if (fp.neeEnabled && prevNee && fp.depth > 0u) {
+                    float lp = areaToSolidAngle(sv.meshes[sf.mesh].lightPdfArea, sf.ng, d, h.t);
+                    F4 ptpp = ld4(pv.ptpp[q]), pbtp = ld4(pv.pbtp[q]);
+                    w = spectralMis(ptpp * pbtp, ptpp * lp);
+                }
+                em = spec4(fp.spec, mat.emission, lam) * w;
+            } else {
+                float w = 1.0f;
+                if (fp.neeEnabled && prevNee && fp.depth != 0u && misc.x > 0.0f) {
+                    float lp = areaToSolidAngle(sv.meshes[sf.mesh].lightPdfArea, sf.ng, d, h.t);
+                    w = lp <= 0.0f ? 1.0f : powerHeuristic(misc.x, lp);
+                }
+                if (MODE == kModeRgb)
+                    em = F4{mat.emission.x * w, mat.emission.y * w, mat.emission.z * w, 0.0f};
+                else
+                    em.x = spec1(fp.spec, mat.emission, stateLambda) * w;
+            }
+        }
+
+        // radiance += throughput * contribution, now or once visibility is known
+        F4 thrUse = (MODE == kModeHero && !heroActive) ? F4{misc.y, 0.0f, 0.0f, 0.0f} : thr;
+        if (neeTried && (fp.traceShadowAlways || anyPos(nee))) {
+            shadowEmit = true;
+            F4 cv = f4(0.0f) + nee;
+            if (hasEm) cv = cv + em;
+            F4 ch = f4(0.0f) + f4(0.0f);
+            if (hasEm) ch = ch + em;
+            shA = thrUse * cv;
+            shB = thrUse * ch;
+        } else if (hasEm || neeTried) {
+            F4 cv = f4(0.0f);
+            if (neeTried) cv = cv + f4(0.0f);
+            if (hasEm) cv = cv + em;
+            F4 add = thrUse * cv;
+            if (MODE == kModeRgb) {
+                F4 rad = ld4(pv.rad[q]);
+                pv.rad[q] = st4(F3{rad.x, rad.y, rad.z} + F3{add.x, add.y, add.z}, 0.0f);
+            } else if (MODE == kModeSingle) {
+                F4 rad = ld4(pv.rad[q]);
+                rad.x += add.x;
+                pv.rad[q] = st4(rad);
+            } else if (heroActive) {
+                pv.rad[q] = st4(ld4(pv.rad[q]) + add);
+            } else {
+                F4 rx = ld4(pv.radXYZ[q]);
+                rx.w += add.x;
+                pv.radXYZ[q] = st4(rx);
+            }
+        }
+
+        // the last vertex of the loop never traces its continuation
+        if (fp.depth + 1u >= sd.rrMaxDepth) break;
+
+        // BSDF sampling (src/shaders/bsdf/sample_rgb.slang, sample_spectral.slang)
+        if (wo.z <= 0.0f) break;
+        F3 wl;
+        uint32_t isT;
+        if (!sampleLobe(st, rng, wl, isT)) break;
+        F3 wi = toWorld(wl, sf.basis);
+        uint32_t newFlags = flags & ~(kFPrevNee | kFPendingCollapse);
+        if (heroActive) {
+            F4 btp;
+            F4 val = evalHero(fp.spec, st, wl, lam, btp);
+            float sp = btp.x;
+            if (!(sp > 0.0f)) break;
+            F4 weight = val * fabsf(wl.z) / sp;
+            if (!anyPos(weight)) break;
+            pv.ptpp[q] = st4(tpp);
+            pv.pbtp[q] = st4(btp);
+            tpp = tpp * btp;
+            thr = thr * weight;
+            if (!anyPos(thr)) break;
+            if (isT != 0u && isRefractive(mat) && mat.abbeNumber > 0.0f) {
+                // collapse to the hero wavelength (spectral_hero/transport.slang:77-87)
+                misc.y = thr.x;
+                thr = f4(0.0f);
+                misc.z = lam.x;
+                misc.w = 470.0f;
+                misc.x = btp.x;
+                newFlags &= ~kFHeroActive;
+                if (shadowEmit) {
+                    newFlags |= kFPendingCollapse;
+                } else {
+                    F4 rx = ld4(pv.radXYZ[q]);
+                    F3 xyzv = F3{rx.x, rx.y, rx.z} + toXYZ4(ld4(pv.rad[q]), lam, f4(470.0f));
+                    pv.radXYZ[q] = make_float4(xyzv.x, xyzv.y, xyzv.z, rx.w);
+                    pv.rad[q] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                }
+            }
+        } else {
+            Eval e = evalScalar(fp.spec, st, wl);
+            if (e.pdf <= 0.0f) break;
+            F3 weight = e.value * fabsf(wl.z) / e.pdf;
+            if (!(e.pdf > 0.0f && anyPos(weight))) break;
+            if (MODE == kModeRgb) {
+                F3 t = F3{thr.x, thr.y, thr.z} * weight;
+                thr = F4{t.x, t.y, t.z, 0.0f};
+                if (!anyPos(t)) break;
+            } else if (MODE == kModeSingle) {
+                thr.x *= weight.x;
+                if (thr.x <= 0.0f) break;
+            } else {
+                misc.y *= weight.x;
+                if (misc.y <= 0.0f) break;
+            }
+            misc.x = e.pdf;
+        }
+        const bool nowHero = MODE == kModeHero && (newFlags & kFHeroActive) != 0u;
+        if (curNee && isT == 0u) newFlags |= kFPrevNee;
+        if (nowHero)
+            updateMedium4(fp.spec, mat, sf.frontFace, isT, lam, newFlags, sigma);
+        else
+            updateMedium(fp.spec, mat, sf.frontFace, isT, MODE == kModeRgb ? 0.0f : (MODE == kModeSingle ? lam.x : misc.z),
+                         MODE == kModeRgb ? 0u : 1u, newFlags, sigma);
+
+        // Russian roulette (rgb/transport.slang:56-62, spectral_hero/transport.slang:138-152)
+        if (fp.depth + 1u >= sd.rrMinDepth) {
+            float p;
+            if (MODE == kModeRgb)
+                p = clampf(maxc(F3{thr.x, thr.y, thr.z}), kRRMin, kRRMax);
+            else if (MODE == kModeSingle)
+                p = clampf(thr.x, kRRMin, kRRMax);
+            else
+                p = nowHero ? clampf(maxc(thr), kRRMin, kRRMax) : clampf(misc.y, kRRMin, kRRMax);
+            if (rnd(rng) > p) break;
+            if (MODE == kModeRgb) {
+                F3 t = F3{thr.x, thr.y, thr.z} / p;
+                thr = F4{t.x, t.y, t.z, 0.0f};
+            } else if (MODE == kModeSingle) {
+                thr.x /= p;
+            } else if (nowHero) {
+                tpp = tpp * F4{p, p, p, p};
+                thr = thr / p;
+            } else {
+                misc.y /= p;
+            }
+        }
+        // src/shaders/integrator/path/loop.slang:113-120
+        F3 off = isT ? -sf.ng : sf.ng;
+        nextO = sf.p + off * kShadowOriginOffset;
+        nextD = wi;
+        cont = true;
+        pv.rng[q] = storeRng(rng);
+        pv.thr[q] = st4(thr);
+        pv.misc[q] = st4(misc);
+        pv.flags[q] = newFlags;
+        pv.sigma[q] = st4(sigma);
+        if (MODE == kModeHero) pv.tpp[q] = st4(tpp);
+    } while (false);
+
+    // a path that emitted a shadow ray but stopped here still needs its flags
+    // for the deferred radiance update
+    if (shadowEmit && !cont && MODE == kModeHero) {
+        // flags only matter for the pending-collapse case, which implies cont
+    }
+    uint32_t slot = waveAppend(cont, &counters[outSlot]);
+    if (cont) {
+        out.o[slot] = make_float4(nextO.x, nextO.y, nextO.z, __uint_as_float(q));
+        out.d[slot] = make_float4(nextD.x, nextD.y, nextD.z, kRayTMax);
+    }
+    uint32_t sslot = waveAppend(shadowEmit, &counters[kCntShadow]);
+    if (shadowEmit) {
+        sh.o[sslot] = make_float4(shO.x, shO.y, shO.z, __uint_as_float(q));
+        sh.d[sslot] = make_float4(shD.x, shD.y, shD.z, shT);
+        sh.a[sslot] = st4(shA);
+        sh.b[sslot] = st4(shB);
+    }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void kFinalize(uint32_t count, PathView pv, float4* sampleRad) {
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= count) return;
+    F4 rad = ld4(pv.rad[i]);
+    F3 out;
+    if (MODE == kModeRgb) {
+        out = F3{rad.x, rad.y, rad.z};
+    } else if (MODE == kModeSingle) {
+        F4 l = ld4(pv.lam[i]);
+        out = toXYZ1(rad.x, l.x, l.y);
+    } else {
+        F4 rx = ld4(pv.radXYZ[i]);
+        out = F3{rx.x, rx.y, rx.z};
+        if (pv.flags[i] & kFHeroActive) {
+            out = out + toXYZ4(rad, ld4(pv.lam[i]), f4(470.0f));
+        } else {
+            F4 m = ld4(pv.misc[i]);
+            out = out + toXYZ1(rx.w, m.z, m.w);
+        }
+    }
+    sampleRad[i] = make_float4(out.x, out.y, out.z, 0.0f);
+}
+
+// frameSum[k] += radiance of every sample of pixel k inside [pathBegin, pathBegin+count), in sample order
+__global__ __launch_bounds__(kBlock) void kReduce(uint32_t numPixels, uint64_t pathBegin, uint32_t count,
+                                                  const float4* sampleRad, float4* frameSum) {
+    uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= numPixels) return;
+    uint64_t end = pathBegin + count;
+    uint64_t s0 = pathBegin / numPixels;
+    F4 acc = ld4(frameSum[k]);
+    for (uint64_t s = s0;; s++) {
+        uint64_t p = s * numPixels + k;
+        if (p >= end) break;
+        if (p < pathBegin) continue;
+        float4 v = sampleRad[p - pathBegin];
+        acc = F4{acc.x + v.x, acc.y + v.y, acc.z + v.z, 0.0f};
+    }
+    frameSum[k] = st4(acc);
+}
+
+DEV float srgbEncode(float v) {
+    if (v <= 0.0031308f) return 12.92f * v;
+    return 1.055f * powf(v, 1.0f / 2.4f) - 0.055f;
+}
+
+// src/shaders/integrator/path/writeback.slang:9-123, film/tonemap.slang:9-31
+__global__ __launch_bounds__(kBlock) void kAccumulate(SceneData sd, uint32_t width, uint32_t height, uint32_t vw,
+                                                      uint32_t vh, const float4* frameSum, float4* accum,
+                                                      ushort4* output, uint32_t spectralOutput) {
+    uint32_t idx = blockIdx.x * kBlock + threadIdx.x;
+    if (idx >= width * height) return;
+    uint32_t x = idx % width, y = idx / width;
+    // vw, vh: viewport extent clamped to the accumulation image, so k < vw * vh
+    bool inside = vw > 0 && vh > 0 && x >= sd.viewportRect[0] && y >= sd.viewportRect[1] &&
+                  x < sd.viewportRect[0] + vw && y < sd.viewportRect[1] + vh;
+    if (!inside) {
+        accum[idx] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        output[idx] = make_ushort4(0, 0, 0, 0);
+        return;
+    }
+    uint32_t k = (y - sd.viewportRect[1]) * vw + (x - sd.viewportRect[0]);
+    uint32_t spp = sd.samplesPerPixel > 1u ? sd.samplesPerPixel : 1u;
+    F4 fs = ld4(frameSum[k]);
+    F3 frame = F3{fs.x, fs.y, fs.z} / (float)spp;
+    float4 prev = accum[idx];
+    float pw = (float)(uint32_t)(prev.w + 0.5f), cw = (float)spp, tw = pw + cw;
+    F3 acc = tw <= 0.0f ? f3(0.0f) : (F3{prev.x, prev.y, prev.z} * pw + frame * cw) / tw;
+    accum[idx] = make_float4(acc.x, acc.y, acc.z, tw);
+    F3 c = spectralOutput ? xyzToSrgb(acc) : acc;
+    c = maxs(c * sd.exposure, 0.0f);
+    if (VKRT_SCENE_TONE_MAPPING_MODE_FROM_RENDER_SETTINGS(sd.packedRenderSettings) == VKRT_TONE_MAPPING_MODE_ACES)
+        c = (c * (c * 2.51f + f3(0.03f))) / (c * (c * 2.43f + f3(0.59f)) + f3(0.14f));
+    c = sat(c);
+    float e[3] = {srgbEncode(c.x), srgbEncode(c.y), srgbEncode(c.z)};
+    unsigned short o[3];
+    for (int j = 0; j < 3; j++) o[j] = (unsigned short)floorf(sat(e[j]) * 65535.0f + 0.5f);
+    output[idx] = make_ushort4(o[0], o[1], o[2], 65535);
+}
+
+__global__ void kFill4(float4* p, size_t n, float4 v) {
+    size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ __launch_bounds__(kTraceBlock) void kTraceQuery(SceneView sv, const float4* rays, uint32_t n,
+                                                          float4* out, uint32_t* gstack, uint32_t anyHit) {
+    __shared__ uint32_t lds[kLdsStack * kTraceBlock];
+    TraversalStack st;
+    st.lds = lds + threadIdx.x;
+    st.glob = gstack + (size_t)(blockIdx.x * kTraceBlock + threadIdx.x) * kGlobalStack;
+    st.dropped = 0;
+    for (uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x; i < n; i += gridDim.x * kTraceBlock) {
+        float4 a = rays[2 * i], b = rays[2 * i + 1];
+        TraversalRay r = makeTraversalRay(xyz(a), xyz(b));
+        if (anyHit) {
+            uint32_t vis = classifyOccluder(sv, traceAny(sv, r, a.w, b.w, st));
+            out[i] = make_float4(__uint_as_float(vis), 0.0f, 0.0f, 0.0f);
+        } else {
+            Hit h = traceClosest(sv, r, a.w, b.w, st);
+            out[i] = make_float4(h.t, __uint_as_float(h.tri), h.u, h.v);
+        }
+    }
+}
+
+__global__ __launch_bounds__(kTraceBlock) void kPrimary(SceneView sv, SceneData sd, uint32_t rngPolicy,
+                                                       uint64_t rngSeed, uint32_t sampleIndex, uint32_t* outMesh,
+                                                       uint32_t* outPrim, uint32_t* gstack) {
+    __shared__ uint32_t lds[kLdsStack * kTraceBlock];
+    TraversalStack st;
+    st.lds = lds + threadIdx.x;
+    st.glob = gstack + (size_t)(blockIdx.x * kTraceBlock + threadIdx.x) * kGlobalStack;
+    st.dropped = 0;
+    uint32_t vw = sd.viewportRect[2], vh = sd.viewportRect[3];
+    uint32_t n = vw * vh;
+    for (uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x; i < n; i += gridDim.x * kTraceBlock) {
+        int px = (int)(sd.viewportRect[0] + i % vw), py = (int)(sd.viewportRect[1] + i / vw);
+        Rng rng = rngInit(rngPolicy, rngSeed, px, py, sd.frameNumber, sampleIndex);
+        float jx = rnd(rng);
+        float jy = rnd(rng);
+        F3 o, d;
+        primaryRay(sd, px, py, jx - 0.5f, jy - 0.5f, o, d);
+        TraversalRay r = makeTraversalRay(o, d);
+        Hit h = traceClosest(sv, r, kRayTMin, kRayTMax, st);
+        if (h.tri == kNoHit) {
+            outMesh[i] = VKRT_INVALID_INDEX;
+            outPrim[i] = VKRT_INVALID_INDEX;
+        } else {
+            uint32_t mesh = sv.triMesh[h.tri];
+            outMesh[i] = mesh;
+            outPrim[i] = h.tri - sv.meshTriBase[mesh];
+        }
+    }
+}
+
+// ------------------------------------------------------------- host side --
+static uint32_t blocksFor(uint64_t n) { return (uint32_t)((n + kBlock - 1) / kBlock); }
+
+bool ensureTraversalScratch(mrt_context* ctx) {
+    if (!ctx->counters.reserve(64)) return false;
+    if (ctx->traceGrid == 0) {
+        // persistent traversal grid: enough resident blocks to fill every CU
+        ctx->traceGrid = (uint32_t)std::max(1, ctx->cuCount) * 8u;
+    }
+    size_t need = (size_t)ctx->traceGrid * kTraceBlock * kGlobalStack;
+    return ctx->traversalStack.reserve(need);
+}
+
+static bool ensurePaths(mrt_context* ctx, size_t n, int mode) {
+    PathBuffers& b = ctx->paths;
+    bool ok = b.rng.reserve(n) && b.flags.reserve(n) && b.thr.reserve(n) && b.rad.reserve(n) && b.misc.reserve(n) &&
+              b.sigma.reserve(n) && b.hits.reserve(n) && b.sampleRad.reserve(n);
+    for (int qi = 0; qi < 2; qi++)
+        for (int j = 0; j < 2; j++) ok = ok && b.rays[qi][j].reserve(n);
+    for (int j = 0; j < 4; j++) ok = ok && b.shadow[j].reserve(n);
+    if (mode != kModeRgb) ok = ok && b.lam.reserve(n);
+    if (mode == kModeHero)
+        ok = ok && b.radXYZ.reserve(n) && b.tpp.reserve(n) && b.ptpp.reserve(n) && b.pbtp.reserve(n);
+    return ok;
+}
+
+static PathView pathView(mrt_context* ctx) {
+    PathBuffers& b = ctx->paths;
+    PathView v;
+    v.rng = b.rng.ptr;
+    v.flags = b.flags.ptr;
+    v.thr = b.thr.ptr;
+    v.rad = b.rad.ptr;
+    v.radXYZ = b.radXYZ.ptr;
+    v.tpp = b.tpp.ptr;
+    v.ptpp = b.ptpp.ptr;
+    v.pbtp = b.pbtp.ptr;
+    v.lam = b.lam.ptr;
+    v.misc = b.misc.ptr;
+    v.sigma = b.sigma.ptr;
+    return v;
+}
+
+template <int MODE>
+static bool launchChunk(mrt_context* ctx, FrameParams fp, std::vector<hipEvent_t>& extEvents, size_t& evUsed) {
+    hipStream_t s = ctx->stream;
+    PathBuffers& b = ctx->paths;
+    PathView pv = pathView(ctx);
+    SceneView sv = ctx->view();
+    uint32_t* cnt = ctx->counters.ptr;
+    QueueView qv[2];
+    for (int qi = 0; qi < 2; qi++) {
+        qv[qi].o = b.rays[qi][0].ptr;
+        qv[qi].d = b.rays[qi][1].ptr;
+        qv[qi].count = cnt + qi;
+    }
+    ShadowView shv{b.shadow[0].ptr, b.shadow[1].ptr, b.shadow[2].ptr, b.shadow[3].ptr, cnt + kCntShadow};
+    hipLaunchKernelGGL(kGenerate<MODE>, dim3(blocksFor(fp.count)), dim3(kBlock), 0, s, fp, pv, qv[0], ctx->accum.ptr);
+    hipLaunchKernelGGL(kSetCount, dim3(1), dim3(1), 0, s, cnt, kCntQueue0, fp.count);
+    int cur = 0;
+    uint32_t maxDepth = fp.sd.rrMaxDepth;
+    for (uint32_t depth = 0; depth < maxDepth; depth++) {
+        int nxt = 1 - cur;
+        fp.depth = depth;
+        hipLaunchKernelGGL(kPrepDepth, dim3(1), dim3(1), 0, s, cnt, cur, nxt);
+        bool timed = evUsed + 2 <= extEvents.size();
+        if (timed) hipEventRecord(extEvents[evUsed], s);
+        hipLaunchKernelGGL(kExtend, dim3(ctx->traceGrid), dim3(kTraceBlock), 0, s, sv, qv[cur].o, qv[cur].d,
+                           b.hits.ptr, cnt, cur, ctx->traversalStack.ptr);
+        if (timed) {
+            hipEventRecord(extEvents[evUsed + 1], s);
+            evUsed += 2;
+        }
+        hipLaunchKernelGGL(kShade<MODE>, dim3(blocksFor(fp.count)), dim3(kBlock), 0, s, fp, sv, pv, qv[cur], b.hits.ptr,
+                           qv[nxt], shv, cnt, cur, nxt);
+        hipLaunchKernelGGL(kPrepShadow, dim3(1), dim3(1), 0, s, cnt);
+        hipLaunchKernelGGL(kShadow<MODE>, dim3(ctx->traceGrid), dim3(kTraceBlock), 0, s, sv, pv, shv, cnt,
+                           ctx->traversalStack.ptr);
+        cur = nxt;
+    }
+    hipLaunchKernelGGL(kFinalize<MODE>, dim3(blocksFor(fp.count)), dim3(kBlock), 0, s, fp.count, pv, b.sampleRad.ptr);
+    hipLaunchKernelGGL(kReduce, dim3(blocksFor(fp.numPixels)), dim3(kBlock), 0, s, fp.numPixels, fp.pathBegin, fp.count,
+                       b.sampleRad.ptr, ctx->frameSum.ptr);
+    return check(hipGetLastError(), "kernel launch");
+}
+
+}  // namespace mrt
+
+using namespace mrt;
+
+extern "C" int mrt_resize_accumulation(mrt_context* ctx, uint32_t width, uint32_t height) {
+    if (!ctx || width == 0 || height == 0) {
+        setError("invalid accumulation extent");
+        return -1;
+    }
+    if (!check(hipSetDevice(ctx->device), "hipSetDevice")) return -2;
+    if (ctx->width == width && ctx->height == height && ctx->accum.ptr) return 0;
+    size_t n = (size_t)width * height;
+    if (!ctx->accum.reserve(n) || !ctx->output.reserve(n)) return -3;
+    ctx->accum.count = n;
+    ctx->output.count = n;
+    ctx->width = width;
+    ctx->height = height;
+    return mrt_clear_accumulation(ctx);
+}
+
+extern "C" int mrt_clear_accumulation(mrt_context* ctx) {
+    if (!ctx || !ctx->accum.ptr) return 0;
+    size_t n = (size_t)ctx->width * ctx->height;
+    hipLaunchKernelGGL(kFill4, dim3(blocksFor(n)), dim3(kBlock), 0, ctx->stream, ctx->accum.ptr, n,
+                       make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+    return check(hipGetLastError(), "clear accumulation") ? 0 : -2;
+}
+
+extern "C" int mrt_render_frame(mrt_context* ctx, const SceneData* scene, const mrt_render_params* prm,
+                                mrt_render_stats* stats) {
+    if (!ctx || !scene || !prm) {
+        setError("mrt_render_frame: null argument");
+        return -1;
+    }
+    if (!check(hipSetDevice(ctx->device), "hipSetDevice")) return -2;
+    if (mrt_resize_accumulation(ctx, prm->width, prm->height) != 0) return -2;
+    if (!ctx->rgb2spec.ptr) {
+        setError("rgb2spec table not uploaded");
+        return -2;
+    }
+    if (ctx->triCount > 0 && ctx->bvhMaxDepth * 3u + 1u > (uint32_t)(kLdsStack + kGlobalStack)) {
+        setError("BVH deeper than the traversal stack supports");
+        return -2;
+    }
+    auto wall0 = std::chrono::steady_clock::now();
+    hipStream_t s = ctx->stream;
+    uint32_t vx = scene->viewportRect[0], vy = scene->viewportRect[1];
+    uint32_t vw = scene->viewportRect[2], vh = scene->viewportRect[3];
+    if (vx + vw > prm->width) vw = prm->width > vx ? prm->width - vx : 0;
+    if (vy + vh > prm->height) vh = prm->height > vy ? prm->height - vy : 0;
+    uint32_t renderMode = VKRT_SCENE_RENDER_MODE_FROM_RENDER_SETTINGS(scene->packedRenderSettings);
+    uint32_t specMode = VKRT_SCENE_SPECTRAL_SAMPLING_MODE_FROM_RENDER_SETTINGS(scene->packedRenderSettings);
+    int mode = renderMode == VKRT_RENDER_MODE_SPECTRAL ? (specMode == VKRT_SPECTRAL_SAMPLING_MODE_HERO ? kModeHero
+                                                                                                      : kModeSingle)
+                                                       : kModeRgb;
+    uint32_t spp = scene->samplesPerPixel > 1u ? scene->samplesPerPixel : 1u;
+    uint64_t numPixels = (uint64_t)vw * vh;
+    if (prm->resetAccumulation && mrt_clear_accumulation(ctx) != 0) return -2;
+    if (!ensureTraversalScratch(ctx)) return -3;
+    if (!ctx->frameSum.reserve(numPixels ? numPixels : 1)) return -3;
+    hipLaunchKernelGGL(kFill4, dim3(blocksFor(numPixels ? numPixels : 1)), dim3(kBlock), 0, s, ctx->frameSum.ptr,
+                       (size_t)numPixels, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+    hipMemsetAsync(ctx->counters.ptr, 0, 64 * sizeof(uint32_t), s);
+
+    uint64_t totalPaths = numPixels * spp;
+    uint64_t chunk = std::min<uint64_t>(totalPaths, kMaxChunk);
+    if (numPixels > 0 && numPixels <= kMaxChunk) chunk = std::min<uint64_t>(totalPaths, (kMaxChunk / numPixels) * numPixels);
+    if (chunk > 0 && !ensurePaths(ctx, chunk, mode)) return -3;
+
+    FrameParams fp;
+    memset(&fp, 0, sizeof(fp));
+    fp.sd = *scene;
+    fp.numPixels = (uint32_t)numPixels;
+    fp.vpW = vw;
+    fp.accumWidth = prm->width;
+    fp.rngPolicy = prm->rngPolicy;
+    fp.rngSeed = prm->rngSeed;
+    fp.neeEnabled = scene->misNeeEnabled != 0u && scene->emissiveMeshCount > 0u;
+    fp.bsdfOnly = scene->debugMode == VKRT_DEBUG_MODE_BSDF_ONLY;
+    fp.neeOnly = scene->debugMode == VKRT_DEBUG_MODE_NEE_ONLY;
+    fp.traceShadowAlways = prm->hasTransmissive ? 1u : 0u;
+    fp.spec.table = ctx->rgb2spec.ptr;
+    fp.spec.res = scene->rgb2specSRGB.res;
+    fp.spec.scaleOffset = scene->rgb2specSRGB.scaleOffset;
+    fp.spec.dataOffset = scene->rgb2specSRGB.dataOffset;
+
+    static std::vector<hipEvent_t> extEvents;
+    if (extEvents.empty()) {
+        extEvents.resize(2048);
+        for (auto& e : extEvents) hipEventCreate(&e);
+    }
+    size_t evUsed = 0;
+    hipEventRecord(ctx->evBegin, s);
+    uint32_t chunks = 0;
+    for (uint64_t begin = 0; begin < totalPaths; begin += chunk) {
+        fp.pathBegin = begin;
+        fp.count = (uint32_t)std::min<uint64_t>(chunk, totalPaths - begin);
+        bool ok = mode == kModeRgb    ? launchChunk<kModeRgb>(ctx, fp, extEvents, evUsed)
+                  : mode == kModeHero ? launchChunk<kModeHero>(ctx, fp, extEvents, evUsed)
+                                      : launchChunk<kModeSingle>(ctx, fp, extEvents, evUsed);
+        if (!ok) return -4;
+        chunks++;
+    }
+    size_t n = (size_t)prm->width * prm->height;
+    hipLaunchKernelGGL(kAccumulate, dim3(blocksFor(n)), dim3(kBlock), 0, s, fp.sd, prm->width, prm->height, vw, vh,
+                       ctx->frameSum.ptr, ctx->accum.ptr, ctx->output.ptr, renderMode == VKRT_RENDER_MODE_SPECTRAL ? 1u : 0u);
+    hipEventRecord(ctx->evEnd, s);
+    if (!check(hipEventSynchronize(ctx->evEnd), "frame")) return -4;
+    uint32_t hostCounters[64];
+    if (!check(hipMemcpy(hostCounters, ctx->counters.ptr, sizeof(hostCounters), hipMemcpyDeviceToHost), "counters"))
+        return -4;
+    if (hostCounters[kCntDropped]) {
+        setError("traversal stack overflow");
+        return -5;
+    }
+    if (stats) {
+        unsigned long long st64[2];
+        memcpy(st64, hostCounters + kCntStats, sizeof(st64));
+        stats->extensionRays = st64[0];
+        stats->shadowRays = st64[1];
+        stats->paths = totalPaths;
+        float ms = 0.0f;
+        hipEventElapsedTime(&ms, ctx->evBegin, ctx->evEnd);
+        stats->traceMs = ms;
+        float ext = 0.0f;
+        for (size_t e = 0; e + 1 < evUsed; e += 2) {
+            float t = 0.0f;
+            hipEventElapsedTime(&t, extEvents[e], extEvents[e + 1]);
+            ext += t;
+        }
+        stats->extendMs = ext;
+        stats->extendLaunches = (uint32_t)(evUsed / 2);
+        (void)wall0;
+    }
+    return 0;
+}
+
+extern "C" int mrt_read_accumulation(mrt_context* ctx, float* out, size_t floatCount) {
+    if (!ctx || !out || floatCount != (size_t)ctx->width * ctx->height * 4u) {
+        setError("mrt_read_accumulation: size mismatch");
+        return -1;
+    }
+    hipStreamSynchronize(ctx->stream);
+    return check(hipMemcpy(out, ctx->accum.ptr, floatCount * sizeof(float), hipMemcpyDeviceToHost), "read accum") ? 0 : -2;
+}
+
+extern "C" int mrt_read_output(mrt_context* ctx, uint16_t* out, size_t count) {
+    if (!ctx || !out || count != (size_t)ctx->width * ctx->height * 4u) {
+        setError("mrt_read_output: size mismatch");
+        return -1;
+    }
+    hipStreamSynchronize(ctx->stream);
+    return check(hipMemcpy(out, ctx->output.ptr, count * sizeof(uint16_t), hipMemcpyDeviceToHost), "read output") ? 0 : -2;
+}
+
+static uint32_t hostBits(float f) {
+    uint32_t u;
+    memcpy(&u, &f, sizeof(u));
+    return u;
+}
+
+static int runQuery(mrt_context* ctx, const float* rays, size_t count, uint32_t anyHit, std::vector<float4>& result) {
+    if (!check(hipSetDevice(ctx->device), "hipSetDevice")) return -2;
+    if (!ensureTraversalScratch(ctx)) return -3;
+    if (count > 0xFFFFFFFFull / 2) {
+        setError("too many rays");
+        return -1;
+    }
+    float4* dRays = nullptr;
+    float4* dOut = nullptr;
+    if (!check(hipMalloc(&dRays, count * 2 * sizeof(float4)), "hipMalloc rays")) return -3;
+    if (!check(hipMalloc(&dOut, count * sizeof(float4)), "hipMalloc out")) {
+        hipFree(dRays);
+        return -3;
+    }
+    int rc = 0;
+    if (check(hipMemcpy(dRays, rays, count * 8 * sizeof(float), hipMemcpyHostToDevice), "upload rays")) {
+        hipLaunchKernelGGL(kTraceQuery, dim3(ctx->traceGrid), dim3(kTraceBlock), 0, ctx->stream, ctx->view(), dRays,
+                           (uint32_t)count, dOut, ctx->traversalStack.ptr, anyHit);
+        result.resize(count);
+        if (!check(hipStreamSynchronize(ctx->stream), "trace query") ||
+            !check(hipMemcpy(result.data(), dOut, count * sizeof(float4), hipMemcpyDeviceToHost), "read hits"))
+            rc = -4;
+    } else {
+        rc = -4;
+    }
+    hipFree(dRays);
+    hipFree(dOut);
+    return rc;
+}
+
+extern "C" int mrt_trace_rays(mrt_context* ctx, const float* rays, size_t count, uint32_t* outMesh, uint32_t* outPrim,
+                              float* outT, float* outBary) {
+    if (!ctx || !rays || !outMesh || !outPrim) return -1;
+    std::vector<float4> res;
+    int rc = runQuery(ctx, rays, count, 0u, res);
+    if (rc) return rc;
+    std::vector<uint32_t> triMesh(ctx->triMesh.count), triBase(ctx->meshTriBase.count);
+    if (!triMesh.empty()) hipMemcpy(triMesh.data(), ctx->triMesh.ptr, triMesh.size() * 4, hipMemcpyDeviceToHost);
+    if (!triBase.empty()) hipMemcpy(triBase.data(), ctx->meshTriBase.ptr, triBase.size() * 4, hipMemcpyDeviceToHost);
+    for (size_t i = 0; i < count; i++) {
+        uint32_t tri = hostBits(res[i].y);
+        if (tri == kNoHit || tri >= triMesh.size()) {
+            outMesh[i] = VKRT_INVALID_INDEX;
+            outPrim[i] = VKRT_INVALID_INDEX;
+        } else {
+            outMesh[i] = triMesh[tri];
+            outPrim[i] = tri - triBase[triMesh[tri]];
+        }
+        if (outT) outT[i] = tri == kNoHit ? 0.0f : res[i].x;
+        if (outBary) {
+            outBary[2 * i] = tri == kNoHit ? 0.0f : res[i].z;
+            outBary[2 * i + 1] = tri == kNoHit ? 0.0f : res[i].w;
+        }
+    }
+    return 0;
+}
+
+extern "C" int mrt_trace_shadow_rays(mrt_context* ctx, const float* rays, size_t count, uint32_t* outVisibility) {
+    if (!ctx || !rays || !outVisibility) return -1;
+    std::vector<float4> res;
+    int rc = runQuery(ctx, rays, count, 1u, res);
+    if (rc) return rc;
+    for (size_t i = 0; i < count; i++) outVisibility[i] = hostBits(res[i].x);
+    return 0;
+}
+
+extern "C" int mrt_primary_hits(mrt_context* ctx, const SceneData* scene, const mrt_render_params* prm,
+                                uint32_t sampleIndex, uint32_t* outMesh, uint32_t* outPrim) {
+    if (!ctx || !scene || !prm || !outMesh || !outPrim) return -1;
+    if (!check(hipSetDevice(ctx->device), "hipSetDevice")) return -2;
+    if (!ensureTraversalScratch(ctx)) return -3;
+    size_t n = (size_t)scene->viewportRect[2] * scene->viewportRect[3];
+    if (n == 0) return 0;
+    uint32_t *dMesh = nullptr, *dPrim = nullptr;
+    if (!check(hipMalloc(&dMesh, n * 4), "hipMalloc") || !check(hipMalloc(&dPrim, n * 4), "hipMalloc")) {
+        if (dMesh) hipFree(dMesh);
+        return -3;
+    }
+    hipLaunchKernelGGL(kPrimary, dim3(ctx->traceGrid), dim3(kTraceBlock), 0, ctx->stream, ctx->view(), *scene,
+                       prm->rngPolicy, prm->rngSeed, sampleIndex, dMesh, dPrim, ctx->traversalStack.ptr);
+    int rc = 0;
+    if (!check(hipStreamSynchronize(ctx->stream), "primary hits") ||
+        !check(hipMemcpy(outMesh, dMesh, n * 4, hipMemcpyDeviceToHost), "read") ||
+        !check(hipMemcpy(outPrim, dPrim, n * 4, hipMemcpyDeviceToHost), "read"))
+        rc = -4;
+    hipFree(dMesh);
+    hipFree(dPrim);
+    return rc;
+}

@@ -1,0 +1,398 @@
+/*
+ * Lifecycle, frame loop and render sessions.
+ *
+ * The frame loop keeps the reference's contract (src/core/api/frame.c:74-420):
+ * VKRT_draw = beginFrame, updateScene, trace, present, endFrame. Without a
+ * window every frame is "offscreen" and is traced only while a render session
+ * is sampling (frame.c:107-113); endFrame advances frameNumber and
+ * totalSamples only when the trace contributed (frame.c:370-402) and stops
+ * sampling once the target is reached.
+ *
+ * VKRT_updateScene re-packs dirty host data and uploads it; a geometry change
+ * rebuilds the BVH on the GPU (the reference's BLAS/TLAS builds,
+ * src/core/render/accel/blas.c:222-262, tlas.c:535-559). VKRT_trace launches
+ * the wavefront path tracer in place of vkCmdTraceRaysKHR
+ * (src/core/runtime/command/record.c:448-486).
+ */
+#include "internal.h"
+
+#include <stdlib.h>
+#include <string.h>
+
+void VKRT_defaultCreateInfo(VKRT_CreateInfo* createInfo) {
+    if (!createInfo) return;
+    memset(createInfo, 0, sizeof(*createInfo));
+    createInfo->width = VKRT_DEFAULT_WIDTH;
+    createInfo->height = VKRT_DEFAULT_HEIGHT;
+    createInfo->title = "vkrt";
+    createInfo->headless = 1;
+    createInfo->preferredDeviceIndex = -1;
+}
+
+VKRT_Result VKRT_create(VKRT** outVkrt) {
+    if (!outVkrt) return VKRT_ERROR_INVALID_ARGUMENT;
+    VKRT* vkrt = (VKRT*)calloc(1, sizeof(VKRT));
+    if (!vkrt) return VKRT_ERROR_OUT_OF_MEMORY;
+    vkrtInitDefaultSettings(vkrt, VKRT_DEFAULT_WIDTH, VKRT_DEFAULT_HEIGHT);
+    vkrt->sceneData.exposure = 1.0f;
+    vkrtWriteSceneStateUniform(vkrt);
+    vkrtSyncCameraMatrices(vkrt);
+    vkrt->accumulationNeedsReset = 1;
+    vkrt->rngPolicy = VKRT_RNG_HASH;
+    vkrt->renderStatus.renderPhase = VKRT_RENDER_PHASE_INACTIVE;
+    *outVkrt = vkrt;
+    return VKRT_SUCCESS;
+}
+
+void VKRT_deinit(VKRT* vkrt) {
+    if (!vkrt) return;
+    if (vkrt->device) {
+        mrt_destroy(vkrt->device);
+        vkrt->device = NULL;
+    }
+}
+
+void VKRT_destroy(VKRT* vkrt) {
+    if (!vkrt) return;
+    VKRT_deinit(vkrt);
+    for (uint32_t i = 0; i < vkrt->meshCount; i++) {
+        free(vkrt->meshes[i].vertices);
+        free(vkrt->meshes[i].indices);
+    }
+    free(vkrt->meshes);
+    free(vkrt->materials);
+    for (uint32_t i = 0; i < vkrt->textureCount; i++) free(vkrt->textures[i].rgba);
+    free(vkrt->textures);
+    vkrtFreePackedScene(&vkrt->packed);
+    free(vkrt->rgb2spec);
+    free(vkrt);
+}
+
+VKRT_Result VKRT_initWithCreateInfo(VKRT* vkrt, const VKRT_CreateInfo* createInfo) {
+    if (!vkrt) return VKRT_ERROR_INVALID_ARGUMENT;
+    VKRT_CreateInfo info;
+    if (createInfo)
+        info = *createInfo;
+    else
+        VKRT_defaultCreateInfo(&info);
+    uint32_t w = info.width ? info.width : VKRT_DEFAULT_WIDTH;
+    uint32_t h = info.height ? info.height : VKRT_DEFAULT_HEIGHT;
+    vkrt->headless = 1;
+    vkrt->renderWidth = w;
+    vkrt->renderHeight = h;
+    VKRT_setRenderViewport(vkrt, 0, 0, w, h);
+    VKRT_Result r = vkrtLoadRGB2SpecTable(vkrt);
+    if (r != VKRT_SUCCESS) return r;
+    if (vkrt->device) return VKRT_SUCCESS;
+    int device = info.preferredDeviceIndex >= 0 ? info.preferredDeviceIndex : 0;
+    if (mrt_create(device, &vkrt->device) != 0) {
+        VKRT_LOG_ERROR("GPU initialisation failed: %s", mrt_last_error());
+        vkrt->device = NULL;
+        return VKRT_ERROR_INITIALIZATION_FAILED;
+    }
+    uint32_t vendor = 0, driver = 0;
+    mrt_device_name(vkrt->device, vkrt->systemInfo.deviceName, sizeof(vkrt->systemInfo.deviceName), &vendor, &driver);
+    vkrt->systemInfo.vendorID = vendor;
+    vkrt->systemInfo.driverVersion = driver;
+    if (mrt_upload_rgb2spec(vkrt->device, vkrt->rgb2spec, vkrt->rgb2specFloatCount) != 0) {
+        VKRT_LOG_ERROR("%s", mrt_last_error());
+        return VKRT_ERROR_INITIALIZATION_FAILED;
+    }
+    /* force a full upload on the first frame */
+    vkrt->geometryUploaded = vkrt->geometryRevision - 1u;
+    vkrt->materialUploaded = vkrt->materialRevision - 1u;
+    vkrt->environmentUploaded = vkrt->environmentRevision - 1u;
+    vkrtResetSceneData(vkrt);
+    return VKRT_SUCCESS;
+}
+
+VKRT_Result VKRT_init(VKRT* vkrt) { return VKRT_initWithCreateInfo(vkrt, NULL); }
+
+int VKRT_shouldDeinit(VKRT* vkrt) { return vkrt ? vkrt->shouldDeinit : 1; }
+
+void VKRT_poll(VKRT* vkrt) { (void)vkrt; }
+
+VKRT_Result VKRT_getSystemInfo(const VKRT* vkrt, VKRT_SystemInfo* out) {
+    if (!vkrt || !out) return VKRT_ERROR_INVALID_ARGUMENT;
+    *out = vkrt->systemInfo;
+    return VKRT_SUCCESS;
+}
+
+static VKRT_Result deviceError(void) {
+    VKRT_LOG_ERROR("%s", mrt_last_error());
+    return VKRT_ERROR_OPERATION_FAILED;
+}
+
+/* Re-packs and uploads whatever changed since the last upload. */
+static VKRT_Result syncScene(VKRT* vkrt, int requireDevice) {
+    int geometryDirty = vkrt->geometryUploaded != vkrt->geometryRevision;
+    int materialDirty = vkrt->materialUploaded != vkrt->materialRevision;
+    int envDirty = vkrt->environmentUploaded != vkrt->environmentRevision;
+    if (!geometryDirty && !materialDirty && !envDirty && vkrt->packed.meshInfos) return VKRT_SUCCESS;
+    VKRT_Result r = vkrtPrepareScene(vkrt);
+    if (r != VKRT_SUCCESS) return r;
+    if (!vkrt->device) {
+        if (requireDevice) return VKRT_ERROR_INITIALIZATION_FAILED;
+        return VKRT_SUCCESS;
+    }
+    PackedScene* p = &vkrt->packed;
+    if (geometryDirty) {
+        mrt_geometry_desc g = {p->vertices, p->vertexCount, p->indices, p->indexCount, p->worldTriangles,
+                               p->triangleMesh, p->triangleCount, p->meshTriangleBase, vkrt->meshCount};
+        if (mrt_upload_geometry(vkrt->device, &g) != 0) return deviceError();
+        mrt_bvh_stats bs;
+        memset(&bs, 0, sizeof(bs));
+        if (mrt_build_bvh(vkrt->device, &bs) != 0) return deviceError();
+        vkrt->stats.bvhBuildMs = bs.buildMs;
+        vkrt->stats.bvhNodeCount = bs.nodeCount;
+        vkrt->stats.bvhLeafTriangles = bs.leafTriangles;
+        vkrt->stats.bvhMaxDepth = bs.maxDepth;
+        vkrt->stats.triangleCount = p->triangleCount;
+    }
+    if (mrt_upload_materials(vkrt->device, p->materials, p->materialCount) != 0) return deviceError();
+    if (mrt_upload_meshes(vkrt->device, p->meshInfos, vkrt->meshCount) != 0) return deviceError();
+    mrt_light_desc l = {p->emissiveMeshes,    p->emissiveMeshCount, p->emissiveTriangles, p->emissiveTriangleCount,
+                        p->meshAliasQ,        p->meshAliasIdx,      p->triAliasQ,         p->triAliasIdx};
+    if (mrt_upload_lights(vkrt->device, &l) != 0) return deviceError();
+    if (envDirty) {
+        mrt_environment_desc e;
+        memset(&e, 0, sizeof(e));
+        uint32_t ti = vkrt->settings.environmentTextureIndex;
+        if (ti != VKRT_INVALID_INDEX && ti < vkrt->textureCount) {
+            e.pixels = vkrt->textures[ti].rgba;
+            e.width = vkrt->textures[ti].width;
+            e.height = vkrt->textures[ti].height;
+            e.aliasQ = p->envAliasQ;
+            e.aliasIdx = p->envAliasIdx;
+            e.pdf = p->envPdf;
+        }
+        if (mrt_upload_environment(vkrt->device, &e) != 0) return deviceError();
+    }
+    vkrt->geometryUploaded = vkrt->geometryRevision;
+    vkrt->materialUploaded = vkrt->materialRevision;
+    vkrt->environmentUploaded = vkrt->environmentRevision;
+    return VKRT_SUCCESS;
+}
+
+VKRT_Result VKRT_beginFrame(VKRT* vkrt) {
+    if (!vkrt) return VKRT_ERROR_INVALID_ARGUMENT;
+    vkrt->frameTraced = 0;
+    return VKRT_SUCCESS;
+}
+
+static int sessionSampling(const VKRT* vkrt) { return vkrt->renderStatus.renderPhase == VKRT_RENDER_PHASE_SAMPLING; }
+
+VKRT_Result VKRT_updateScene(VKRT* vkrt) {
+    if (!vkrt) return VKRT_ERROR_INVALID_ARGUMENT;
+    if (!sessionSampling(vkrt)) return VKRT_SUCCESS;
+    if (!vkrt->device) return VKRT_ERROR_DEVICE_LOST;
+    return syncScene(vkrt, 1);
+}
+
+static void fillRenderParams(const VKRT* vkrt, mrt_render_params* rp) {
+    memset(rp, 0, sizeof(*rp));
+    rp->width = vkrt->renderWidth;
+    rp->height = vkrt->renderHeight;
+    rp->rngPolicy = vkrt->rngPolicy;
+    rp->rngSeed = vkrt->rngSeed;
+    rp->envImportance = vkrt->envImportance;
+    rp->hasTransmissive = vkrt->packed.hasTransmissive;
+}
+
+VKRT_Result VKRT_trace(VKRT* vkrt) {
+    if (!vkrt) return VKRT_ERROR_INVALID_ARGUMENT;
+    if (!sessionSampling(vkrt)) return VKRT_SUCCESS;
+    if (!vkrt->device) return VKRT_ERROR_DEVICE_LOST;
+    mrt_render_params rp;
+    fillRenderParams(vkrt, &rp);
+    if (mrt_resize_accumulation(vkrt->device, vkrt->renderWidth, vkrt->renderHeight) != 0) return deviceError();
+    if (vkrt->accumulationNeedsReset) {
+        vkrt->renderStatus.accumulationFrame = 0;
+        vkrt->renderStatus.totalSamples = 0;
+        rp.resetAccumulation = 1;
+        vkrt->accumulationNeedsReset = 0;
+    }
+    mrt_render_stats rs;
+    memset(&rs, 0, sizeof(rs));
+    if (mrt_render_frame(vkrt->device, &vkrt->sceneData, &rp, &rs) != 0) return deviceError();
+    vkrt->stats.extensionRays = rs.extensionRays;
+    vkrt->stats.shadowRays = rs.shadowRays;
+    vkrt->stats.paths = rs.paths;
+    vkrt->stats.traceMs = rs.traceMs;
+    vkrt->stats.extendMs = rs.extendMs;
+    vkrt->stats.shadowMs = rs.shadowMs;
+    vkrt->stats.shadeMs = rs.shadeMs;
+    vkrt->stats.extendLaunches = rs.extendLaunches;
+    vkrt->renderStatus.renderTimeMs = rs.traceMs;
+    vkrt->frameTraced = 1;
+    return VKRT_SUCCESS;
+}
+
+VKRT_Result VKRT_present(VKRT* vkrt) { return vkrt ? VKRT_SUCCESS : VKRT_ERROR_INVALID_ARGUMENT; }
+
+VKRT_Result VKRT_endFrame(VKRT* vkrt) {
+    if (!vkrt) return VKRT_ERROR_INVALID_ARGUMENT;
+    if (vkrt->frameTraced && sessionSampling(vkrt)) {
+        uint32_t spp = vkrt->sceneData.samplesPerPixel ? vkrt->sceneData.samplesPerPixel : 1u;
+        vkrt->renderStatus.accumulationFrame++;
+        vkrt->renderStatus.totalSamples += spp;
+        vkrt->sceneData.frameNumber++;
+        uint32_t slot = vkrt->renderStatus.accumulationFrame % VKRT_FRAMETIME_HISTORY_SIZE;
+        vkrt->renderStatus.frametimes[slot] = vkrt->renderStatus.renderTimeMs;
+        if (vkrt->renderStatus.renderTargetSamples > 0 &&
+            vkrt->renderStatus.totalSamples >= vkrt->renderStatus.renderTargetSamples) {
+            vkrt->renderStatus.renderPhase = VKRT_RENDER_PHASE_COMPLETE_RAW;
+        }
+    }
+    vkrt->frameTraced = 0;
+    return VKRT_SUCCESS;
+}
+
+VKRT_Result VKRT_draw(VKRT* vkrt) {
+    if (!vkrt) return VKRT_ERROR_INVALID_ARGUMENT;
+    VKRT_Result r = VKRT_beginFrame(vkrt);
+    if (r == VKRT_SUCCESS) r = VKRT_updateScene(vkrt);
+    if (r == VKRT_SUCCESS) r = VKRT_trace(vkrt);
+    if (r == VKRT_SUCCESS) r = VKRT_present(vkrt);
+    if (r == VKRT_SUCCESS) r = VKRT_endFrame(vkrt);
+    return r;
+}
+
+/* src/core/api/render.c:248-265 */
+VKRT_Result VKRT_startRender(VKRT* vkrt, uint32_t width, uint32_t height, uint32_t targetSamples) {
+    if (!vkrt || width == 0 || height == 0) return VKRT_ERROR_INVALID_ARGUMENT;
+    if (width > 16384) width = 16384;
+    if (height > 16384) height = 16384;
+    vkrt->renderWidth = width;
+    vkrt->renderHeight = height;
+    vkrt->renderStatus.renderPhase = VKRT_RENDER_PHASE_SAMPLING;
+    vkrt->renderStatus.renderTargetSamples = targetSamples;
+    return VKRT_setRenderViewport(vkrt, 0, 0, width, height);
+}
+
+VKRT_Result VKRT_continueRender(VKRT* vkrt, uint32_t targetSamples) {
+    if (!vkrt) return VKRT_ERROR_INVALID_ARGUMENT;
+    if (vkrt->renderStatus.renderPhase != VKRT_RENDER_PHASE_COMPLETE_RAW &&
+        vkrt->renderStatus.renderPhase != VKRT_RENDER_PHASE_COMPLETE_DENOISED)
+        return VKRT_ERROR_OPERATION_FAILED;
+    vkrt->renderStatus.renderPhase = VKRT_RENDER_PHASE_SAMPLING;
+    vkrt->renderStatus.renderTargetSamples = targetSamples;
+    vkrtWriteSceneStateUniform(vkrt);
+    return VKRT_SUCCESS;
+}
+
+VKRT_Result VKRT_stopRenderSampling(VKRT* vkrt) {
+    if (!vkrt) return VKRT_ERROR_INVALID_ARGUMENT;
+    if (sessionSampling(vkrt)) vkrt->renderStatus.renderPhase = VKRT_RENDER_PHASE_COMPLETE_RAW;
+    return VKRT_SUCCESS;
+}
+
+VKRT_Result VKRT_stopRender(VKRT* vkrt) {
+    if (!vkrt) return VKRT_ERROR_INVALID_ARGUMENT;
+    if (vkrt->renderStatus.renderPhase == VKRT_RENDER_PHASE_INACTIVE) return VKRT_SUCCESS;
+    vkrt->renderStatus.renderPhase = VKRT_RENDER_PHASE_INACTIVE;
+    vkrt->renderStatus.renderTargetSamples = 0;
+    vkrtResetSceneData(vkrt);
+    return VKRT_SUCCESS;
+}
+
+VKRT_Result VKRT_getRenderStatus(const VKRT* vkrt, VKRT_RenderStatusSnapshot* out) {
+    if (!vkrt || !out) return VKRT_ERROR_INVALID_ARGUMENT;
+    *out = vkrt->renderStatus;
+    return VKRT_SUCCESS;
+}
+
+VKRT_Result VKRT_getTraceStats(const VKRT* vkrt, VKRT_TraceStats* out) {
+    if (!vkrt || !out) return VKRT_ERROR_INVALID_ARGUMENT;
+    *out = vkrt->stats;
+    return VKRT_SUCCESS;
+}
+
+VKRT_Result VKRT_readAccumulation(const VKRT* vkrt, float* out, size_t floatCount) {
+    if (!vkrt || !out) return VKRT_ERROR_INVALID_ARGUMENT;
+    if (!vkrt->device) return VKRT_ERROR_DEVICE_LOST;
+    if (floatCount != (size_t)vkrt->renderWidth * vkrt->renderHeight * 4u) return VKRT_ERROR_INVALID_ARGUMENT;
+    return mrt_read_accumulation(vkrt->device, out, floatCount) == 0 ? VKRT_SUCCESS : deviceError();
+}
+
+VKRT_Result VKRT_readOutputImage(const VKRT* vkrt, uint16_t* out, size_t count) {
+    if (!vkrt || !out) return VKRT_ERROR_INVALID_ARGUMENT;
+    if (!vkrt->device) return VKRT_ERROR_DEVICE_LOST;
+    if (count != (size_t)vkrt->renderWidth * vkrt->renderHeight * 4u) return VKRT_ERROR_INVALID_ARGUMENT;
+    return mrt_read_output(vkrt->device, out, count) == 0 ? VKRT_SUCCESS : deviceError();
+}
+
+VKRT_Result VKRT_exportSceneBuffers(VKRT* vkrt, VKRT_SceneBuffers* out) {
+    if (!vkrt || !out) return VKRT_ERROR_INVALID_ARGUMENT;
+    VKRT_Result r = vkrtLoadRGB2SpecTable(vkrt);
+    if (r != VKRT_SUCCESS) return r;
+    r = syncScene(vkrt, 0);
+    if (r != VKRT_SUCCESS) return r;
+    const PackedScene* p = &vkrt->packed;
+    memset(out, 0, sizeof(*out));
+    out->sceneData = &vkrt->sceneData;
+    out->vertices = p->vertices;
+    out->vertexCount = p->vertexCount;
+    out->indices = p->indices;
+    out->indexCount = p->indexCount;
+    out->meshInfos = p->meshInfos;
+    out->meshCount = vkrt->meshCount;
+    out->materials = p->materials;
+    out->materialCount = p->materialCount;
+    out->emissiveMeshes = p->emissiveMeshes;
+    out->emissiveTriangles = p->emissiveTriangles;
+    out->meshAliasQ = p->meshAliasQ;
+    out->meshAliasIdx = p->meshAliasIdx;
+    out->triAliasQ = p->triAliasQ;
+    out->triAliasIdx = p->triAliasIdx;
+    out->rgb2specTable = vkrt->rgb2spec;
+    out->rgb2specFloatCount = vkrt->rgb2specFloatCount;
+    uint32_t ti = vkrt->settings.environmentTextureIndex;
+    if (ti != VKRT_INVALID_INDEX && ti < vkrt->textureCount) {
+        out->environmentPixels = vkrt->textures[ti].rgba;
+        out->environmentWidth = vkrt->textures[ti].width;
+        out->environmentHeight = vkrt->textures[ti].height;
+    }
+    out->worldTriangles = p->worldTriangles;
+    out->triangleCount = p->triangleCount;
+    out->triangleMesh = p->triangleMesh;
+    out->rngPolicy = vkrt->rngPolicy;
+    out->rngSeed = vkrt->rngSeed;
+    out->environmentImportanceSampling = vkrt->envImportance;
+    return VKRT_SUCCESS;
+}
+
+static VKRT_Result requireTraceable(VKRT* vkrt) {
+    if (!vkrt) return VKRT_ERROR_INVALID_ARGUMENT;
+    if (!vkrt->device) return VKRT_ERROR_DEVICE_LOST;
+    return syncScene(vkrt, 1);
+}
+
+VKRT_Result VKRT_traceRays(VKRT* vkrt, const float* rays, size_t count, uint32_t* outMesh, uint32_t* outPrimitive,
+                           float* outT, float* outBarycentrics) {
+    if (!rays || !outMesh || !outPrimitive) return VKRT_ERROR_INVALID_ARGUMENT;
+    VKRT_Result r = requireTraceable(vkrt);
+    if (r != VKRT_SUCCESS) return r;
+    if (count == 0) return VKRT_SUCCESS;
+    return mrt_trace_rays(vkrt->device, rays, count, outMesh, outPrimitive, outT, outBarycentrics) == 0 ? VKRT_SUCCESS
+                                                                                                        : deviceError();
+}
+
+VKRT_Result VKRT_traceShadowRays(VKRT* vkrt, const float* rays, size_t count, uint32_t* outVisibility) {
+    if (!rays || !outVisibility) return VKRT_ERROR_INVALID_ARGUMENT;
+    VKRT_Result r = requireTraceable(vkrt);
+    if (r != VKRT_SUCCESS) return r;
+    if (count == 0) return VKRT_SUCCESS;
+    return mrt_trace_shadow_rays(vkrt->device, rays, count, outVisibility) == 0 ? VKRT_SUCCESS : deviceError();
+}
+
+VKRT_Result VKRT_tracePrimaryHits(VKRT* vkrt, uint32_t sampleIndex, uint32_t* outMesh, uint32_t* outPrimitive) {
+    if (!outMesh || !outPrimitive) return VKRT_ERROR_INVALID_ARGUMENT;
+    VKRT_Result r = requireTraceable(vkrt);
+    if (r != VKRT_SUCCESS) return r;
+    mrt_render_params rp;
+    fillRenderParams(vkrt, &rp);
+    return mrt_primary_hits(vkrt->device, &vkrt->sceneData, &rp, sampleIndex, outMesh, outPrimitive) == 0
+               ? VKRT_SUCCESS
+               : deviceError();
+}

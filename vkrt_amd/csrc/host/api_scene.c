@@ -1,0 +1,361 @@
+/*
+ * Scene objects: meshes, materials, textures and transforms.
+ * Behaviour follows src/core/api/mesh.c and src/core/scene/geometry.c:
+ *  - the first upload or material creates the default material at index 0
+ *    (src/core/internal/state.c:42-54);
+ *  - vertex / index arrays are copied (geometry.c:411-429);
+ *  - material fields are clamped rather than rejected (mesh.c:107-195);
+ *  - every change restarts accumulation (resetSceneData).
+ */
+#include "internal.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+static float finiteClamp(float v, float fallback, float lo, float hi) {
+    if (!isfinite(v)) {
+        if (isinf(v) && v > 0.0f && isinf(hi)) return v;
+        return fallback;
+    }
+    if (v < lo) return lo;
+    if (v > hi) return hi;
+    return v;
+}
+
+static Material sanitizeMaterial(const VKRT* vkrt, Material m) {
+    for (int c = 0; c < 3; c++) {
+        m.baseColor[c] = finiteClamp(m.baseColor[c], 0.0f, 0.0f, 1.0f);
+        m.emissionColor[c] = finiteClamp(m.emissionColor[c], 0.0f, 0.0f, INFINITY);
+        m.sheenTintWeight[c] = finiteClamp(m.sheenTintWeight[c], 0.0f, 0.0f, 1.0f);
+        m.attenuationColor[c] = finiteClamp(m.attenuationColor[c], 1.0f, 0.0f, 1.0f);
+        m.eta[c] = finiteClamp(m.eta[c], 0.0f, 0.0f, INFINITY);
+        m.k[c] = finiteClamp(m.k[c], 0.0f, 0.0f, INFINITY);
+    }
+    m.metallic = finiteClamp(m.metallic, 0.0f, 0.0f, 1.0f);
+    m.roughness = finiteClamp(m.roughness, 0.0f, 0.0f, 1.0f);
+    m.diffuseRoughness = finiteClamp(m.diffuseRoughness, 0.0f, 0.0f, 1.0f);
+    m.specular = finiteClamp(m.specular, 0.0f, 0.0f, 1.0f);
+    m.specularTint = finiteClamp(m.specularTint, 0.0f, 0.0f, 1.0f);
+    m.anisotropic = finiteClamp(m.anisotropic, 0.0f, 0.0f, 1.0f);
+    m.sheenTintWeight[3] = finiteClamp(m.sheenTintWeight[3], 0.0f, 0.0f, 1.0f);
+    /* The sheen lobe needs the reference's LTC fit table, which this build does
+     * not carry; a sheen layer is dropped (weight 0) rather than approximated. */
+    if (m.sheenTintWeight[3] > 0.0f) {
+        VKRT_LOG_ERROR("sheen is not supported; sheen weight set to 0");
+        m.sheenTintWeight[3] = 0.0f;
+    }
+    m.clearcoat = finiteClamp(m.clearcoat, 0.0f, 0.0f, 1.0f);
+    m.clearcoatGloss = finiteClamp(m.clearcoatGloss, 0.0f, 0.0f, 1.0f);
+    m.ior = finiteClamp(m.ior, 1.0f, 1.0f, 4.0f);
+    m.abbeNumber = finiteClamp(m.abbeNumber, 0.0f, 0.0f, 200.0f);
+    m.transmission = finiteClamp(m.transmission, 0.0f, 0.0f, 1.0f);
+    m.subsurface = finiteClamp(m.subsurface, 0.0f, 0.0f, 1.0f);
+    m.sheenRoughness = finiteClamp(m.sheenRoughness, 0.0f, 0.0f, 1.0f);
+    m.absorptionCoefficient = finiteClamp(m.absorptionCoefficient, 0.0f, 0.0f, VKRT_MAX_ABSORPTION_COEFFICIENT);
+    m.emissionLuminance = finiteClamp(m.emissionLuminance, 0.0f, 0.0f, INFINITY);
+    if (!isfinite(m.normalTextureScale)) m.normalTextureScale = 1.0f;
+    if (m.normalTextureScale < 0.0f) m.normalTextureScale = 0.0f;
+    m.opacity = finiteClamp(m.opacity, 1.0f, 0.0f, 1.0f);
+    m.alphaCutoff = finiteClamp(m.alphaCutoff, 0.5f, 0.0f, 1.0f);
+    if (m.alphaMode != VKRT_MATERIAL_ALPHA_MODE_MASK && m.alphaMode != VKRT_MATERIAL_ALPHA_MODE_BLEND) {
+        m.alphaMode = VKRT_MATERIAL_ALPHA_MODE_OPAQUE;
+    }
+    uint32_t* slots[4] = {&m.baseColorTextureIndex, &m.metallicRoughnessTextureIndex, &m.normalTextureIndex,
+                          &m.emissiveTextureIndex};
+    for (int i = 0; i < 4; i++) {
+        if (*slots[i] != VKRT_INVALID_INDEX && *slots[i] >= vkrt->textureCount) *slots[i] = VKRT_INVALID_INDEX;
+    }
+    return m;
+}
+
+static VKRT_Result ensureDefaultMaterial(VKRT* vkrt) {
+    if (vkrt->materialCount > 0) return VKRT_SUCCESS;
+    vkrt->materials = (HostMaterial*)calloc(4, sizeof(HostMaterial));
+    if (!vkrt->materials) return VKRT_ERROR_OUT_OF_MEMORY;
+    vkrt->materialCapacity = 4;
+    vkrt->materials[0].material = VKRT_materialDefault();
+    snprintf(vkrt->materials[0].name, VKRT_NAME_LEN, "%s", "Default");
+    vkrt->materialCount = 1;
+    return VKRT_SUCCESS;
+}
+
+VKRT_Result VKRT_uploadMeshDataBatch(VKRT* vkrt, const VKRT_MeshUpload* uploads, size_t uploadCount) {
+    if (!vkrt || (!uploads && uploadCount)) return VKRT_ERROR_INVALID_ARGUMENT;
+    for (size_t i = 0; i < uploadCount; i++) {
+        const VKRT_MeshUpload* u = &uploads[i];
+        if (!u->vertices || !u->indices || u->vertexCount == 0 || u->indexCount == 0 || u->indexCount % 3u != 0)
+            return VKRT_ERROR_INVALID_ARGUMENT;
+        if (u->vertexCount > UINT32_MAX || u->indexCount > UINT32_MAX) return VKRT_ERROR_INVALID_ARGUMENT;
+        /* every index must address a vertex of its own mesh: the kernels fetch vertices unchecked */
+        for (size_t k = 0; k < u->indexCount; k++) {
+            if (u->indices[k] >= u->vertexCount) return VKRT_ERROR_INVALID_ARGUMENT;
+        }
+    }
+    VKRT_Result r = ensureDefaultMaterial(vkrt);
+    if (r != VKRT_SUCCESS) return r;
+    uint32_t needed = vkrt->meshCount + (uint32_t)uploadCount;
+    if (needed > vkrt->meshCapacity) {
+        uint32_t cap = vkrt->meshCapacity ? vkrt->meshCapacity : 8u;
+        while (cap < needed) cap *= 2u;
+        HostMesh* grown = (HostMesh*)realloc(vkrt->meshes, sizeof(HostMesh) * cap);
+        if (!grown) return VKRT_ERROR_OUT_OF_MEMORY;
+        vkrt->meshes = grown;
+        vkrt->meshCapacity = cap;
+    }
+    for (size_t i = 0; i < uploadCount; i++) {
+        const VKRT_MeshUpload* u = &uploads[i];
+        HostMesh* mesh = &vkrt->meshes[vkrt->meshCount];
+        memset(mesh, 0, sizeof(*mesh));
+        mesh->vertices = (Vertex*)malloc(sizeof(Vertex) * u->vertexCount);
+        mesh->indices = (uint32_t*)malloc(sizeof(uint32_t) * u->indexCount);
+        if (!mesh->vertices || !mesh->indices) {
+            free(mesh->vertices);
+            free(mesh->indices);
+            return VKRT_ERROR_OUT_OF_MEMORY;
+        }
+        memcpy(mesh->vertices, u->vertices, sizeof(Vertex) * u->vertexCount);
+        memcpy(mesh->indices, u->indices, sizeof(uint32_t) * u->indexCount);
+        for (int c = 0; c < 4; c++)
+            for (int rr = 0; rr < 4; rr++) mesh->worldTransform[c][rr] = c == rr ? 1.0f : 0.0f;
+        mesh->info.vertexCount = (uint32_t)u->vertexCount;
+        mesh->info.indexCount = (uint32_t)u->indexCount;
+        mesh->info.materialIndex = 0u;
+        mesh->info.opacity = 1.0f;
+        mesh->info.scale[0] = mesh->info.scale[1] = mesh->info.scale[2] = 1.0f;
+        snprintf(mesh->name, VKRT_NAME_LEN, "Mesh %u", vkrt->meshCount);
+        vkrt->meshCount++;
+    }
+    vkrt->geometryRevision++;
+    vkrtResetSceneData(vkrt);
+    return VKRT_SUCCESS;
+}
+
+VKRT_Result VKRT_uploadMeshData(VKRT* vkrt, const Vertex* vertices, size_t vertexCount, const uint32_t* indices,
+                                size_t indexCount) {
+    VKRT_MeshUpload u = {vertices, vertexCount, indices, indexCount};
+    return VKRT_uploadMeshDataBatch(vkrt, &u, 1);
+}
+
+VKRT_Result VKRT_removeMesh(VKRT* vkrt, uint32_t meshIndex) {
+    if (!vkrt || meshIndex >= vkrt->meshCount) return VKRT_ERROR_INVALID_ARGUMENT;
+    free(vkrt->meshes[meshIndex].vertices);
+    free(vkrt->meshes[meshIndex].indices);
+    memmove(&vkrt->meshes[meshIndex], &vkrt->meshes[meshIndex + 1],
+            sizeof(HostMesh) * (vkrt->meshCount - meshIndex - 1u));
+    vkrt->meshCount--;
+    vkrt->geometryRevision++;
+    vkrtResetSceneData(vkrt);
+    return VKRT_SUCCESS;
+}
+
+VKRT_Result VKRT_getMeshCount(const VKRT* vkrt, uint32_t* outMeshCount) {
+    if (!vkrt || !outMeshCount) return VKRT_ERROR_INVALID_ARGUMENT;
+    *outMeshCount = vkrt->meshCount;
+    return VKRT_SUCCESS;
+}
+
+VKRT_Result VKRT_getMeshSnapshot(const VKRT* vkrt, uint32_t meshIndex, VKRT_MeshSnapshot* out) {
+    if (!vkrt || !out || meshIndex >= vkrt->meshCount) return VKRT_ERROR_INVALID_ARGUMENT;
+    memset(out, 0, sizeof(*out));
+    const HostMesh* mesh = &vkrt->meshes[meshIndex];
+    out->info = mesh->info;
+    if (mesh->info.materialIndex < vkrt->materialCount) out->material = vkrt->materials[mesh->info.materialIndex].material;
+    out->materialIndex = mesh->info.materialIndex;
+    out->geometrySource = meshIndex;
+    out->hasMaterialAssignment = mesh->hasMaterialAssignment;
+    out->ownsGeometry = 1u;
+    snprintf(out->name, VKRT_NAME_LEN, "%s", mesh->name);
+    return VKRT_SUCCESS;
+}
+
+VKRT_Result VKRT_setMeshName(VKRT* vkrt, uint32_t meshIndex, const char* name) {
+    if (!vkrt || !name || meshIndex >= vkrt->meshCount) return VKRT_ERROR_INVALID_ARGUMENT;
+    snprintf(vkrt->meshes[meshIndex].name, VKRT_NAME_LEN, "%s", name);
+    return VKRT_SUCCESS;
+}
+
+VKRT_Result VKRT_getMaterialCount(const VKRT* vkrt, uint32_t* outMaterialCount) {
+    if (!vkrt || !outMaterialCount) return VKRT_ERROR_INVALID_ARGUMENT;
+    *outMaterialCount = vkrt->materialCount;
+    return VKRT_SUCCESS;
+}
+
+VKRT_Result VKRT_getMaterialSnapshot(const VKRT* vkrt, uint32_t materialIndex, VKRT_MaterialSnapshot* out) {
+    if (!vkrt || !out || materialIndex >= vkrt->materialCount) return VKRT_ERROR_INVALID_ARGUMENT;
+    memset(out, 0, sizeof(*out));
+    out->material = vkrt->materials[materialIndex].material;
+    for (uint32_t i = 0; i < vkrt->meshCount; i++) out->useCount += vkrt->meshes[i].info.materialIndex == materialIndex;
+    snprintf(out->name, VKRT_NAME_LEN, "%s", vkrt->materials[materialIndex].name);
+    return VKRT_SUCCESS;
+}
+
+VKRT_Result VKRT_addMaterial(VKRT* vkrt, const Material* material, const char* name, uint32_t* outMaterialIndex) {
+    if (outMaterialIndex) *outMaterialIndex = VKRT_INVALID_INDEX;
+    if (!vkrt) return VKRT_ERROR_INVALID_ARGUMENT;
+    VKRT_Result r = ensureDefaultMaterial(vkrt);
+    if (r != VKRT_SUCCESS) return r;
+    if (vkrt->materialCount == vkrt->materialCapacity) {
+        uint32_t cap = vkrt->materialCapacity * 2u;
+        HostMaterial* grown = (HostMaterial*)realloc(vkrt->materials, sizeof(HostMaterial) * cap);
+        if (!grown) return VKRT_ERROR_OUT_OF_MEMORY;
+        vkrt->materials = grown;
+        vkrt->materialCapacity = cap;
+    }
+    uint32_t index = vkrt->materialCount++;
+    HostMaterial* hm = &vkrt->materials[index];
+    memset(hm, 0, sizeof(*hm));
+    hm->material = sanitizeMaterial(vkrt, material ? *material : VKRT_materialDefault());
+    if (name && name[0])
+        snprintf(hm->name, VKRT_NAME_LEN, "%s", name);
+    else
+        snprintf(hm->name, VKRT_NAME_LEN, "Material %u", index);
+    vkrt->materialRevision++;
+    vkrtResetSceneData(vkrt);
+    if (outMaterialIndex) *outMaterialIndex = index;
+    return VKRT_SUCCESS;
+}
+
+VKRT_Result VKRT_setMaterial(VKRT* vkrt, uint32_t materialIndex, const Material* material) {
+    if (!vkrt || !material || materialIndex >= vkrt->materialCount) return VKRT_ERROR_INVALID_ARGUMENT;
+    vkrt->materials[materialIndex].material = sanitizeMaterial(vkrt, *material);
+    vkrt->materialRevision++;
+    vkrtResetSceneData(vkrt);
+    return VKRT_SUCCESS;
+}
+
+VKRT_Result VKRT_setMeshMaterialIndex(VKRT* vkrt, uint32_t meshIndex, uint32_t materialIndex) {
+    if (!vkrt || meshIndex >= vkrt->meshCount || materialIndex >= vkrt->materialCount)
+        return VKRT_ERROR_INVALID_ARGUMENT;
+    HostMesh* mesh = &vkrt->meshes[meshIndex];
+    if (mesh->info.materialIndex == materialIndex && mesh->hasMaterialAssignment) return VKRT_SUCCESS;
+    mesh->info.materialIndex = materialIndex;
+    mesh->hasMaterialAssignment = 1u;
+    vkrt->materialRevision++;
+    vkrtResetSceneData(vkrt);
+    return VKRT_SUCCESS;
+}
+
+static int matrixValid(float m[4][4]) {
+    for (int c = 0; c < 4; c++)
+        for (int r = 0; r < 4; r++)
+            if (!isfinite(m[c][r])) return 0;
+    return 1;
+}
+
+VKRT_Result VKRT_setMeshTransformMatrix(VKRT* vkrt, uint32_t meshIndex, float worldTransform[4][4]) {
+    if (!vkrt || !worldTransform || meshIndex >= vkrt->meshCount || !matrixValid(worldTransform))
+        return VKRT_ERROR_INVALID_ARGUMENT;
+    HostMesh* mesh = &vkrt->meshes[meshIndex];
+    if (memcmp(mesh->worldTransform, worldTransform, sizeof(mesh->worldTransform)) == 0) return VKRT_SUCCESS;
+    memcpy(mesh->worldTransform, worldTransform, sizeof(mesh->worldTransform));
+    VKRT_decomposeMeshTransform(mesh->worldTransform, mesh->info.position, mesh->info.rotation, mesh->info.scale);
+    vkrt->geometryRevision++;
+    vkrtResetSceneData(vkrt);
+    return VKRT_SUCCESS;
+}
+
+VKRT_Result VKRT_setMeshTransform(VKRT* vkrt, uint32_t meshIndex, const float position[3], const float rotation[3],
+                                  const float scale[3]) {
+    if (!vkrt || meshIndex >= vkrt->meshCount) return VKRT_ERROR_INVALID_ARGUMENT;
+    const MeshInfo* info = &vkrt->meshes[meshIndex].info;
+    float p[3], r[3], s[3];
+    for (int i = 0; i < 3; i++) {
+        p[i] = position && isfinite(position[i]) ? position[i] : info->position[i];
+        r[i] = rotation && isfinite(rotation[i]) ? rotation[i] : info->rotation[i];
+        s[i] = scale && isfinite(scale[i]) ? scale[i] : info->scale[i];
+    }
+    float m[4][4];
+    VKRT_buildMeshTransformMatrix(p, r, s, m);
+    return VKRT_setMeshTransformMatrix(vkrt, meshIndex, m);
+}
+
+static float halfToFloat(uint16_t h) {
+    uint32_t sign = (uint32_t)(h & 0x8000u) << 16;
+    uint32_t exp = (h >> 10) & 0x1fu;
+    uint32_t man = h & 0x3ffu;
+    uint32_t bits;
+    if (exp == 0) {
+        if (man == 0) {
+            bits = sign;
+        } else {
+            int e = -1;
+            do {
+                e++;
+                man <<= 1;
+            } while ((man & 0x400u) == 0);
+            bits = sign | ((uint32_t)(127 - 15 - e) << 23) | ((man & 0x3ffu) << 13);
+        }
+    } else if (exp == 31) {
+        bits = sign | 0x7f800000u | (man << 13);
+    } else {
+        bits = sign | ((exp + 112u) << 23) | (man << 13);
+    }
+    float f;
+    memcpy(&f, &bits, sizeof(f));
+    return f;
+}
+
+static float srgbToLinear(float v) { return v <= 0.04045f ? v / 12.92f : powf((v + 0.055f) / 1.055f, 2.4f); }
+
+VKRT_Result VKRT_addTextureFromPixels(VKRT* vkrt, const VKRT_TextureUpload* upload, uint32_t* outTextureIndex) {
+    if (outTextureIndex) *outTextureIndex = VKRT_INVALID_INDEX;
+    if (!vkrt || !upload || !upload->pixels || upload->width == 0 || upload->height == 0 ||
+        upload->format >= VKRT_TEXTURE_FORMAT_COUNT || upload->colorSpace > VKRT_TEXTURE_COLOR_SPACE_LINEAR)
+        return VKRT_ERROR_INVALID_ARGUMENT;
+    if (vkrt->textureCount == vkrt->textureCapacity) {
+        uint32_t cap = vkrt->textureCapacity ? vkrt->textureCapacity * 2u : 4u;
+        HostTexture* grown = (HostTexture*)realloc(vkrt->textures, sizeof(HostTexture) * cap);
+        if (!grown) return VKRT_ERROR_OUT_OF_MEMORY;
+        vkrt->textures = grown;
+        vkrt->textureCapacity = cap;
+    }
+    size_t n = (size_t)upload->width * upload->height * 4u;
+    float* rgba = (float*)malloc(sizeof(float) * n);
+    if (!rgba) return VKRT_ERROR_OUT_OF_MEMORY;
+    for (size_t i = 0; i < n; i++) {
+        float v;
+        switch (upload->format) {
+            case VKRT_TEXTURE_FORMAT_RGBA8_UNORM: v = (float)((const uint8_t*)upload->pixels)[i] / 255.0f; break;
+            case VKRT_TEXTURE_FORMAT_RGBA16_UNORM: v = (float)((const uint16_t*)upload->pixels)[i] / 65535.0f; break;
+            case VKRT_TEXTURE_FORMAT_RGBA16_SFLOAT: v = halfToFloat(((const uint16_t*)upload->pixels)[i]); break;
+            default: v = ((const float*)upload->pixels)[i]; break;
+        }
+        if (upload->colorSpace == VKRT_TEXTURE_COLOR_SPACE_SRGB && (i & 3u) != 3u) v = srgbToLinear(v);
+        rgba[i] = v;
+    }
+    HostTexture* t = &vkrt->textures[vkrt->textureCount];
+    memset(t, 0, sizeof(*t));
+    t->rgba = rgba;
+    t->width = upload->width;
+    t->height = upload->height;
+    t->format = upload->format;
+    t->colorSpace = upload->colorSpace;
+    snprintf(t->name, VKRT_NAME_LEN, "%s", upload->name ? upload->name : "Texture");
+    if (outTextureIndex) *outTextureIndex = vkrt->textureCount;
+    vkrt->textureCount++;
+    return VKRT_SUCCESS;
+}
+
+VKRT_Result VKRT_getTextureCount(const VKRT* vkrt, uint32_t* outTextureCount) {
+    if (!vkrt || !outTextureCount) return VKRT_ERROR_INVALID_ARGUMENT;
+    *outTextureCount = vkrt->textureCount;
+    return VKRT_SUCCESS;
+}
+
+/* src/core/scene/environment.c:11-41: only linear textures may light the scene. */
+VKRT_Result VKRT_setEnvironmentTexture(VKRT* vkrt, uint32_t textureIndex) {
+    if (!vkrt) return VKRT_ERROR_INVALID_ARGUMENT;
+    if (textureIndex != VKRT_INVALID_INDEX) {
+        if (textureIndex >= vkrt->textureCount) return VKRT_ERROR_INVALID_ARGUMENT;
+        if (vkrt->textures[textureIndex].colorSpace != VKRT_TEXTURE_COLOR_SPACE_LINEAR)
+            return VKRT_ERROR_INVALID_ARGUMENT;
+    }
+    if (vkrt->settings.environmentTextureIndex == textureIndex) return VKRT_SUCCESS;
+    vkrt->settings.environmentTextureIndex = textureIndex;
+    vkrt->environmentRevision++;
+    vkrtResetSceneData(vkrt);
+    return VKRT_SUCCESS;
+}
+
+VKRT_Result VKRT_clearEnvironmentTexture(VKRT* vkrt) { return VKRT_setEnvironmentTexture(vkrt, VKRT_INVALID_INDEX); }

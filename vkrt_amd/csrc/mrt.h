@@ -1,0 +1,120 @@
+/*
+ * Internal C ABI between the C host layer (csrc/host) and the gfx950 kernels
+ * (csrc/hip). Plain pointers and sizes only. Every function returns 0 on
+ * success and a negative VKRT_Result value on failure; a message describing
+ * the last failure is available from mrt_last_error().
+ *
+ * This seam replaces the reference's Vulkan command recording: the BLAS/TLAS
+ * builds (src/core/render/accel/blas.c:222-262, tlas.c:535-559) become
+ * mrt_upload_geometry + mrt_build_bvh, and the main trace pass
+ * (src/core/runtime/command/record.c:448-486) becomes mrt_render_frame.
+ */
+#ifndef VKRT_MRT_H
+#define VKRT_MRT_H
+
+#include "vkrt_shared.h"
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mrt_context mrt_context;
+
+const char* mrt_last_error(void);
+int mrt_device_count(void);
+int mrt_create(int deviceIndex, mrt_context** out);
+void mrt_destroy(mrt_context* ctx);
+int mrt_device_name(const mrt_context* ctx, char* out, size_t len, uint32_t* vendorId, uint32_t* driverVersion);
+
+typedef struct mrt_geometry_desc {
+    const ShaderVertex* vertices;
+    uint32_t vertexCount;
+    const uint32_t* indices;
+    uint32_t indexCount;
+    const float* worldTriangles; /* 9 floats per global triangle */
+    const uint32_t* triangleMesh;
+    uint32_t triangleCount;
+    const uint32_t* meshTriangleBase; /* first global triangle of each mesh */
+    uint32_t meshCount;
+} mrt_geometry_desc;
+int mrt_upload_geometry(mrt_context* ctx, const mrt_geometry_desc* desc);
+
+typedef struct mrt_bvh_stats {
+    float buildMs;
+    uint32_t nodeCount;
+    uint32_t leafTriangles;
+    uint32_t maxDepth;
+} mrt_bvh_stats;
+int mrt_build_bvh(mrt_context* ctx, mrt_bvh_stats* stats);
+
+int mrt_upload_meshes(mrt_context* ctx, const MeshInfo* meshInfos, uint32_t meshCount);
+int mrt_upload_materials(mrt_context* ctx, const Material* materials, uint32_t materialCount);
+
+typedef struct mrt_light_desc {
+    const EmissiveMesh* emissiveMeshes;
+    uint32_t emissiveMeshCount;
+    const EmissiveTriangle* emissiveTriangles;
+    uint32_t emissiveTriangleCount;
+    const float* meshAliasQ;
+    const uint32_t* meshAliasIdx;
+    const float* triAliasQ;
+    const uint32_t* triAliasIdx;
+} mrt_light_desc;
+int mrt_upload_lights(mrt_context* ctx, const mrt_light_desc* desc);
+
+typedef struct mrt_environment_desc {
+    const float* pixels; /* RGBA32F, width*height*4, NULL clears */
+    uint32_t width;
+    uint32_t height;
+    const float* aliasQ;   /* width*height entries: importance alias table over texels */
+    const uint32_t* aliasIdx;
+    const float* pdf;      /* width*height texel selection probabilities */
+} mrt_environment_desc;
+int mrt_upload_environment(mrt_context* ctx, const mrt_environment_desc* desc);
+
+int mrt_upload_rgb2spec(mrt_context* ctx, const float* table, uint32_t floatCount);
+
+typedef struct mrt_render_params {
+    uint32_t width;  /* accumulation extent */
+    uint32_t height;
+    uint32_t rngPolicy;
+    uint64_t rngSeed;
+    uint32_t envImportance;
+    uint32_t hasTransmissive; /* any material with transmission > 0 */
+    uint32_t resetAccumulation;
+} mrt_render_params;
+
+typedef struct mrt_render_stats {
+    uint64_t extensionRays;
+    uint64_t shadowRays;
+    uint64_t paths;
+    float traceMs;
+    float extendMs;
+    float shadowMs;
+    float shadeMs;
+    uint32_t extendLaunches;
+} mrt_render_stats;
+
+/* One frame: samplesPerPixel paths for every viewport pixel, accumulated into
+ * the running mean exactly as src/shaders/integrator/path/writeback.slang:72-123. */
+int mrt_render_frame(mrt_context* ctx, const SceneData* scene, const mrt_render_params* params,
+                     mrt_render_stats* stats);
+int mrt_resize_accumulation(mrt_context* ctx, uint32_t width, uint32_t height);
+int mrt_clear_accumulation(mrt_context* ctx);
+int mrt_read_accumulation(mrt_context* ctx, float* out, size_t floatCount);
+int mrt_read_output(mrt_context* ctx, uint16_t* out, size_t count);
+
+int mrt_trace_rays(mrt_context* ctx, const float* rays, size_t count, uint32_t* outMesh, uint32_t* outPrim,
+                   float* outT, float* outBary);
+int mrt_trace_shadow_rays(mrt_context* ctx, const float* rays, size_t count, uint32_t* outVisibility);
+int mrt_primary_hits(mrt_context* ctx, const SceneData* scene, const mrt_render_params* params, uint32_t sampleIndex,
+                     uint32_t* outMesh, uint32_t* outPrim);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

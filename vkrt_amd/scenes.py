@@ -1,0 +1,285 @@
+"""Procedural scenes for the benchmark configurations.
+
+cornell_box   -- 5 wall quads + two closed axis-aligned boxes (heights 0.3 and
+                 0.6) = 34 triangles, plus a 2-triangle ceiling light of
+                 radiance 15; Lambertian materials expressed in the reference's
+                 principled Material (roughness 1, specular 0, metallic 0 makes
+                 its BSDF exactly Lambertian).
+displaced_sphere -- geodesic icosphere of frequency f (20*f^2 triangles; f = 229
+                 gives 1,048,820), displaced along the normal by seeded
+                 3-octave value noise of amplitude 0.1*radius, split into 64
+                 meshes by material id = hash(face) mod 64.
+sky_environment -- lat-long RGBA32F sky gradient with a 0.5 degree sun disc.
+Scenes are returned as plain data and loaded through the C API by `load`.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import Renderer, default_material, make_vertices
+
+
+@dataclass
+class MeshData:
+    vertices: np.ndarray  # N x 20 float32 (reference Vertex layout)
+    indices: np.ndarray  # M x 3 uint32
+    material: int  # index into Scene.materials
+
+
+@dataclass
+class Scene:
+    meshes: list = field(default_factory=list)
+    materials: list = field(default_factory=list)
+    camera: tuple = ((0.0, -1.0, 0.0), (0.0, 0.0, 0.0), (0.0, 0.0, 1.0), 40.0)
+    environment: np.ndarray | None = None  # H x W x 4 float32
+    environment_color: tuple = (0.0, 0.0, 0.0)
+    environment_strength: float = 0.0
+
+    @property
+    def triangle_count(self) -> int:
+        return sum(len(m.indices) for m in self.meshes)
+
+
+def _quad(p0, p1, p2, p3, normal):
+    pos = np.array([p0, p1, p2, p3], dtype=np.float32)
+    nrm = np.tile(np.asarray(normal, dtype=np.float32), (4, 1))
+    return pos, nrm, np.array([[0, 1, 2], [0, 2, 3]], dtype=np.uint32)
+
+
+def _merge(parts):
+    pos, nrm, idx = [], [], []
+    base = 0
+    for p, n, i in parts:
+        pos.append(p)
+        nrm.append(n)
+        idx.append(i + base)
+        base += len(p)
+    return make_vertices(np.concatenate(pos), np.concatenate(nrm)), np.concatenate(idx).astype(np.uint32)
+
+
+def _box(x0, x1, y0, y1, h):
+    """Closed axis-aligned box on the floor: 6 faces, outward normals."""
+    z0, z1 = 0.0, h
+    return [
+        _quad((x0, y0, z1), (x1, y0, z1), (x1, y1, z1), (x0, y1, z1), (0, 0, 1)),
+        _quad((x0, y0, z0), (x0, y1, z0), (x1, y1, z0), (x1, y0, z0), (0, 0, -1)),
+        _quad((x0, y0, z0), (x1, y0, z0), (x1, y0, z1), (x0, y0, z1), (0, -1, 0)),
+        _quad((x1, y1, z0), (x0, y1, z0), (x0, y1, z1), (x1, y1, z1), (0, 1, 0)),
+        _quad((x0, y1, z0), (x0, y0, z0), (x0, y0, z1), (x0, y1, z1), (-1, 0, 0)),
+        _quad((x1, y0, z0), (x1, y1, z0), (x1, y1, z1), (x1, y0, z1), (1, 0, 0)),
+    ]
+
+
+def lambert(rgb, emission=0.0):
+    m = default_material(baseColor=rgb, roughness=1.0, specular=0.0, metallic=0.0)
+    m.emissionLuminance = float(emission)
+    return m
+
+
+def cornell_box() -> Scene:
+    """Unit-cube Cornell box, z up, open towards -y, camera on the -y side."""
+    white, red, green = (0.73, 0.73, 0.73), (0.63, 0.065, 0.05), (0.14, 0.45, 0.09)
+    s = Scene()
+    s.materials = [lambert(white), lambert(red), lambert(green), lambert(white, emission=15.0)]
+    walls = [
+        _quad((0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1)),  # floor
+        _quad((0, 0, 1), (0, 1, 1), (1, 1, 1), (1, 0, 1), (0, 0, -1)),  # ceiling
+        _quad((0, 1, 0), (1, 1, 0), (1, 1, 1), (0, 1, 1), (0, -1, 0)),  # back
+    ]
+    s.meshes.append(MeshData(*_merge(walls), 0))
+    s.meshes.append(MeshData(*_merge([_quad((0, 0, 0), (0, 1, 0), (0, 1, 1), (0, 0, 1), (1, 0, 0))]), 1))
+    s.meshes.append(MeshData(*_merge([_quad((1, 0, 0), (1, 0, 1), (1, 1, 1), (1, 1, 0), (-1, 0, 0))]), 2))
+    s.meshes.append(MeshData(*_merge(_box(0.53, 0.83, 0.12, 0.42, 0.3)), 0))
+    s.meshes.append(MeshData(*_merge(_box(0.17, 0.47, 0.48, 0.78, 0.6)), 0))
+    zl = 1.0 - 1e-3
+    light = _quad((0.37, 0.37, zl), (0.63, 0.37, zl), (0.63, 0.63, zl), (0.37, 0.63, zl), (0, 0, -1))
+    s.meshes.append(MeshData(*_merge([light]), 3))
+    s.camera = ((0.5, -1.35, 0.5), (0.5, 0.5, 0.5), (0.0, 0.0, 1.0), 40.0)
+    s.environment_color = (0.0, 0.0, 0.0)
+    s.environment_strength = 0.0
+    return s
+
+
+# ---------------------------------------------------------------- sphere --
+_ICO_T = (1.0 + 5.0 ** 0.5) / 2.0
+_ICO_V = np.array(
+    [[-1, _ICO_T, 0], [1, _ICO_T, 0], [-1, -_ICO_T, 0], [1, -_ICO_T, 0],
+     [0, -1, _ICO_T], [0, 1, _ICO_T], [0, -1, -_ICO_T], [0, 1, -_ICO_T],
+     [_ICO_T, 0, -1], [_ICO_T, 0, 1], [-_ICO_T, 0, -1], [-_ICO_T, 0, 1]], dtype=np.float64)
+_ICO_F = np.array(
+    [[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], [5, 11, 4], [11, 10, 2], [10, 7, 6],
+     [7, 1, 8], [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9], [4, 9, 5], [2, 4, 11], [6, 2, 10], [8, 6, 7],
+     [9, 8, 1]], dtype=np.int64)
+
+
+def geodesic_sphere(freq: int):
+    """Frequency-f geodesic subdivision: 20 f^2 triangles, 10 f^2 + 2 unit vertices."""
+    f = int(freq)
+    # lattice points (i, j) with i + j <= f on each face; vertex = A + (B-A) i/f + (C-A) j/f
+    ii, jj = np.meshgrid(np.arange(f + 1), np.arange(f + 1), indexing="ij")
+    keep = ii + jj <= f
+    ii, jj = ii[keep], jj[keep]
+    local = -np.ones((f + 1, f + 1), dtype=np.int64)
+    local[ii, jj] = np.arange(len(ii))
+    # triangles in the lattice
+    a = local[:-1, :-1]
+    up_mask = (np.add.outer(np.arange(f), np.arange(f)) <= f - 1)
+    i0, j0 = np.nonzero(up_mask)
+    tri_up = np.stack([local[i0, j0], local[i0 + 1, j0], local[i0, j0 + 1]], axis=1)
+    dn_mask = (np.add.outer(np.arange(f), np.arange(f)) <= f - 2)
+    i1, j1 = np.nonzero(dn_mask)
+    tri_dn = np.stack([local[i1 + 1, j1], local[i1 + 1, j1 + 1], local[i1, j1 + 1]], axis=1)
+    lat_tris = np.concatenate([tri_up, tri_dn])
+    del a
+    pts, tris = [], []
+    base = 0
+    for face in _ICO_F:
+        A, B, Cc = _ICO_V[face]
+        p = A[None] + np.outer(ii / f, B - A) + np.outer(jj / f, Cc - A)
+        pts.append(p)
+        tris.append(lat_tris + base)
+        base += len(p)
+    pts = np.concatenate(pts)
+    tris = np.concatenate(tris)
+    pts /= np.linalg.norm(pts, axis=1, keepdims=True)
+    # weld duplicates on shared edges
+    key = np.round(pts * (1 << 20)).astype(np.int64)
+    _, first, inverse = np.unique(key, axis=0, return_index=True, return_inverse=True)
+    verts = pts[first]
+    tris = inverse.reshape(-1)[tris]
+    return verts, tris.astype(np.int64)
+
+
+def _value_noise(p: np.ndarray, seed: int, octaves: int = 3, base_freq: float = 2.0) -> np.ndarray:
+    """Seeded 3D value noise: lattice values hashed from integer coordinates,
+    smoothstep-weighted trilinear interpolation, octaves at doubling frequency
+    and halving amplitude; output roughly in [-1, 1]."""
+    total = np.zeros(len(p))
+    amp, freq, norm = 1.0, base_freq, 0.0
+    for o in range(octaves):
+        q = p * freq + 17.0 * o
+        i0 = np.floor(q).astype(np.int64)
+        t = q - i0
+        w = t * t * (3.0 - 2.0 * t)
+
+        def lattice(ix, iy, iz):
+            h = (ix * 73856093) ^ (iy * 19349663) ^ (iz * 83492791) ^ (seed * 2654435761 + o * 40503)
+            h = (h ^ (h >> 13)) * 0x5BD1E995
+            h = (h ^ (h >> 15)) & 0xFFFFFF
+            return h / float(0xFFFFFF) * 2.0 - 1.0
+
+        acc = 0.0
+        for dx in (0, 1):
+            for dy in (0, 1):
+                for dz in (0, 1):
+                    wx = w[:, 0] if dx else 1.0 - w[:, 0]
+                    wy = w[:, 1] if dy else 1.0 - w[:, 1]
+                    wz = w[:, 2] if dz else 1.0 - w[:, 2]
+                    acc = acc + wx * wy * wz * lattice(i0[:, 0] + dx, i0[:, 1] + dy, i0[:, 2] + dz)
+        total += amp * acc
+        norm += amp
+        amp *= 0.5
+        freq *= 2.0
+    return total / norm
+
+
+def _vertex_normals(v: np.ndarray, f: np.ndarray) -> np.ndarray:
+    fn = np.cross(v[f[:, 1]] - v[f[:, 0]], v[f[:, 2]] - v[f[:, 0]])
+    n = np.zeros_like(v)
+    for k in range(3):
+        np.add.at(n, f[:, k], fn)
+    return n / np.maximum(np.linalg.norm(n, axis=1, keepdims=True), 1e-30)
+
+
+def _face_hash(n: int, seed: int) -> np.ndarray:
+    x = (np.arange(n, dtype=np.uint64) * np.uint64(0x9E3779B97F4A7C15)) ^ np.uint64((seed * 0x632BE59BD9B4E019) & 0xFFFFFFFFFFFFFFFF)
+    x ^= x >> np.uint64(31)
+    x *= np.uint64(0xBF58476D1CE4E5B9)
+    x ^= x >> np.uint64(27)
+    return x
+
+
+def sphere_materials(count: int = 64, seed: int = 7, dielectric_fraction: float = 0.2):
+    rng = np.random.default_rng(seed)
+    mats = []
+    kinds = rng.random(count) < dielectric_fraction
+    for k in range(count):
+        rough = float(rng.uniform(0.05, 1.0))
+        albedo = tuple(float(x) for x in rng.uniform(0.05, 0.95, 3))
+        if kinds[k]:
+            m = default_material(baseColor=albedo, roughness=rough, metallic=0.0, specular=0.5)
+            m.transmission = 1.0
+            m.ior = 1.5
+            m.abbeNumber = 0.0
+        else:
+            m = default_material(baseColor=albedo, roughness=rough, metallic=0.0, specular=0.5)
+        mats.append(m)
+    return mats
+
+
+def displaced_sphere(freq: int = 229, radius: float = 1.0, seed: int = 11, materials: int = 64) -> Scene:
+    verts, tris = geodesic_sphere(freq)
+    disp = _value_noise(verts * radius, seed) * (0.1 * radius)
+    pos = verts * (radius + disp)[:, None]
+    nrm = _vertex_normals(pos, tris)
+    mat_id = (_face_hash(len(tris), seed) % np.uint64(materials)).astype(np.int64)
+    s = Scene()
+    s.materials = sphere_materials(materials, seed=seed)
+    order = np.argsort(mat_id, kind="stable")
+    bounds = np.searchsorted(mat_id[order], np.arange(materials + 1))
+    for m in range(materials):
+        ft = tris[order[bounds[m]:bounds[m + 1]]]
+        if len(ft) == 0:
+            continue
+        used, local = np.unique(ft, return_inverse=True)
+        s.meshes.append(MeshData(make_vertices(pos[used], nrm[used]), local.reshape(-1, 3).astype(np.uint32), m))
+    s.camera = ((2.6, -2.2, 1.1), (0.0, 0.0, 0.0), (0.0, 0.0, 1.0), 40.0)
+    return s
+
+
+def sky_environment(width: int = 2048, height: int = 1024, sun_elevation_deg: float = 35.0,
+                    sun_azimuth_deg: float = 40.0, sun_radiance: float = 1.0e4,
+                    sun_diameter_deg: float = 0.5) -> np.ndarray:
+    """Lat-long (z up) sky: zenith-to-horizon gradient, dim ground, sun disc.
+    Texel (x, y) covers u = (x + 0.5)/W, v = (y + 0.5)/H with the reference's
+    mapping u = frac(atan2(d.y, d.x)/(2 pi) + 0.5), v = acos(d.z)/pi."""
+    u = (np.arange(width) + 0.5) / width
+    v = (np.arange(height) + 0.5) / height
+    phi = (u - 0.5) * 2.0 * np.pi
+    theta = v * np.pi
+    st, ct = np.sin(theta)[:, None], np.cos(theta)[:, None]
+    d = np.stack([st * np.cos(phi)[None, :], st * np.sin(phi)[None, :], np.broadcast_to(ct, (height, width))], axis=-1)
+    elev = np.clip(d[..., 2], -1.0, 1.0)
+    zenith = np.array([0.15, 0.35, 0.85])
+    horizon = np.array([0.75, 0.85, 1.0])
+    ground = np.array([0.08, 0.07, 0.06])
+    t = np.clip(elev, 0.0, 1.0)[..., None] ** 0.5
+    sky = horizon * (1.0 - t) + zenith * t
+    rgb = np.where(elev[..., None] >= 0.0, sky, ground)
+    se, sa = np.radians(sun_elevation_deg), np.radians(sun_azimuth_deg)
+    sun = np.array([np.cos(se) * np.cos(sa), np.cos(se) * np.sin(sa), np.sin(se)])
+    cosang = d @ sun
+    inside = cosang >= np.cos(np.radians(sun_diameter_deg * 0.5))
+    rgb[inside] = sun_radiance
+    out = np.ones((height, width, 4), dtype=np.float32)
+    out[..., :3] = rgb
+    return out
+
+
+def load(renderer: Renderer, scene: Scene):
+    """Creates the scene's materials and meshes through the C API."""
+    mat_ids = [renderer.add_material(m, f"m{i}") for i, m in enumerate(scene.materials)]
+    for mesh in scene.meshes:
+        idx = renderer.add_mesh(mesh.vertices, mesh.indices)
+        renderer.set_mesh_material(idx, mat_ids[mesh.material])
+    pos, target, up, vfov = scene.camera
+    renderer.set_camera(pos, target, up, vfov)
+    if scene.environment is not None:
+        tex = renderer.add_texture_rgba32f(scene.environment)
+        renderer.set_environment_texture(tex)
+        renderer.set_environment_light((1.0, 1.0, 1.0), 1.0)
+    else:
+        renderer.set_environment_light(scene.environment_color, scene.environment_strength)
