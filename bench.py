@@ -1,0 +1,240 @@
+"""Throughput benchmark of the gfx950 path tracer on the 1080p spectral workload.
+
+Workload: a geodesic icosphere at frequency 229 (1,048,820 triangles) displaced
+by seeded 3-octave value noise, split into 64 material IDs (GGX roughness
+U(0.05,1), sRGB albedo U(0.05,0.95), 20% dielectric IOR 1.5), lit by a
+synthetic 2048x1024 RGBA32F sky with a 0.5 degree sun of radiance 1e4;
+1920x1080, 64 samples per pixel per frame, path depth 12 (Russian roulette
+from depth 4), hero-wavelength spectral mode, PCG32 streams with seed 1.
+
+A step is one VKRT_draw: one full frame of 1920*1080*64 paths traced through
+the wavefront pipeline and accumulated. The metric counts bounces: one
+extension ray per path per depth iteration (the reference's depth loop,
+src/shaders/integrator/path/spectral_hero/integrator.slang:25-27), counted on
+the device with an atomic counter.
+
+Prints ONE JSON line on stdout; progress goes to stderr.
+
+    python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--no-cpu-baseline]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0  # MI355X HBM3E, published peak
+
+# Algorithmic bytes per bounce (one fixed formula for the whole pipeline):
+#   B_ray   ray write 32 + ray read 32 + hit write 20 + hit read 20
+#   B_hit   3 x u32 indices + 3 x 48 B ShaderVertex
+#   B_state hero path state read + write
+#   B_nee   per shadow ray: 2x32 ray + 2x4 result + 2x(4+4) alias + 48 EmissiveTriangle + 32 EmissiveMesh
+#   B_trav  BVH nodes fetched x node bytes + triangle records fetched x record bytes (instrumented traversal)
+#   B_accum 72 B per pixel per frame
+B_RAY = 104
+B_HIT = 156
+B_STATE = 192
+B_NEE = 170
+B_ACCUM_PER_PIXEL = 72
+NODE_BYTES = 128  # 4-wide node: per-axis child bounds 96 B + 4 child references 16 B + pad
+TRI_BYTES = 48    # triangle record: 3 vertices x float3 + global triangle id
+# The traversal kernel's own share of B_ray: ray read 32 B + hit write 20 B.
+B_EXTEND_IO = 32 + 20
+
+WIDTH, HEIGHT, SPP, MAX_DEPTH, MIN_DEPTH = 1920, 1080, 64, 12, 4
+CPU_ROWS = 512  # centred band of full-width rows for the CPU baseline sample
+
+
+def log(msg: str):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def parse_args():
+    p = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=10)
+    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--no-cpu-baseline", action="store_true", help="skip the CPU oracle timing leg")
+    p.add_argument("--cpu-rows", type=int, default=CPU_ROWS)
+    return p.parse_args()
+
+
+def build_workload():
+    import vkrt_amd as V
+    from vkrt_amd import scenes
+
+    t0 = time.time()
+    scene = scenes.displaced_sphere(229)
+    scene.environment = scenes.sky_environment(2048, 1024)
+    r = V.Renderer(WIDTH, HEIGHT)
+    scenes.load(r, scene)
+    r.configure(spp=SPP, max_depth=MAX_DEPTH, min_depth=MIN_DEPTH, mode=V.RENDER_MODE_SPECTRAL,
+                spectral=V.SPECTRAL_HERO, rng=V.RNG_PCG32, seed=1)
+    r.start(WIDTH, HEIGHT, 0)
+    log(f"scene ready in {time.time() - t0:.1f} s: {scene.triangle_count} triangles")
+    return r, scene
+
+
+def cpu_baseline(renderer, rows: int):
+    """Times the CPU oracle (a C restatement of the reference integrator) on a
+    centred band of `rows` full-width rows of the same frame, 64 spp."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    threads = max(1, min(16, os.cpu_count() or 1))
+    buffers = renderer.export()
+    t0 = time.time()
+    osc = O.OracleScene(buffers)
+    build_s = time.time() - t0
+    r0 = HEIGHT // 2 - rows // 2
+    accum = np.zeros((HEIGHT, WIDTH, 4), dtype=np.float32)
+    t0 = time.time()
+    _, bounces = osc.render(WIDTH, HEIGHT, accum=accum, rows=(r0, r0 + rows), threads=threads)
+    dt = time.time() - t0
+    osc.close()
+    log(f"cpu oracle: {bounces} bounces in {dt:.2f} s on {threads} threads (BVH build {build_s:.1f} s, excluded)")
+    return {
+        "value": bounces / dt / 1e9,
+        "unit": "Gbounces/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"rows {r0}-{r0 + rows - 1} of the 1920x1080 frame, all {WIDTH} columns, {SPP} spp, depth "
+                  f"{MAX_DEPTH}, hero spectral, same scene and seeds; FP32 C oracle with OpenMP; "
+                  f"{bounces} bounces in {dt:.2f} s; oracle BVH build ({build_s:.1f} s) excluded",
+    }
+
+
+def main():
+    args = parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}, WORLD_SIZE {world}); "
+              "the path shards by sample range with one all-reduce of the accumulation buffer per frame, "
+              "which this build does not run.", file=sys.stderr, flush=True)
+        sys.exit(2)
+
+    import vkrt_amd as V
+
+    r, scene = build_workload()
+    log(f"warmup {args.warmup} frame(s) (the first uploads the scene and builds the BVH)")
+    for _ in range(args.warmup):
+        r.draw()
+    st = r.stats()
+    log(f"BVH build {st.bvhBuildMs:.1f} ms, {st.bvhNodeCount} nodes, depth {st.bvhMaxDepth}")
+
+    r.synchronize()
+    t0 = time.perf_counter()
+    bounces = shadow = paths = 0
+    ext_ms = 0.0
+    ext_launches = 0
+    ext_rays = 0
+    for i in range(args.steps):
+        r.draw()
+        s = r.stats()
+        bounces += s.extensionRays
+        shadow += s.shadowRays
+        paths += s.paths
+        ext_ms += s.extendMs
+        ext_launches += s.extendLaunches
+    r.synchronize()
+    elapsed = time.perf_counter() - t0
+    ext_rays = bounces
+    value = bounces / elapsed / 1e9
+    ms_per_step = elapsed * 1e3 / args.steps
+    log(f"timed {args.steps} frames: {elapsed:.3f} s, {bounces} bounces, {shadow} shadow rays -> {value:.3f} Gbounces/s")
+
+    acc = r.accumulation()
+    expected = SPP * (args.warmup + args.steps)
+    if not np.isfinite(acc).all() or int(acc[HEIGHT // 2, WIDTH // 2, 3]) != expected:
+        log(f"accumulation check failed: finite={np.isfinite(acc).all()} samples={acc[HEIGHT // 2, WIDTH // 2, 3]}")
+        sys.exit(1)
+
+    # One more frame with the instrumented traversal for nodes and triangles per ray.
+    r.set_traversal_statistics(True)
+    r.draw()
+    s = r.stats()
+    r.set_traversal_statistics(False)
+    nodes_per_ray = s.nodeVisits / max(1, s.extensionRays)
+    tris_per_ray = s.triangleTests / max(1, s.extensionRays)
+    b_trav = nodes_per_ray * NODE_BYTES + tris_per_ray * TRI_BYTES
+    log(f"traversal: {nodes_per_ray:.2f} nodes/ray, {tris_per_ray:.2f} triangles/ray -> {b_trav:.0f} B/ray")
+
+    # Dominant kernel: the closest-hit traversal (kExtend), timed with HIP events on its stream.
+    ext_bytes = ext_rays * (B_EXTEND_IO + b_trav)
+    ext_avg_ms = ext_ms / max(1, ext_launches)
+    achieved = ext_bytes / (ext_ms * 1e-3) / 1e9 if ext_ms > 0 else 0.0
+    traffic = None
+    roofline = {
+        "bound": "hbm",
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBPS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBPS, 4),
+        "traffic": traffic,
+        "kernel": "kExtend<false> (closest-hit traversal)",
+        "launches": ext_launches,
+        "avg_launch_ms": round(ext_avg_ms, 4),
+        "bytes_per_ray": round(B_EXTEND_IO + b_trav, 1),
+        "nodes_per_ray": round(nodes_per_ray, 3),
+        "tris_per_ray": round(tris_per_ray, 3),
+    }
+    # Whole-pipeline algorithmic bytes per bounce over wall time.
+    bounces_per_px = bounces / (WIDTH * HEIGHT * args.steps)
+    b_bounce = (B_RAY + B_HIT + B_STATE + b_trav + B_NEE * shadow / max(1, bounces)
+                + B_ACCUM_PER_PIXEL / max(bounces_per_px, 1e-9))
+    pipeline = {
+        "bytes_per_bounce": round(b_bounce, 1),
+        "achieved": round(b_bounce * bounces / elapsed / 1e9, 1),
+        "frac": round(b_bounce * bounces / elapsed / 1e9 / HBM_PEAK_GBPS, 4),
+        "unit": "GB/s",
+        "extend_share_of_time": round(ext_ms * 1e-3 / elapsed, 3),
+    }
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(r, args.cpu_rows)
+
+    line = {
+        "metric": "Gbounces/s at 1080p spectral",
+        "value": round(value, 4),
+        "unit": "Gbounces/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (procedural displaced icosphere and HDR sky generated in-process)",
+        "config": {
+            "workload": "displaced icosphere 1,048,820 triangles, 64 materials (20% dielectric IOR 1.5), "
+                        "2048x1024 HDR sky + 0.5 deg sun 1e4; 1920x1080, 64 spp, depth 12, hero spectral, PCG32 seed 1",
+            "width": WIDTH,
+            "height": HEIGHT,
+            "spp": SPP,
+            "max_depth": MAX_DEPTH,
+            "triangles": scene.triangle_count,
+            "paths_per_step": paths // max(1, args.steps),
+            "bounces_per_step": bounces // max(1, args.steps),
+            "shadow_rays_per_step": shadow // max(1, args.steps),
+        },
+        "roofline": roofline,
+        "pipeline": pipeline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    r.close()
+
+
+if __name__ == "__main__":
+    main()

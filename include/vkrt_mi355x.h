@@ -54,8 +54,21 @@ typedef struct VKRT_TraceStats {
     uint32_t bvhLeafTriangles;
     uint32_t bvhMaxDepth;
     uint32_t triangleCount;
+    uint32_t shadeLaunches;
+    uint32_t shadowLaunches;
+    uint64_t nodeVisits;    /* closest-hit traversal only, when statistics are enabled */
+    uint64_t triangleTests; /* closest-hit traversal only, when statistics are enabled */
 } VKRT_TraceStats;
 VKRT_Result VKRT_getTraceStats(const VKRT* vkrt, VKRT_TraceStats* outStats);
+
+/* Blocks until every kernel and copy the instance has queued on its device has
+ * finished (hipDeviceSynchronize); the reference waits on its frame fences
+ * instead (src/core/api/frame.c:74-105). */
+VKRT_Result VKRT_deviceSynchronize(VKRT* vkrt);
+
+/* Counts BVH nodes visited and triangles tested by every closest-hit query of
+ * the following frames (an instrumented traversal; slower, for measurement). */
+VKRT_Result VKRT_setTraversalStatistics(VKRT* vkrt, uint8_t enabled);
 
 /* Host copies of the packed scene buffers exactly as the kernels consume them
  * (the reference's descriptor bindings 10-24, src/shaders/scene/resources.slang:7-59).

@@ -52,6 +52,7 @@ def lib():
         g("oracle_trace_rays").argtypes = [P, P, f32p, C.c_uint64, u32p, u32p, f32p, f32p, C.c_int]
         g("oracle_rand_sequence").argtypes = [C.c_uint32, C.c_uint64, C.c_int, C.c_int, C.c_uint32, C.c_uint32,
                                               C.c_uint32, rp]
+        g("oracle_pcg32_raw").argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, u32p]
         g("oracle_hash").restype = C.c_uint32
         g("oracle_hash").argtypes = [C.c_uint32]
         g("oracle_camera_ray").argtypes = [P, C.c_int, C.c_int, rt, rt, rp]
@@ -161,6 +162,12 @@ def rand_sequence(policy, seed, px, py, frame, sample, n, precision=32):
     dt, sfx = _dtype(precision)
     out = np.zeros(n, dtype=dt)
     getattr(lib(), "oracle_rand_sequence" + sfx)(policy, seed, px, py, frame, sample, n, ptr(out))
+    return out
+
+
+def pcg32_raw(init_state, init_seq, n):
+    out = np.zeros(n, dtype=np.uint32)
+    lib().oracle_pcg32_raw_f32(C.c_uint64(init_state), C.c_uint64(init_seq), n, ptr(out))
     return out
 
 

@@ -1,0 +1,172 @@
+"""Host-side logic of the C library, without a GPU: vertex quantisation
+(src/core/utility/packing.c:92-156), mesh transforms (src/core/scene/transform.c),
+camera matrices (src/core/scene/camera.c:128-143) and the procedural scenes."""
+
+import ctypes as C
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+import vkrt_amd as V
+from vkrt_amd import scenes
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def oct_decode(packed, bits=16):
+    """Octahedral decoding (Cigolle et al. 2014, "A survey of efficient
+    representations for independent unit vectors")."""
+    if bits == 16:
+        x = float(np.array(packed & 0xFFFF, np.uint16).view(np.int16)) / 32767.0
+        y = float(np.array((packed >> 16) & 0xFFFF, np.uint16).view(np.int16)) / 32767.0
+    else:
+        def s15(v):
+            v = int(v) & 0x7FFF
+            return (v - 0x8000 if v & 0x4000 else v) / 16383.0
+        x, y = s15(packed), s15(packed >> 15)
+    z = 1.0 - abs(x) - abs(y)
+    if z < 0:
+        x, y = (1.0 - abs(y)) * math.copysign(1.0, x), (1.0 - abs(x)) * math.copysign(1.0, y)
+    n = np.array([x, y, z])
+    return n / np.linalg.norm(n)
+
+
+def exported_vertices(verts):
+    r = V.Renderer(init_device=False)
+    idx = np.arange(len(verts), dtype=np.uint32)
+    idx = idx[: len(idx) // 3 * 3].reshape(-1, 3)
+    r.add_mesh(verts, idx)
+    b = r.export()
+    out = [b.vertices[i] for i in range(b.vertexCount)]
+    return r, out
+
+
+def test_packed_normals_tangents_colours_round_trip():
+    rng = np.random.default_rng(0)
+    n = rng.normal(size=(300, 3))
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    pos = rng.normal(size=(300, 3)).astype(np.float32)
+    col = rng.uniform(0, 1, (300, 4)).astype(np.float32)
+    verts = V.make_vertices(pos, n.astype(np.float32), col)
+    verts[150:, 11] = -1.0  # tangent handedness
+    r, sv = exported_vertices(verts)
+    for i, s in enumerate(sv):
+        assert np.allclose(np.array(s.position[:3]), pos[i])
+        dn = oct_decode(s.packedNormal)
+        assert np.degrees(np.arccos(np.clip(dn @ n[i], -1, 1))) < 0.01
+        t = verts[i, 8:11].astype(np.float64)
+        dt = oct_decode(s.packedTangent, bits=15)
+        assert np.degrees(np.arccos(np.clip(dt @ (t / np.linalg.norm(t)), -1, 1))) < 0.02
+        assert bool(s.packedTangent >> 31) == (verts[i, 11] < 0)
+        rgba = [(s.packedColor >> (8 * k)) & 0xFF for k in range(4)]
+        assert np.allclose(np.array(rgba) / 255.0, col[i], atol=0.5 / 255 + 1e-6)
+    r.close()
+
+
+def transform_matrix(pos, rot, scale):
+    out = (C.c_float * 16)()
+    V.library().VKRT_buildMeshTransformMatrix((C.c_float * 3)(*pos), (C.c_float * 3)(*rot),
+                                              (C.c_float * 3)(*scale), out)
+    return np.array(out, dtype=np.float64).reshape(4, 4).T  # column-major storage
+
+
+def test_transform_build_and_decompose_round_trip():
+    rng = np.random.default_rng(1)
+    lib = V.library()
+    for _ in range(50):
+        pos = rng.uniform(-5, 5, 3)
+        rot = rng.uniform(-80, 80, 3)
+        scale = rng.uniform(0.2, 3.0, 3)
+        m = transform_matrix(pos, rot, scale)
+        assert np.allclose(m[:3, 3], pos, atol=1e-5)
+        assert np.allclose(m[3], (0, 0, 0, 1))
+        assert np.linalg.det(m[:3, :3]) == pytest.approx(np.prod(scale), rel=1e-4)
+        cols = np.linalg.norm(m[:3, :3], axis=0)
+        assert np.allclose(cols, scale, rtol=1e-4)
+        mc = (C.c_float * 16)(*m.T.reshape(-1))
+        p, rr, s = (C.c_float * 3)(), (C.c_float * 3)(), (C.c_float * 3)()
+        lib.VKRT_decomposeMeshTransform(mc, p, rr, s)
+        assert np.allclose(np.array(p), pos, atol=1e-4)
+        assert np.allclose(np.array(s), scale, rtol=1e-4)
+        assert np.allclose(transform_matrix(np.array(p), np.array(rr), np.array(s)), m, atol=1e-4)
+
+
+def test_transform_moves_world_triangles():
+    r = V.Renderer(init_device=False)
+    verts = V.make_vertices(np.array([[1, 0, 0], [0, 1, 0], [0, 0, 1]], np.float32),
+                            np.tile([[0, 0, 1]], (3, 1)).astype(np.float32))
+    mesh = r.add_mesh(verts, np.array([[0, 1, 2]], np.uint32))
+    m = transform_matrix((1.0, 2.0, 3.0), (0.0, 0.0, 90.0), (2.0, 2.0, 2.0))
+    r.set_mesh_transform_matrix(mesh, m.T.reshape(-1).astype(np.float32))
+    b = r.export()
+    tri = np.ctypeslib.as_array(b.worldTriangles, shape=(9,)).reshape(3, 3)
+    want = (m @ np.array([[1, 0, 0, 1], [0, 1, 0, 1], [0, 0, 1, 1]], dtype=np.float64).T).T[:, :3]
+    assert np.allclose(tri, want, atol=1e-5)
+    # +90 degrees about z maps +x to +y
+    assert np.allclose(tri[0], (1.0, 4.0, 3.0), atol=1e-5)
+    r.close()
+
+
+def test_camera_matrices_invert_lookat_and_perspective():
+    r = V.Renderer(init_device=False)
+    r.start(80, 40, 0)
+    r.set_camera((1.0, -3.0, 0.5), (0.0, 0.0, 0.0), (0.0, 0.0, 1.0), 50.0)
+    sd = r.export().sceneData.contents
+    view_inv = np.array(sd.viewInverse, dtype=np.float64).reshape(4, 4).T
+    assert np.allclose(view_inv[:3, 3], (1.0, -3.0, 0.5), atol=1e-6)  # camera position
+    fwd = -view_inv[:3, 2]  # right-handed view looks down -z
+    want = -np.array([1.0, -3.0, 0.5])
+    assert np.allclose(fwd, want / np.linalg.norm(want), atol=1e-6)
+    proj_inv = np.array(sd.projInverse, dtype=np.float64).reshape(4, 4).T
+    proj = np.linalg.inv(proj_inv)
+    f = 1.0 / math.tan(math.radians(25.0))
+    assert proj[0, 0] == pytest.approx(f / 2.0, rel=1e-5)  # aspect 80/40
+    assert proj[1, 1] == pytest.approx(-f, rel=1e-5)       # Vulkan y flip
+    assert list(sd.viewportRect) == [0, 0, 80, 40]
+    r.close()
+
+
+def test_scene_sizes_match_the_fixture():
+    g = json.load(open(os.path.join(GOLDEN, "scenes.json")))
+    cb = scenes.cornell_box()
+    assert cb.triangle_count == g["cornell_box"]["triangles"]
+    assert len(cb.meshes) == g["cornell_box"]["meshes"]
+    verts, tris = scenes.geodesic_sphere(229)
+    assert len(tris) == g["displaced_sphere_229"]["triangles"]
+    assert len(verts) == g["displaced_sphere_229"]["vertices"]
+
+
+def test_geodesic_sphere_is_closed_and_consistently_wound():
+    verts, tris = scenes.geodesic_sphere(7)
+    assert np.allclose(np.linalg.norm(verts, axis=1), 1.0)
+    edges = np.concatenate([tris[:, [0, 1]], tris[:, [1, 2]], tris[:, [2, 0]]])
+    # every directed edge appears once and its reverse once: closed, oriented 2-manifold
+    fwd = {tuple(e) for e in edges.tolist()}
+    assert len(fwd) == len(edges)
+    assert all((b, a) in fwd for a, b in fwd)
+    assert len(verts) - len(edges) // 2 + len(tris) == 2  # Euler characteristic
+
+
+def test_displaced_sphere_materials():
+    s = scenes.displaced_sphere(20)
+    assert len(s.materials) == 64
+    trans = [m.transmission for m in s.materials]
+    assert 0.1 < np.mean(np.array(trans) > 0) < 0.3
+    for m in s.materials:
+        assert 0.05 <= m.roughness <= 1.0
+        assert all(0.05 <= c <= 0.95 for c in m.baseColor)
+        if m.transmission > 0:
+            assert m.ior == pytest.approx(1.5)
+    r = np.linalg.norm(np.concatenate([m.vertices[:, :3] for m in s.meshes]), axis=1)
+    assert r.min() > 0.85 and r.max() < 1.15  # displacement amplitude 0.1 * radius
+
+
+def test_sky_environment_sun():
+    env = scenes.sky_environment()
+    assert env.shape == (1024, 2048, 4) and env.dtype == np.float32
+    assert env[..., :3].max() == pytest.approx(1.0e4)
+    # a 0.5 degree disc covers about pi/4 * (0.5 / 0.176)^2 texels at 2048x1024, more near the poles
+    assert 4 <= (env[..., 0] >= 1.0e4).sum() <= 20

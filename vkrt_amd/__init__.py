@@ -272,6 +272,10 @@ class TraceStats(C.Structure):
         ("bvhLeafTriangles", C.c_uint32),
         ("bvhMaxDepth", C.c_uint32),
         ("triangleCount", C.c_uint32),
+        ("shadeLaunches", C.c_uint32),
+        ("shadowLaunches", C.c_uint32),
+        ("nodeVisits", C.c_uint64),
+        ("triangleTests", C.c_uint64),
     ]
 
 
@@ -393,6 +397,8 @@ def library() -> C.CDLL:
         "VKRT_traceRays": (C.c_int, [P, fp, sz, up, up, fp, fp]),
         "VKRT_traceShadowRays": (C.c_int, [P, fp, sz, up]),
         "VKRT_tracePrimaryHits": (C.c_int, [P, u32, up, up]),
+        "VKRT_setTraversalStatistics": (C.c_int, [P, C.c_uint8]),
+        "VKRT_deviceSynchronize": (C.c_int, [P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -577,6 +583,12 @@ class Renderer:
         _ok(self.lib.VKRT_readOutputImage(self.h, out.ctypes.data_as(C.POINTER(C.c_uint16)), out.size),
             "VKRT_readOutputImage")
         return out
+
+    def synchronize(self):
+        _ok(self.lib.VKRT_deviceSynchronize(self.h), "VKRT_deviceSynchronize")
+
+    def set_traversal_statistics(self, enabled: bool):
+        _ok(self.lib.VKRT_setTraversalStatistics(self.h, 1 if enabled else 0), "VKRT_setTraversalStatistics")
 
     def stats(self) -> TraceStats:
         s = TraceStats()

@@ -116,6 +116,12 @@ extern "C" void mrt_destroy(mrt_context* ctx) {
     delete ctx;
 }
 
+extern "C" int mrt_device_synchronize(mrt_context* ctx) {
+    if (!ctx) return -1;
+    hipSetDevice(ctx->device);
+    return check(hipDeviceSynchronize(), "device synchronize") ? 0 : -4;
+}
+
 extern "C" int mrt_device_name(const mrt_context* ctx, char* out, size_t len, uint32_t* vendorId,
                                uint32_t* driverVersion) {
     if (!ctx || !out || len == 0) return -1;

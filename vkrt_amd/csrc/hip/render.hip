@@ -297,8 +297,11 @@ __global__ void kPrepShadow(uint32_t* counters) {
 __global__ void kSetCount(uint32_t* counters, int slot, uint32_t value) { counters[slot] = value; }
 
 // Closest-hit traversal, persistent: each wavefront claims 64 consecutive rays at a time.
+// COUNT adds the number of BVH nodes visited and triangles tested to the frame statistics.
+template <bool COUNT>
 __global__ __launch_bounds__(kTraceBlock) void kExtend(SceneView sv, const float4* ro, const float4* rd, float4* hits,
                                                       uint32_t* counters, int countSlot, uint32_t* gstack) {
+    uint32_t nodeVisits = 0, triTests = 0;
     __shared__ uint32_t lds[kLdsStack * kTraceBlock];
     TraversalStack st;
     st.lds = lds + threadIdx.x;
@@ -315,11 +318,16 @@ __global__ __launch_bounds__(kTraceBlock) void kExtend(SceneView sv, const float
         if (i < n) {
             float4 a = ro[i], b = rd[i];
             TraversalRay r = makeTraversalRay(xyz(a), xyz(b));
-            Hit h = traceClosest(sv, r, kRayTMin, b.w, st);
+            Hit h = traceClosest(sv, r, kRayTMin, b.w, st, COUNT ? &nodeVisits : nullptr, COUNT ? &triTests : nullptr);
             hits[i] = make_float4(h.t, __uint_as_float(h.tri), h.u, h.v);
         }
     }
     if (st.dropped) atomicAdd(&counters[kCntDropped], st.dropped);
+    if (COUNT) {
+        unsigned long long* stats = reinterpret_cast<unsigned long long*>(counters + kCntStats);
+        atomicAdd(&stats[2], (unsigned long long)nodeVisits);
+        atomicAdd(&stats[3], (unsigned long long)triTests);
+    }
 }
 
 DEV uint32_t classifyOccluder(const SceneView& sv, uint32_t tri) {
@@ -857,8 +865,24 @@ static PathView pathView(mrt_context* ctx) {
     return v;
 }
 
+struct StageEvents {
+    std::vector<hipEvent_t> pool;
+    std::vector<int> stage;  // per event pair: 0 extend, 1 shade, 2 shadow
+    size_t used = 0;
+    bool begin(hipStream_t s, int which) {
+        if (used + 2 > pool.size()) return false;
+        hipEventRecord(pool[used], s);
+        stage.push_back(which);
+        return true;
+    }
+    void end(hipStream_t s) {
+        hipEventRecord(pool[used + 1], s);
+        used += 2;
+    }
+};
+
 template <int MODE>
-static bool launchChunk(mrt_context* ctx, FrameParams fp, std::vector<hipEvent_t>& extEvents, size_t& evUsed) {
+static bool launchChunk(mrt_context* ctx, FrameParams fp, StageEvents& ev, bool countTraversal) {
     hipStream_t s = ctx->stream;
     PathBuffers& b = ctx->paths;
     PathView pv = pathView(ctx);
@@ -879,19 +903,23 @@ static bool launchChunk(mrt_context* ctx, FrameParams fp, std::vector<hipEvent_t
         int nxt = 1 - cur;
         fp.depth = depth;
         hipLaunchKernelGGL(kPrepDepth, dim3(1), dim3(1), 0, s, cnt, cur, nxt);
-        bool timed = evUsed + 2 <= extEvents.size();
-        if (timed) hipEventRecord(extEvents[evUsed], s);
-        hipLaunchKernelGGL(kExtend, dim3(ctx->traceGrid), dim3(kTraceBlock), 0, s, sv, qv[cur].o, qv[cur].d,
-                           b.hits.ptr, cnt, cur, ctx->traversalStack.ptr);
-        if (timed) {
-            hipEventRecord(extEvents[evUsed + 1], s);
-            evUsed += 2;
-        }
+        bool timed = ev.begin(s, 0);
+        if (countTraversal)
+            hipLaunchKernelGGL(kExtend<true>, dim3(ctx->traceGrid), dim3(kTraceBlock), 0, s, sv, qv[cur].o, qv[cur].d,
+                               b.hits.ptr, cnt, cur, ctx->traversalStack.ptr);
+        else
+            hipLaunchKernelGGL(kExtend<false>, dim3(ctx->traceGrid), dim3(kTraceBlock), 0, s, sv, qv[cur].o, qv[cur].d,
+                               b.hits.ptr, cnt, cur, ctx->traversalStack.ptr);
+        if (timed) ev.end(s);
+        timed = ev.begin(s, 1);
         hipLaunchKernelGGL(kShade<MODE>, dim3(blocksFor(fp.count)), dim3(kBlock), 0, s, fp, sv, pv, qv[cur], b.hits.ptr,
                            qv[nxt], shv, cnt, cur, nxt);
+        if (timed) ev.end(s);
         hipLaunchKernelGGL(kPrepShadow, dim3(1), dim3(1), 0, s, cnt);
+        timed = ev.begin(s, 2);
         hipLaunchKernelGGL(kShadow<MODE>, dim3(ctx->traceGrid), dim3(kTraceBlock), 0, s, sv, pv, shv, cnt,
                            ctx->traversalStack.ptr);
+        if (timed) ev.end(s);
         cur = nxt;
     }
     hipLaunchKernelGGL(kFinalize<MODE>, dim3(blocksFor(fp.count)), dim3(kBlock), 0, s, fp.count, pv, b.sampleRad.ptr);
@@ -986,20 +1014,22 @@ extern "C" int mrt_render_frame(mrt_context* ctx, const SceneData* scene, const 
     fp.spec.scaleOffset = scene->rgb2specSRGB.scaleOffset;
     fp.spec.dataOffset = scene->rgb2specSRGB.dataOffset;
 
-    static std::vector<hipEvent_t> extEvents;
-    if (extEvents.empty()) {
-        extEvents.resize(2048);
-        for (auto& e : extEvents) hipEventCreate(&e);
+    static StageEvents ev;
+    if (ev.pool.empty()) {
+        ev.pool.resize(8192);
+        for (auto& e : ev.pool) hipEventCreate(&e);
     }
-    size_t evUsed = 0;
+    ev.used = 0;
+    ev.stage.clear();
+    const bool countTraversal = prm->traversalStatistics != 0u;
     hipEventRecord(ctx->evBegin, s);
     uint32_t chunks = 0;
     for (uint64_t begin = 0; begin < totalPaths; begin += chunk) {
         fp.pathBegin = begin;
         fp.count = (uint32_t)std::min<uint64_t>(chunk, totalPaths - begin);
-        bool ok = mode == kModeRgb    ? launchChunk<kModeRgb>(ctx, fp, extEvents, evUsed)
-                  : mode == kModeHero ? launchChunk<kModeHero>(ctx, fp, extEvents, evUsed)
-                                      : launchChunk<kModeSingle>(ctx, fp, extEvents, evUsed);
+        bool ok = mode == kModeRgb    ? launchChunk<kModeRgb>(ctx, fp, ev, countTraversal)
+                  : mode == kModeHero ? launchChunk<kModeHero>(ctx, fp, ev, countTraversal)
+                                      : launchChunk<kModeSingle>(ctx, fp, ev, countTraversal);
         if (!ok) return -4;
         chunks++;
     }
@@ -1016,22 +1046,31 @@ extern "C" int mrt_render_frame(mrt_context* ctx, const SceneData* scene, const 
         return -5;
     }
     if (stats) {
-        unsigned long long st64[2];
+        unsigned long long st64[4];
         memcpy(st64, hostCounters + kCntStats, sizeof(st64));
         stats->extensionRays = st64[0];
         stats->shadowRays = st64[1];
+        stats->nodeVisits = st64[2];
+        stats->triangleTests = st64[3];
         stats->paths = totalPaths;
         float ms = 0.0f;
         hipEventElapsedTime(&ms, ctx->evBegin, ctx->evEnd);
         stats->traceMs = ms;
-        float ext = 0.0f;
-        for (size_t e = 0; e + 1 < evUsed; e += 2) {
+        float stageMs[3] = {0.0f, 0.0f, 0.0f};
+        uint32_t stageN[3] = {0u, 0u, 0u};
+        for (size_t e = 0; e + 1 < ev.used; e += 2) {
             float t = 0.0f;
-            hipEventElapsedTime(&t, extEvents[e], extEvents[e + 1]);
-            ext += t;
+            hipEventElapsedTime(&t, ev.pool[e], ev.pool[e + 1]);
+            int which = ev.stage[e / 2];
+            stageMs[which] += t;
+            stageN[which]++;
         }
-        stats->extendMs = ext;
-        stats->extendLaunches = (uint32_t)(evUsed / 2);
+        stats->extendMs = stageMs[0];
+        stats->shadeMs = stageMs[1];
+        stats->shadowMs = stageMs[2];
+        stats->extendLaunches = stageN[0];
+        stats->shadeLaunches = stageN[1];
+        stats->shadowLaunches = stageN[2];
         (void)wall0;
     }
     return 0;

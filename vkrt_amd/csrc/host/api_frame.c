@@ -133,6 +133,11 @@ static VKRT_Result syncScene(VKRT* vkrt, int requireDevice) {
     if (r != VKRT_SUCCESS) return r;
     if (!vkrt->device) {
         if (requireDevice) return VKRT_ERROR_INITIALIZATION_FAILED;
+        /* packed buffers stay valid until the next change; device initialisation
+         * forces a full upload of whatever is prepared by then */
+        vkrt->geometryUploaded = vkrt->geometryRevision;
+        vkrt->materialUploaded = vkrt->materialRevision;
+        vkrt->environmentUploaded = vkrt->environmentRevision;
         return VKRT_SUCCESS;
     }
     PackedScene* p = &vkrt->packed;
@@ -197,6 +202,7 @@ static void fillRenderParams(const VKRT* vkrt, mrt_render_params* rp) {
     rp->rngSeed = vkrt->rngSeed;
     rp->envImportance = vkrt->envImportance;
     rp->hasTransmissive = vkrt->packed.hasTransmissive;
+    rp->traversalStatistics = vkrt->traversalStatistics;
 }
 
 VKRT_Result VKRT_trace(VKRT* vkrt) {
@@ -223,6 +229,10 @@ VKRT_Result VKRT_trace(VKRT* vkrt) {
     vkrt->stats.shadowMs = rs.shadowMs;
     vkrt->stats.shadeMs = rs.shadeMs;
     vkrt->stats.extendLaunches = rs.extendLaunches;
+    vkrt->stats.shadeLaunches = rs.shadeLaunches;
+    vkrt->stats.shadowLaunches = rs.shadowLaunches;
+    vkrt->stats.nodeVisits = rs.nodeVisits;
+    vkrt->stats.triangleTests = rs.triangleTests;
     vkrt->renderStatus.renderTimeMs = rs.traceMs;
     vkrt->frameTraced = 1;
     return VKRT_SUCCESS;
@@ -305,6 +315,18 @@ VKRT_Result VKRT_getRenderStatus(const VKRT* vkrt, VKRT_RenderStatusSnapshot* ou
 VKRT_Result VKRT_getTraceStats(const VKRT* vkrt, VKRT_TraceStats* out) {
     if (!vkrt || !out) return VKRT_ERROR_INVALID_ARGUMENT;
     *out = vkrt->stats;
+    return VKRT_SUCCESS;
+}
+
+VKRT_Result VKRT_deviceSynchronize(VKRT* vkrt) {
+    if (!vkrt) return VKRT_ERROR_INVALID_ARGUMENT;
+    if (!vkrt->device) return VKRT_ERROR_INITIALIZATION_FAILED;
+    return mrt_device_synchronize(vkrt->device) == 0 ? VKRT_SUCCESS : deviceError();
+}
+
+VKRT_Result VKRT_setTraversalStatistics(VKRT* vkrt, uint8_t enabled) {
+    if (!vkrt) return VKRT_ERROR_INVALID_ARGUMENT;
+    vkrt->traversalStatistics = enabled ? 1u : 0u;
     return VKRT_SUCCESS;
 }
 

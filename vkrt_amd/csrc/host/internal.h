@@ -104,6 +104,7 @@ struct VKRT {
     uint32_t rngPolicy;
     uint64_t rngSeed;
     uint8_t envImportance;
+    uint8_t traversalStatistics;
 
     VKRT_RenderStatusSnapshot renderStatus;
     uint8_t frameTraced;

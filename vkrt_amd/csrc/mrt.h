@@ -27,6 +27,7 @@ const char* mrt_last_error(void);
 int mrt_device_count(void);
 int mrt_create(int deviceIndex, mrt_context** out);
 void mrt_destroy(mrt_context* ctx);
+int mrt_device_synchronize(mrt_context* ctx);
 int mrt_device_name(const mrt_context* ctx, char* out, size_t len, uint32_t* vendorId, uint32_t* driverVersion);
 
 typedef struct mrt_geometry_desc {
@@ -85,6 +86,7 @@ typedef struct mrt_render_params {
     uint32_t envImportance;
     uint32_t hasTransmissive; /* any material with transmission > 0 */
     uint32_t resetAccumulation;
+    uint32_t traversalStatistics;
 } mrt_render_params;
 
 typedef struct mrt_render_stats {
@@ -96,6 +98,10 @@ typedef struct mrt_render_stats {
     float shadowMs;
     float shadeMs;
     uint32_t extendLaunches;
+    uint32_t shadeLaunches;
+    uint32_t shadowLaunches;
+    uint64_t nodeVisits;
+    uint64_t triangleTests;
 } mrt_render_stats;
 
 /* One frame: samplesPerPixel paths for every viewport pixel, accumulated into
