@@ -65,7 +65,28 @@ def parse_args():
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--no-cpu-baseline", action="store_true", help="skip the CPU oracle timing leg")
     p.add_argument("--cpu-rows", type=int, default=CPU_ROWS)
+    p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                   help="only run device-to-device copies of MIB mebibytes (reference kernel for HBM counters)")
     return p.parse_args()
+
+
+def copy_calibration(mib: int):
+    """Copies a buffer of `mib` MiB device-to-device 5 times with the HIP
+    runtime's copy kernel, so that rocprofv3's FETCH_SIZE / WRITE_SIZE can be
+    scaled: each dispatch reads and writes exactly mib MiB."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(n)) == 0
+    for _ in range(5):
+        assert hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"copied {mib} MiB device-to-device 5 times")
 
 
 def build_workload():
@@ -122,7 +143,9 @@ def main():
               "which this build does not run.", file=sys.stderr, flush=True)
         sys.exit(2)
 
-    import vkrt_amd as V
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
 
     r, scene = build_workload()
     log(f"warmup {args.warmup} frame(s) (the first uploads the scene and builds the BVH)")
@@ -172,7 +195,14 @@ def main():
     ext_bytes = ext_rays * (B_EXTEND_IO + b_trav)
     ext_avg_ms = ext_ms / max(1, ext_launches)
     achieved = ext_bytes / (ext_ms * 1e-3) / 1e9 if ext_ms > 0 else 0.0
+    # Memory-fabric bytes per launch from the committed FETCH_SIZE / WRITE_SIZE
+    # passes (copy-calibrated), per ray, times this run's rays per launch.
     traffic = None
+    pmc_path = os.path.join(ROOT, "profiles", "pmc_traffic_1080p_spectral.json")
+    if os.path.exists(pmc_path):
+        with open(pmc_path) as f:
+            pmc = json.load(f)
+        traffic = round(pmc["kExtend"]["hbm_bytes_per_ray"] * ext_rays / max(1, ext_launches))
     roofline = {
         "bound": "hbm",
         "achieved": round(achieved, 1),

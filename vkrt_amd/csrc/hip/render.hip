@@ -38,8 +38,18 @@ constexpr uint64_t kMaxChunk = 1ull << 23;
 constexpr int kBlock = 256;
 
 // counters[] layout
-constexpr int kCntQueue0 = 0, kCntQueue1 = 1, kCntShadow = 2, kCntFetchExt = 3, kCntFetchShadow = 4,
-              kCntDropped = 5, kCntStats = 8;  // 64-bit stats at counters+8: [0] ext rays, [1] shadow rays
+constexpr int kCntQueue0 = 0, kCntQueue1 = 1, kCntShadow = 2, kCntDropped = 5,
+              kCntStats = 8;  // 64-bit stats at counters+8: [0] ext rays, [1] shadow rays, [2] nodes, [3] triangles
+// Work claims of the persistent traversal kernels. The batch space of a queue
+// is split into kFetchParts interleaved partitions, each with its own counter
+// on its own 256-byte line: device-scope atomics from the 8 XCDs then spread
+// over 8 addresses instead of serialising on one. A wave claims kFetchBatch
+// rays per atomic.
+constexpr int kFetchParts = 8;
+constexpr uint32_t kFetchStride = 64;  // uint32 slots between partition counters
+constexpr uint32_t kFetchBatch = 64;
+constexpr int kCntFetchExt = 64, kCntFetchShadow = kCntFetchExt + kFetchParts * kFetchStride;
+constexpr uint32_t kCounterWords = kCntFetchShadow + kFetchParts * kFetchStride;
 
 struct FrameParams {
     SceneData sd;
@@ -95,6 +105,16 @@ DEV uint32_t waveAppend(bool pred, uint32_t* counter) {
     if ((int)lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
     base = (uint32_t)__shfl((int)base, leader);
     return pred ? base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull)) : 0xFFFFFFFFu;
+}
+
+// First ray of the next batch of this wave's partition, or >= n once the
+// partition is exhausted. Wave-uniform; every lane must call it.
+DEV uint32_t claimBatch(uint32_t* fetchBase, uint32_t part, uint32_t n) {
+    uint32_t k = 0;
+    if ((threadIdx.x & 63u) == 0u) k = atomicAdd(&fetchBase[part * kFetchStride], 1u);
+    k = (uint32_t)__shfl((int)k, 0);
+    uint64_t first = ((uint64_t)k * kFetchParts + part) * kFetchBatch;
+    return first >= n ? n : (uint32_t)first;
 }
 
 // ------------------------------------------------------------ lights --
@@ -283,12 +303,14 @@ __global__ __launch_bounds__(kBlock) void kGenerate(FrameParams fp, PathView pv,
 }
 
 __global__ void kPrepDepth(uint32_t* counters, int cur, int nxt) {
-    unsigned long long* stats = reinterpret_cast<unsigned long long*>(counters + kCntStats);
-    stats[0] += counters[cur];
-    counters[nxt] = 0u;
-    counters[kCntShadow] = 0u;
-    counters[kCntFetchExt] = 0u;
-    counters[kCntFetchShadow] = 0u;
+    const uint32_t t = threadIdx.x;
+    if (t == 0) {
+        unsigned long long* stats = reinterpret_cast<unsigned long long*>(counters + kCntStats);
+        stats[0] += counters[cur];
+        counters[nxt] = 0u;
+        counters[kCntShadow] = 0u;
+    }
+    if (t < 2u * kFetchParts) counters[kCntFetchExt + t * kFetchStride] = 0u;
 }
 __global__ void kPrepShadow(uint32_t* counters) {
     unsigned long long* stats = reinterpret_cast<unsigned long long*>(counters + kCntStats);
@@ -308,18 +330,21 @@ __global__ __launch_bounds__(kTraceBlock) void kExtend(SceneView sv, const float
     st.glob = gstack + (size_t)(blockIdx.x * kTraceBlock + threadIdx.x) * kGlobalStack;
     st.dropped = 0;
     const uint32_t n = counters[countSlot];
+    if (n == 0u) return;
     const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t part = blockIdx.x % kFetchParts;
     while (true) {
-        uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(&counters[kCntFetchExt], 64u);
-        base = (uint32_t)__shfl((int)base, 0);
+        const uint32_t base = claimBatch(counters + kCntFetchExt, part, n);
         if (base >= n) break;
-        uint32_t i = base + lane;
-        if (i < n) {
-            float4 a = ro[i], b = rd[i];
-            TraversalRay r = makeTraversalRay(xyz(a), xyz(b));
-            Hit h = traceClosest(sv, r, kRayTMin, b.w, st, COUNT ? &nodeVisits : nullptr, COUNT ? &triTests : nullptr);
-            hits[i] = make_float4(h.t, __uint_as_float(h.tri), h.u, h.v);
+        for (uint32_t r0 = 0; r0 < kFetchBatch; r0 += 64u) {
+            uint32_t i = base + r0 + lane;
+            if (i < n) {
+                float4 a = ro[i], b = rd[i];
+                TraversalRay r = makeTraversalRay(xyz(a), xyz(b));
+                Hit h = traceClosest(sv, r, kRayTMin, b.w, st, COUNT ? &nodeVisits : nullptr,
+                                     COUNT ? &triTests : nullptr);
+                hits[i] = make_float4(h.t, __uint_as_float(h.tri), h.u, h.v);
+            }
         }
     }
     if (st.dropped) atomicAdd(&counters[kCntDropped], st.dropped);
@@ -345,13 +370,18 @@ __global__ __launch_bounds__(kTraceBlock) void kShadow(SceneView sv, PathView pv
     st.glob = gstack + (size_t)(blockIdx.x * kTraceBlock + threadIdx.x) * kGlobalStack;
     st.dropped = 0;
     const uint32_t n = counters[kCntShadow];
+    if (n == 0u) return;
     const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t part = blockIdx.x % kFetchParts;
+    uint32_t base = n, round = kFetchBatch;
     while (true) {
-        uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(&counters[kCntFetchShadow], 64u);
-        base = (uint32_t)__shfl((int)base, 0);
+        if (round == kFetchBatch) {
+            base = claimBatch(counters + kCntFetchShadow, part, n);
+            round = 0;
+        }
         if (base >= n) break;
-        uint32_t i = base + lane;
+        uint32_t i = base + round + lane;
+        round += 64u;
         if (i >= n) continue;
         float4 o = sh.o[i], d = sh.d[i];
         uint32_t q = __float_as_uint(o.w);
@@ -826,10 +856,12 @@ __global__ __launch_bounds__(kTraceBlock) void kPrimary(SceneView sv, SceneData 
 static uint32_t blocksFor(uint64_t n) { return (uint32_t)((n + kBlock - 1) / kBlock); }
 
 bool ensureTraversalScratch(mrt_context* ctx) {
-    if (!ctx->counters.reserve(64)) return false;
+    if (!ctx->counters.reserve(kCounterWords)) return false;
     if (ctx->traceGrid == 0) {
         // persistent traversal grid: enough resident blocks to fill every CU
+        // 8 blocks per CU; a multiple of kFetchParts so every claim partition has blocks
         ctx->traceGrid = (uint32_t)std::max(1, ctx->cuCount) * 8u;
+        static_assert(8 % kFetchParts == 0, "trace grid must cover every claim partition");
     }
     size_t need = (size_t)ctx->traceGrid * kTraceBlock * kGlobalStack;
     return ctx->traversalStack.reserve(need);
@@ -902,7 +934,7 @@ static bool launchChunk(mrt_context* ctx, FrameParams fp, StageEvents& ev, bool 
     for (uint32_t depth = 0; depth < maxDepth; depth++) {
         int nxt = 1 - cur;
         fp.depth = depth;
-        hipLaunchKernelGGL(kPrepDepth, dim3(1), dim3(1), 0, s, cnt, cur, nxt);
+        hipLaunchKernelGGL(kPrepDepth, dim3(1), dim3(64), 0, s, cnt, cur, nxt);
         bool timed = ev.begin(s, 0);
         if (countTraversal)
             hipLaunchKernelGGL(kExtend<true>, dim3(ctx->traceGrid), dim3(kTraceBlock), 0, s, sv, qv[cur].o, qv[cur].d,
@@ -915,11 +947,13 @@ static bool launchChunk(mrt_context* ctx, FrameParams fp, StageEvents& ev, bool 
         hipLaunchKernelGGL(kShade<MODE>, dim3(blocksFor(fp.count)), dim3(kBlock), 0, s, fp, sv, pv, qv[cur], b.hits.ptr,
                            qv[nxt], shv, cnt, cur, nxt);
         if (timed) ev.end(s);
-        hipLaunchKernelGGL(kPrepShadow, dim3(1), dim3(1), 0, s, cnt);
-        timed = ev.begin(s, 2);
-        hipLaunchKernelGGL(kShadow<MODE>, dim3(ctx->traceGrid), dim3(kTraceBlock), 0, s, sv, pv, shv, cnt,
-                           ctx->traversalStack.ptr);
-        if (timed) ev.end(s);
+        if (fp.neeEnabled) {  // otherwise no shadow record can exist
+            hipLaunchKernelGGL(kPrepShadow, dim3(1), dim3(1), 0, s, cnt);
+            timed = ev.begin(s, 2);
+            hipLaunchKernelGGL(kShadow<MODE>, dim3(ctx->traceGrid), dim3(kTraceBlock), 0, s, sv, pv, shv, cnt,
+                               ctx->traversalStack.ptr);
+            if (timed) ev.end(s);
+        }
         cur = nxt;
     }
     hipLaunchKernelGGL(kFinalize<MODE>, dim3(blocksFor(fp.count)), dim3(kBlock), 0, s, fp.count, pv, b.sampleRad.ptr);
@@ -990,7 +1024,7 @@ extern "C" int mrt_render_frame(mrt_context* ctx, const SceneData* scene, const 
     if (!ctx->frameSum.reserve(numPixels ? numPixels : 1)) return -3;
     hipLaunchKernelGGL(kFill4, dim3(blocksFor(numPixels ? numPixels : 1)), dim3(kBlock), 0, s, ctx->frameSum.ptr,
                        (size_t)numPixels, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
-    hipMemsetAsync(ctx->counters.ptr, 0, 64 * sizeof(uint32_t), s);
+    hipMemsetAsync(ctx->counters.ptr, 0, kCounterWords * sizeof(uint32_t), s);
 
     uint64_t totalPaths = numPixels * spp;
     uint64_t chunk = std::min<uint64_t>(totalPaths, kMaxChunk);
