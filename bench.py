@@ -45,13 +45,25 @@ B_HIT = 156
 B_STATE = 192
 B_NEE = 170
 B_ACCUM_PER_PIXEL = 72
-NODE_BYTES = 128  # 4-wide node: per-axis child bounds 96 B + 4 child references 16 B + pad
+NODE_BYTES = 96   # 8-wide node: origin + exponents 16 B, 8-bit child planes 48 B, 8 child references 32 B
 TRI_BYTES = 48    # triangle record: 3 vertices x float3 + global triangle id
 # The traversal kernel's own share of B_ray: ray read 32 B + hit write 20 B.
 B_EXTEND_IO = 32 + 20
 
-WIDTH, HEIGHT, SPP, MAX_DEPTH, MIN_DEPTH = 1920, 1080, 64, 12, 4
+WIDTH, HEIGHT, MIN_DEPTH = 1920, 1080, 4
 CPU_ROWS = 512  # centred band of full-width rows for the CPU baseline sample
+
+# The headline workload (spectral) and the incoherent-traversal stress workload.
+WORKLOADS = {
+    "spectral": dict(spp=64, depth=12, mode="spectral",
+                     name="displaced icosphere 1,048,820 triangles, 64 materials (20% dielectric IOR 1.5), "
+                          "2048x1024 HDR sky + 0.5 deg sun 1e4; 1920x1080, 64 spp, depth 12, hero spectral, "
+                          "PCG32 seed 1"),
+    "stress": dict(spp=16, depth=16, mode="rgb",
+                   name="16 rigidly transformed instances of the displaced icosphere + 4M random triangles in the "
+                        "unit cube (20,781,120 triangles), all diffuse, 2048x1024 sky; 1920x1080, 16 spp, depth 16, "
+                        "RGB, PCG32 seed 1"),
+}
 
 
 def log(msg: str):
@@ -65,6 +77,8 @@ def parse_args():
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--no-cpu-baseline", action="store_true", help="skip the CPU oracle timing leg")
     p.add_argument("--cpu-rows", type=int, default=CPU_ROWS)
+    p.add_argument("--workload", choices=sorted(WORKLOADS), default="spectral",
+                   help="spectral: the headline 1080p spectral workload; stress: the 20.8M-triangle traversal test")
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                    help="only run device-to-device copies of MIB mebibytes (reference kernel for HBM counters)")
     return p.parse_args()
@@ -89,25 +103,27 @@ def copy_calibration(mib: int):
     log(f"copied {mib} MiB device-to-device 5 times")
 
 
-def build_workload():
+def build_workload(kind):
     import vkrt_amd as V
     from vkrt_amd import scenes
 
+    w = WORKLOADS[kind]
     t0 = time.time()
-    scene = scenes.displaced_sphere(229)
+    scene = scenes.displaced_sphere(229) if kind == "spectral" else scenes.stress_scene()
     scene.environment = scenes.sky_environment(2048, 1024)
     r = V.Renderer(WIDTH, HEIGHT)
     scenes.load(r, scene)
-    r.configure(spp=SPP, max_depth=MAX_DEPTH, min_depth=MIN_DEPTH, mode=V.RENDER_MODE_SPECTRAL,
-                spectral=V.SPECTRAL_HERO, rng=V.RNG_PCG32, seed=1)
+    mode = V.RENDER_MODE_SPECTRAL if w["mode"] == "spectral" else V.RENDER_MODE_RGB
+    r.configure(spp=w["spp"], max_depth=w["depth"], min_depth=MIN_DEPTH, mode=mode, spectral=V.SPECTRAL_HERO,
+                rng=V.RNG_PCG32, seed=1)
     r.start(WIDTH, HEIGHT, 0)
-    log(f"scene ready in {time.time() - t0:.1f} s: {scene.triangle_count} triangles")
+    log(f"{kind} scene ready in {time.time() - t0:.1f} s: {scene.triangle_count} triangles")
     return r, scene
 
 
-def cpu_baseline(renderer, rows: int):
+def cpu_baseline(renderer, rows: int, spp: int, depth: int):
     """Times the CPU oracle (a C restatement of the reference integrator) on a
-    centred band of `rows` full-width rows of the same frame, 64 spp."""
+    centred band of `rows` full-width rows of the same frame."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
@@ -128,8 +144,8 @@ def cpu_baseline(renderer, rows: int):
         "unit": "Gbounces/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"rows {r0}-{r0 + rows - 1} of the 1920x1080 frame, all {WIDTH} columns, {SPP} spp, depth "
-                  f"{MAX_DEPTH}, hero spectral, same scene and seeds; FP32 C oracle with OpenMP; "
+        "sample": f"rows {r0}-{r0 + rows - 1} of the 1920x1080 frame, all {WIDTH} columns, {spp} spp, depth "
+                  f"{depth}, same scene, mode and seeds; FP32 C oracle with OpenMP; "
                   f"{bounces} bounces in {dt:.2f} s; oracle BVH build ({build_s:.1f} s) excluded",
     }
 
@@ -147,11 +163,15 @@ def main():
         copy_calibration(args.copy_calibration)
         return
 
-    r, scene = build_workload()
+    wl = WORKLOADS[args.workload]
+    SPP, MAX_DEPTH = wl["spp"], wl["depth"]
+    r, scene = build_workload(args.workload)
     log(f"warmup {args.warmup} frame(s) (the first uploads the scene and builds the BVH)")
     for _ in range(args.warmup):
         r.draw()
     st = r.stats()
+    bvh = {"build_ms": round(st.bvhBuildMs, 2), "nodes": st.bvhNodeCount, "depth": st.bvhMaxDepth,
+           "triangles": st.triangleCount}
     log(f"BVH build {st.bvhBuildMs:.1f} ms, {st.bvhNodeCount} nodes, depth {st.bvhMaxDepth}")
 
     r.synchronize()
@@ -199,7 +219,7 @@ def main():
     # passes (copy-calibrated), per ray, times this run's rays per launch.
     traffic = None
     pmc_path = os.path.join(ROOT, "profiles", "pmc_traffic_1080p_spectral.json")
-    if os.path.exists(pmc_path):
+    if args.workload == "spectral" and os.path.exists(pmc_path):
         with open(pmc_path) as f:
             pmc = json.load(f)
         traffic = round(pmc["kExtend"]["hbm_bytes_per_ray"] * ext_rays / max(1, ext_launches))
@@ -210,7 +230,7 @@ def main():
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBPS, 4),
         "traffic": traffic,
-        "kernel": "kExtend<false> (closest-hit traversal)",
+        "kernel": "kExtend<false> (closest-hit traversal, 8-wide quantised BVH)",
         "launches": ext_launches,
         "avg_launch_ms": round(ext_avg_ms, 4),
         "bytes_per_ray": round(B_EXTEND_IO + b_trav, 1),
@@ -229,12 +249,35 @@ def main():
         "extend_share_of_time": round(ext_ms * 1e-3 / elapsed, 3),
     }
 
+    # Variant: the same workload with environment NEE (an addition of this
+    # build; the reference gathers the environment only on BSDF misses).
+    variants = {}
+    if args.steps > 0 and args.workload == "spectral":
+        r.configure(env_importance=True)
+        r.draw()  # first frame of the new setting
+        r.synchronize()
+        t0 = time.perf_counter()
+        vb = vs = 0
+        vsteps = min(args.steps, 3)
+        for _ in range(vsteps):
+            r.draw()
+            st2 = r.stats()
+            vb += st2.extensionRays
+            vs += st2.shadowRays
+        r.synchronize()
+        vel = time.perf_counter() - t0
+        r.configure(env_importance=False)
+        variants["env_nee"] = {"value": round(vb / vel / 1e9, 4), "unit": "Gbounces/s", "steps": vsteps,
+                               "ms_per_step": round(vel * 1e3 / vsteps, 3),
+                               "shadow_rays_per_step": vs // vsteps}
+        log(f"env NEE variant: {vb / vel / 1e9:.3f} Gbounces/s, {vs // vsteps} shadow rays per frame")
+
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(r, args.cpu_rows)
+        cpu = cpu_baseline(r, args.cpu_rows if args.workload == "spectral" else 64, SPP, MAX_DEPTH)
 
     line = {
-        "metric": "Gbounces/s at 1080p spectral",
+        "metric": "Gbounces/s at 1080p spectral" if args.workload == "spectral" else "Gbounces/s at 1080p, 20.8M-triangle stress",
         "value": round(value, 4),
         "unit": "Gbounces/s",
         "n_gpus": 1,
@@ -247,8 +290,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic (procedural displaced icosphere and HDR sky generated in-process)",
         "config": {
-            "workload": "displaced icosphere 1,048,820 triangles, 64 materials (20% dielectric IOR 1.5), "
-                        "2048x1024 HDR sky + 0.5 deg sun 1e4; 1920x1080, 64 spp, depth 12, hero spectral, PCG32 seed 1",
+            "workload": wl["name"],
             "width": WIDTH,
             "height": HEIGHT,
             "spp": SPP,
@@ -260,6 +302,8 @@ def main():
         },
         "roofline": roofline,
         "pipeline": pipeline,
+        "bvh": bvh,
+        "variants": variants,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

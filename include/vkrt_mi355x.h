@@ -100,6 +100,9 @@ typedef struct VKRT_SceneBuffers {
     uint32_t rngPolicy;
     uint64_t rngSeed;
     uint32_t environmentImportanceSampling;
+    const float* environmentAliasQ;      /* width*height entries, NULL without an environment texture */
+    const uint32_t* environmentAliasIdx;
+    const float* environmentPdf;         /* texel selection probabilities */
 } VKRT_SceneBuffers;
 VKRT_Result VKRT_exportSceneBuffers(VKRT* vkrt, VKRT_SceneBuffers* out);
 

@@ -71,6 +71,9 @@ def lib():
         g("oracle_fresnel_dielectric").argtypes = [rt, rt]
         g("oracle_environment").argtypes = [P, rp, rp]
         g("oracle_hero_wavelengths").argtypes = [C.c_uint32, C.c_uint32, rp]
+        g("oracle_env_sample_batch").argtypes = [P, C.c_uint64, C.c_uint32, rp]
+        g("oracle_env_pdf").restype = rt
+        g("oracle_env_pdf").argtypes = [P, rp]
     _lib = L
     return L
 
@@ -262,6 +265,19 @@ def hero_wavelengths(hash_state, sample_index, precision=64):
     out = np.zeros(4, dtype=dt)
     getattr(lib(), "oracle_hero_wavelengths" + sfx)(hash_state, sample_index, ptr(out))
     return out
+
+
+def env_sample_batch(buffers, n, seed=1, precision=64):
+    dt, sfx = _dtype(precision)
+    out = np.zeros((n, 6), dtype=dt)
+    getattr(lib(), "oracle_env_sample_batch" + sfx)(C.addressof(buffers), seed, n, ptr(out))
+    return out
+
+
+def env_pdf(buffers, direction, precision=64):
+    dt, sfx = _dtype(precision)
+    d = np.ascontiguousarray(direction, dtype=dt)
+    return float(getattr(lib(), "oracle_env_pdf" + sfx)(C.addressof(buffers), ptr(d)))
 
 
 def build_alias(pmf):

@@ -54,6 +54,27 @@ def test_primary_hit_ids_displaced_sphere():
     r.close()
 
 
+def test_primary_hit_ids_instances_and_tiny_triangles():
+    """A scaled-down stress scene: transformed instances of a displaced sphere
+    and 200,000 random triangles with edges of 1e-3..1e-2 (the boxes that
+    8-bit plane quantisation must still bound conservatively)."""
+    w, h = 320, 180
+    s = scenes.stress_scene(instances=4, freq=40, random_count=200_000)
+    s.camera = ((0.5, -1.2, 0.6), (0.5, 0.5, 0.45), (0.0, 0.0, 1.0), 60.0)  # into the cube of small triangles
+    r = V.Renderer(w, h)
+    scenes.load(r, s)
+    r.configure(spp=1, max_depth=4, min_depth=4, rng=V.RNG_PCG32, seed=3)
+    r.start(w, h, 0)
+    gm, gp = r.primary_hits(0)
+    om, op = O.OracleScene(r.export(), use_bvh=True).primary_hits(w, h, 0)
+    hit = om != V.VKRT_INVALID_INDEX
+    agree = np.mean((gm == om) & (gp == op))
+    print(f"primary-hit agreement {agree:.6f} over {hit.sum()} hits, {s.triangle_count} triangles")
+    assert hit.sum() > 0.1 * w * h
+    assert agree >= 0.9999
+    r.close()
+
+
 @pytest.mark.parametrize(
     "mode,spectral,rng",
     [
@@ -114,3 +135,55 @@ def test_sphere_dielectric_ggx_environment_parity(mode):
     assert close > 0.97
     assert err < 1e-3
     r.close()
+
+
+@pytest.mark.parametrize("mode,spectral", [(V.RENDER_MODE_RGB, V.SPECTRAL_HERO),
+                                           (V.RENDER_MODE_SPECTRAL, V.SPECTRAL_HERO),
+                                           (V.RENDER_MODE_SPECTRAL, V.SPECTRAL_SINGLE)])
+def test_environment_nee_same_seed_parity(mode, spectral):
+    """Environment NEE (alias-table sampler, MIS with BSDF misses, its own
+    shadow pass) against the oracle's restatement with the same streams."""
+    w, h = 96, 54
+    r = make(small_sphere_scene(), w, h, 8, mode=mode, spectral=spectral, depth=12)
+    r.configure(env_importance=True)
+    orc = O.OracleScene(r.export(), use_bvh=True)
+    r.draw()
+    gpu = r.accumulation()
+    st = r.stats()
+    ref, bounces = orc.render(w, h, precision=32)
+    err = O.relmse(gpu[..., :3], ref[..., :3])
+    close = np.mean(np.abs(gpu[..., :3] - ref[..., :3]) <= 1e-3 * (1.0 + np.abs(ref[..., :3])))
+    print(f"relMSE {err:.3e}, pixels within 1e-3: {close:.5f}, shadow rays {st.shadowRays}, "
+          f"bounces gpu {st.extensionRays} oracle {bounces}")
+    assert st.shadowRays > 0
+    assert np.isfinite(gpu).all()
+    assert close > 0.97
+    assert err < 1e-3
+    r.close()
+
+
+def test_environment_nee_converges_to_the_reference_estimator():
+    """With and without environment NEE the GPU converges to the same image:
+    the per-frame image means of the two estimators agree within their
+    standard errors."""
+    w, h = 64, 36
+    frames = 12
+    means = {}
+    for nee in (False, True):
+        r = make(small_sphere_scene(), w, h, 256, depth=8, seed=11 if nee else 5)
+        r.configure(env_importance=nee)
+        m = []
+        prev_sum = 0.0
+        for k in range(1, frames + 1):
+            r.draw()  # each frame draws new streams (the frame number is part of the stream id)
+            acc = r.accumulation().astype(np.float64)
+            assert np.all(acc[..., 3] == 256 * k)
+            total = acc[..., :3].mean() * k  # running mean x frames = sum of per-frame means
+            m.append(total - prev_sum)
+            prev_sum = total
+        r.close()
+        means[nee] = np.array(m)
+    se = np.hypot(means[False].std(ddof=1), means[True].std(ddof=1)) / np.sqrt(frames)
+    diff = means[True].mean() - means[False].mean()
+    print(f"image mean without NEE {means[False].mean():.6f}, with {means[True].mean():.6f}, diff {diff:.2e}, se {se:.2e}")
+    assert abs(diff) < 4 * se + 1e-4 * means[False].mean()

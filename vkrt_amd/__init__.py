@@ -307,6 +307,9 @@ class SceneBuffers(C.Structure):
         ("rngPolicy", C.c_uint32),
         ("rngSeed", C.c_uint64),
         ("environmentImportanceSampling", C.c_uint32),
+        ("environmentAliasQ", C.POINTER(C.c_float)),
+        ("environmentAliasIdx", C.POINTER(C.c_uint32)),
+        ("environmentPdf", C.POINTER(C.c_float)),
     ]
 
 

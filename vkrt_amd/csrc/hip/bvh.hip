@@ -7,9 +7,10 @@
 //  4. binary radix tree over the sorted codes (Karras, HPG 2012), each node
 //     covering a contiguous range of sorted triangles
 //  5. bottom-up bounds with one atomic arrival flag per node
-//  6. top-down collapse into 4-wide nodes: a node is opened greedily by
-//     splitting its largest-area child until four children remain; subtrees
-//     of at most kLeafMaxTris triangles become leaves (level-synchronous)
+//  6. top-down collapse into 8-wide nodes: a node is opened greedily by
+//     splitting its largest-area child until eight children remain; subtrees
+//     of at most kLeafMaxTris triangles become leaves (level-synchronous);
+//     child boxes are quantised to 8 bits per plane, conservatively
 //  7. triangles reordered into leaf order as 48-byte records
 #include <cstring>
 #include <rocprim/device/device_radix_sort.hpp>
@@ -188,6 +189,31 @@ __device__ __forceinline__ int rangeFirst(int node, int n, const int2* range) {
     return node >= n - 1 ? node - (n - 1) : range[node].x;
 }
 
+// Quantisation of one axis of a node: the smallest power-of-two step with
+// which 255 steps from the node's lower plane reach its upper plane, checked
+// with the decode arithmetic of the traversal (fmaf(q, step, origin)).
+__device__ uint32_t nodeExponent(float lo, float hi) {
+    float ext = hi - lo;
+    int e = 1;
+    if (ext > 0.0f) {
+        int ex;
+        frexpf(ext / 255.0f, &ex);  // ext/255 = m * 2^ex, m in [0.5, 1): 2^ex >= ext/255
+        e = min(max(ex + 127, 1), 254);
+    }
+    while (e < 254 && fmaf(255.0f, __uint_as_float((uint32_t)e << 23), lo) < hi) e++;
+    return (uint32_t)e;
+}
+// Lower and upper plane codes of a child interval [clo, chi] inside the node:
+// decode(qlo) <= clo and decode(qhi) >= chi exactly.
+__device__ void quantizePlanes(float origin, float step, float clo, float chi, uint32_t& qlo, uint32_t& qhi) {
+    float a = floorf((clo - origin) / step), b = ceilf((chi - origin) / step);
+    int l = (int)fminf(fmaxf(a, 0.0f), 255.0f), h = (int)fminf(fmaxf(b, 0.0f), 255.0f);
+    while (l > 0 && fmaf((float)l, step, origin) > clo) l--;
+    while (h < 255 && fmaf((float)h, step, origin) < chi) h++;
+    qlo = (uint32_t)l;
+    qhi = (uint32_t)h;
+}
+
 __global__ __launch_bounds__(kB) void kCollapse(const CollapseItem* in, const uint32_t* inCount, CollapseItem* out,
                                                uint32_t* outCount, uint32_t* nodeAlloc, int n, const int2* children,
                                                const int2* range, const Aabb* nodeBoxes, BvhNode* nodes,
@@ -195,7 +221,7 @@ __global__ __launch_bounds__(kB) void kCollapse(const CollapseItem* in, const ui
     uint32_t w = blockIdx.x * kB + threadIdx.x;
     if (w >= *inCount) return;
     CollapseItem it = in[w];
-    int slot[4];
+    int slot[kBvhWidth];
     int cnt = 0;
     if (it.binNode >= n - 1) {
         slot[cnt++] = it.binNode;
@@ -203,7 +229,7 @@ __global__ __launch_bounds__(kB) void kCollapse(const CollapseItem* in, const ui
         int2 c = children[it.binNode];
         slot[cnt++] = c.x;
         slot[cnt++] = c.y;
-        while (cnt < 4) {
+        while (cnt < kBvhWidth) {
             // open the largest child that is an internal node with too many triangles for a leaf
             int best = -1;
             float bestArea = -1.0f;
@@ -222,50 +248,46 @@ __global__ __launch_bounds__(kB) void kCollapse(const CollapseItem* in, const ui
             slot[cnt++] = c2.y;
         }
     }
+    const Aabb P = nodeBoxes[it.binNode];
     BvhNode node;
-    float lx[4], hx[4], ly[4], hy[4], lz[4], hz[4];
-    uint32_t ref[4];
-    for (int k = 0; k < 4; k++) {
-        if (k >= cnt) {
-            lx[k] = ly[k] = lz[k] = INFINITY;
-            hx[k] = hy[k] = hz[k] = -INFINITY;
-            ref[k] = kEmptyRef;
-            continue;
-        }
+    node.ox = P.lo[0];
+    node.oy = P.lo[1];
+    node.oz = P.lo[2];
+    uint32_t e[3];
+    float step[3];
+    for (int a = 0; a < 3; a++) {
+        e[a] = nodeExponent(P.lo[a], P.hi[a]);
+        step[a] = __uint_as_float(e[a] << 23);
+    }
+    for (int k = 0; k < 6; k++) node.qlo[k] = node.qhi[k] = 0u;
+    uint32_t used = 0;
+    for (int k = 0; k < cnt; k++) {
         int nd = slot[k];
-        Aabb b = nodeBoxes[nd];
-        lx[k] = b.lo[0];
-        ly[k] = b.lo[1];
-        lz[k] = b.lo[2];
-        hx[k] = b.hi[0];
-        hy[k] = b.hi[1];
-        hz[k] = b.hi[2];
+        uint32_t ref;
         int count = rangeCount(nd, n, range);
         if (count <= (int)kLeafMaxTris) {
-            uint32_t first = (uint32_t)rangeFirst(nd, n, range);
-            ref[k] = kLeafBit | ((uint32_t)(count - 1) << 27) | first;
+            ref = kLeafBit | ((uint32_t)(count - 1) << 27) | (uint32_t)rangeFirst(nd, n, range);
         } else {
             uint32_t child = atomicAdd(nodeAlloc, 1u);
             if (child >= nodeCapacity) {
                 atomicAdd(overflow, 1u);
-                ref[k] = kEmptyRef;
-                lx[k] = ly[k] = lz[k] = INFINITY;
-                hx[k] = hy[k] = hz[k] = -INFINITY;
                 continue;
             }
-            ref[k] = child;
+            ref = child;
             uint32_t o = atomicAdd(outCount, 1u);
             out[o] = CollapseItem{nd, child};
         }
+        const Aabb b = nodeBoxes[nd];
+        for (int a = 0; a < 3; a++) {
+            uint32_t ql, qh;
+            quantizePlanes(P.lo[a], step[a], b.lo[a], b.hi[a], ql, qh);
+            node.qlo[a * 2 + (used >> 2)] |= ql << ((used & 3u) * 8u);
+            node.qhi[a * 2 + (used >> 2)] |= qh << ((used & 3u) * 8u);
+        }
+        node.child[used++] = ref;
     }
-    node.loX = make_float4(lx[0], lx[1], lx[2], lx[3]);
-    node.hiX = make_float4(hx[0], hx[1], hx[2], hx[3]);
-    node.loY = make_float4(ly[0], ly[1], ly[2], ly[3]);
-    node.hiY = make_float4(hy[0], hy[1], hy[2], hy[3]);
-    node.loZ = make_float4(lz[0], lz[1], lz[2], lz[3]);
-    node.hiZ = make_float4(hz[0], hz[1], hz[2], hz[3]);
-    node.child = make_uint4(ref[0], ref[1], ref[2], ref[3]);
-    node.pad = make_uint4(0, 0, 0, 0);
+    for (uint32_t k = used; k < (uint32_t)kBvhWidth; k++) node.child[k] = kEmptyRef;
+    node.packed = e[0] | (e[1] << 8) | (e[2] << 16) | (used << 24);
     nodes[it.wide] = node;
 }
 
@@ -322,7 +344,7 @@ extern "C" int mrt_build_bvh(mrt_context* ctx, mrt_bvh_stats* stats) {
     DeviceArray<CollapseItem> front[2];
     int nn = (int)n;
     size_t binNodes = (size_t)2 * n - 1;
-    size_t wideCap = n;  // a 4-wide tree over n triangles never needs more than n nodes
+    size_t wideCap = n;  // every wide node but the root is a distinct binary node with > kLeafMaxTris triangles
     bool ok = triBoxes.reserve(n) && nodeBoxes.reserve(binNodes) && cb.reserve(6) && vals.reserve(n) &&
               valsSorted.reserve(n) && arrival.reserve(n) && counts.reserve(4) && keys.reserve(n) &&
               keysSorted.reserve(n) && children.reserve(n) && range.reserve(n) && parent.reserve(binNodes) &&

@@ -106,6 +106,7 @@ extern "C" void mrt_destroy(mrt_context* ctx) {
         for (auto& a : q) a.release();
     b.hits.release();
     for (auto& a : b.shadow) a.release();
+    for (auto& a : b.shadowEnv) a.release();
     b.sampleRad.release();
     ctx->counters.release();
     ctx->traversalStack.release();

@@ -76,14 +76,25 @@ constexpr float kRRMax = 0.95f;
 constexpr float kSafeNormalizeEps = 1e-12f;
 
 // ---------------------------------------------------------------- BVH --
-// 4-wide node, 128 bytes = one cache line pair on gfx950; child boxes stored
-// per axis so a lane loads the node with eight 16-byte loads.
+// 8-wide node, 96 bytes = six 16-byte loads. Child boxes are quantised to 8
+// bits per plane relative to the node's own box: a child's bound on axis a is
+// origin[a] + q * 2^(e[a] - 127), with lower planes rounded down and upper
+// planes rounded up when the node is built, so a decoded box always contains
+// the exact one.
+//   word 0..2  origin x, y, z
+//   word 3     e[0] | e[1] << 8 | e[2] << 16 | childCount << 24
+//   word 4..9  lower planes: 8 bytes for x, 8 for y, 8 for z
+//   word 10..15 upper planes, same layout
+//   word 16..23 child references (see below); slots >= childCount are unused
 struct alignas(16) BvhNode {
-    float4 loX, hiX, loY, hiY, loZ, hiZ;
-    uint4 child;  // node reference per slot (see below)
-    uint4 pad;
+    float ox, oy, oz;
+    uint32_t packed;
+    uint32_t qlo[6];
+    uint32_t qhi[6];
+    uint32_t child[8];
 };
-static_assert(sizeof(BvhNode) == 128, "BvhNode layout");
+static_assert(sizeof(BvhNode) == 96, "BvhNode layout");
+constexpr int kBvhWidth = 8;
 
 // node reference: bit31 = leaf; leaf: bits 0..26 first triangle (BVH order),
 // bits 27..30 triangle count - 1; internal: node index. kEmptyRef = empty slot.

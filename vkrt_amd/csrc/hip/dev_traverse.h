@@ -1,22 +1,18 @@
 // Ray traversal for gfx950: replaces the hardware TraceRay the reference
 // relies on (src/shaders/rt/queries/scene_query.slang:10-42,
-// shadow_query.slang:8-28). Closest-hit and any-hit queries over a 4-wide BVH
-// with per-axis child boxes; triangles are two-sided (RAY_FLAG_NONE, no
+// shadow_query.slang:8-28). Closest-hit and any-hit queries over an 8-wide
+// BVH with quantised child boxes; triangles are two-sided (RAY_FLAG_NONE, no
 // culling) and tested with the watertight algorithm of Woop, Benthin and Wald
 // (JCGT 2013), reporting (t, u, v) with u, v the weights of vertices 1 and 2 as
 // BuiltInTriangleIntersectionAttributes.
-//
-// Stack: kLdsStack entries per lane in LDS (bank-conflict free layout
-// [entry][lane]) and kGlobalStack more in a per-thread global region, used only
-// when a traversal gets deep; pushes beyond both are dropped and counted.
 #pragma once
 
 #include "dev_common.h"
 
 namespace mrt {
 
-constexpr int kLdsStack = 16;
-constexpr int kGlobalStack = 112;
+constexpr int kLdsStack = 12;
+constexpr int kGlobalStack = 116;
 constexpr int kTraceBlock = 256;
 // Hard cap on loop iterations of one traversal: a well-formed BVH over 2^27
 // triangles needs far fewer; the cap only guarantees termination.
@@ -97,7 +93,7 @@ DEV bool intersectTriangle(const TriRecord& tr, F3 o, const RayPre& p, float tmi
 struct TraversalRay {
     F3 o, d;
     F3 inv;      // 1/d
-    F3 oInv;     // o/d, so entry distances are lo*inv - oInv
+    F3 oInv;     // o/d, so entry distances are plane*inv - oInv
     bool negX, negY, negZ;
     RayPre pre;
 };
@@ -115,51 +111,132 @@ DEV TraversalRay makeTraversalRay(F3 o, F3 d) {
     return r;
 }
 
-// Slab test of the four children; near/far planes picked by ray octant.
-DEV void boxes4(const BvhNode* node, const TraversalRay& r, float tmin, float tmax, float tn[4], bool hit[4]) {
-    const float4* n4 = reinterpret_cast<const float4*>(node);
-    float4 lx = n4[0], hx = n4[1], ly = n4[2], hy = n4[3], lz = n4[4], hz = n4[5];
-    float4 nx = r.negX ? hx : lx, fx = r.negX ? lx : hx;
-    float4 ny = r.negY ? hy : ly, fy = r.negY ? ly : hy;
-    float4 nz = r.negZ ? hz : lz, fz = r.negZ ? lz : hz;
-    const float nxa[4] = {nx.x, nx.y, nx.z, nx.w}, fxa[4] = {fx.x, fx.y, fx.z, fx.w};
-    const float nya[4] = {ny.x, ny.y, ny.z, ny.w}, fya[4] = {fy.x, fy.y, fy.z, fy.w};
-    const float nza[4] = {nz.x, nz.y, nz.z, nz.w}, fza[4] = {fz.x, fz.y, fz.z, fz.w};
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-        float t0x = nxa[i] * r.inv.x - r.oInv.x, t1x = fxa[i] * r.inv.x - r.oInv.x;
-        float t0y = nya[i] * r.inv.y - r.oInv.y, t1y = fya[i] * r.inv.y - r.oInv.y;
-        float t0z = nza[i] * r.inv.z - r.oInv.z, t1z = fza[i] * r.inv.z - r.oInv.z;
-        float tnear = fmaxf(fmaxf(t0x, t0y), fmaxf(t0z, tmin));
-        float tfar = fminf(fminf(t1x, t1y), fminf(t1z, tmax)) * 1.0000004f;
-        tn[i] = tnear;
-        hit[i] = tnear <= tfar;
-    }
+DEV float planeByte(uint32_t w0, uint32_t w1, int i) {
+    return (float)(((i < 4 ? w0 : w1) >> ((i & 3) * 8)) & 0xFFu);
 }
 
+// Slab test of the eight children of a node: decodes each child box, picks
+// near and far planes by ray octant, and returns the mask of children whose
+// box overlaps [tmin, tmax] with their entry distances in tn and their
+// references in ch. The far distance is widened by 4 ulp so that rounding in
+// the decode and slab arithmetic never rejects a box the ray touches.
+DEV uint32_t boxes8(const BvhNode* node, const TraversalRay& r, float tmin, float tmax, float tn[8], uint32_t ch[8]) {
+    const uint4* w = reinterpret_cast<const uint4*>(node);
+    const uint4 w0 = w[0], w1 = w[1], w2 = w[2], w3 = w[3], w4 = w[4], w5 = w[5];
+    const float ox = __uint_as_float(w0.x), oy = __uint_as_float(w0.y), oz = __uint_as_float(w0.z);
+    const uint32_t packed = w0.w;
+    const float sx = __uint_as_float((packed & 0xFFu) << 23);
+    const float sy = __uint_as_float(((packed >> 8) & 0xFFu) << 23);
+    const float sz = __uint_as_float(((packed >> 16) & 0xFFu) << 23);
+    const uint32_t count = packed >> 24;
+    // near / far plane words per axis (lower planes in w1, w2.xy; upper in w2.zw, w3)
+    const uint32_t nx0 = r.negX ? w2.z : w1.x, nx1 = r.negX ? w2.w : w1.y;
+    const uint32_t fx0 = r.negX ? w1.x : w2.z, fx1 = r.negX ? w1.y : w2.w;
+    const uint32_t ny0 = r.negY ? w3.x : w1.z, ny1 = r.negY ? w3.y : w1.w;
+    const uint32_t fy0 = r.negY ? w1.z : w3.x, fy1 = r.negY ? w1.w : w3.y;
+    const uint32_t nz0 = r.negZ ? w3.z : w2.x, nz1 = r.negZ ? w3.w : w2.y;
+    const uint32_t fz0 = r.negZ ? w2.x : w3.z, fz1 = r.negZ ? w2.y : w3.w;
+    ch[0] = w4.x;
+    ch[1] = w4.y;
+    ch[2] = w4.z;
+    ch[3] = w4.w;
+    ch[4] = w5.x;
+    ch[5] = w5.y;
+    ch[6] = w5.z;
+    ch[7] = w5.w;
+    uint32_t mask = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        float bnx = fmaf(planeByte(nx0, nx1, i), sx, ox), bfx = fmaf(planeByte(fx0, fx1, i), sx, ox);
+        float bny = fmaf(planeByte(ny0, ny1, i), sy, oy), bfy = fmaf(planeByte(fy0, fy1, i), sy, oy);
+        float bnz = fmaf(planeByte(nz0, nz1, i), sz, oz), bfz = fmaf(planeByte(fz0, fz1, i), sz, oz);
+        float t0x = fmaf(bnx, r.inv.x, -r.oInv.x), t1x = fmaf(bfx, r.inv.x, -r.oInv.x);
+        float t0y = fmaf(bny, r.inv.y, -r.oInv.y), t1y = fmaf(bfy, r.inv.y, -r.oInv.y);
+        float t0z = fmaf(bnz, r.inv.z, -r.oInv.z), t1z = fmaf(bfz, r.inv.z, -r.oInv.z);
+        float tnear = fmaxf(fmaxf(t0x, t0y), fmaxf(t0z, tmin));
+        float tfar = fminf(fminf(t1x, t1y), fminf(t1z, tmax)) * 1.0000005f;
+        tn[i] = tnear;
+        mask |= (tnear <= tfar && (uint32_t)i < count) ? (1u << i) : 0u;
+    }
+    return mask;
+}
+
+// Per-lane traversal stack of (reference, entry distance) pairs: kLdsStack
+// entries in LDS, laid out [entry][lane] so a wavefront's pushes hit distinct
+// banks, and kGlobalStack more in a per-thread global region for deep
+// traversals; pushes beyond both are dropped and counted.
 struct TraversalStack {
-    uint32_t* lds;   // entry i at lds[i * kTraceBlock]
-    uint32_t* glob;  // entry i (>= kLdsStack) at glob[i - kLdsStack]
+    uint32_t* lds;   // reference i at lds[i * kTraceBlock], distance at lds[(kLdsStack + i) * kTraceBlock]
+    uint32_t* glob;  // reference i (>= kLdsStack) at glob[i - kLdsStack], distance kGlobalStack words later
     int sp;
     uint32_t dropped;
-    DEV void push(uint32_t ref) {
+    DEV void push(uint32_t ref, float d) {
         if (sp < kLdsStack) {
             lds[sp * kTraceBlock] = ref;
+            lds[(kLdsStack + sp) * kTraceBlock] = __float_as_uint(d);
         } else if (sp < kLdsStack + kGlobalStack) {
             glob[sp - kLdsStack] = ref;
+            glob[kGlobalStack + sp - kLdsStack] = __float_as_uint(d);
         } else {
             dropped++;
             return;
         }
         sp++;
     }
-    DEV uint32_t pop() {
+    DEV void pop(uint32_t& ref, float& d) {
         sp--;
-        return sp < kLdsStack ? lds[sp * kTraceBlock] : glob[sp - kLdsStack];
+        if (sp < kLdsStack) {
+            ref = lds[sp * kTraceBlock];
+            d = __uint_as_float(lds[(kLdsStack + sp) * kTraceBlock]);
+        } else {
+            ref = glob[sp - kLdsStack];
+            d = __uint_as_float(glob[kGlobalStack + sp - kLdsStack]);
+        }
     }
 };
+constexpr int kLdsStackWords = 2 * kLdsStack * kTraceBlock;
+constexpr int kGlobalStackWords = 2 * kGlobalStack;
 
-// Closest hit in (tmin, tmax). Children are visited nearest first.
+// Child i of the mask with the smallest (largest) entry distance, chosen with
+// unrolled selects so tn and ch stay in registers.
+DEV int nearestChild(uint32_t mask, const float tn[8]) {
+    float best = INFINITY;
+    int bi = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        bool take = ((mask >> i) & 1u) && tn[i] < best;
+        best = take ? tn[i] : best;
+        bi = take ? i : bi;
+    }
+    return bi;
+}
+DEV int farthestChild(uint32_t mask, const float tn[8]) {
+    float best = -INFINITY;
+    int bi = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        bool take = ((mask >> i) & 1u) && tn[i] >= best;
+        best = take ? tn[i] : best;
+        bi = take ? i : bi;
+    }
+    return bi;
+}
+DEV uint32_t selectRef(const uint32_t ch[8], int k) {
+    uint32_t r = ch[0];
+#pragma unroll
+    for (int i = 1; i < 8; i++) r = k == i ? ch[i] : r;
+    return r;
+}
+DEV float selectDist(const float tn[8], int k) {
+    float d = tn[0];
+#pragma unroll
+    for (int i = 1; i < 8; i++) d = k == i ? tn[i] : d;
+    return d;
+}
+
+// Closest hit in (tmin, tmax). The nearest overlapping child is visited next
+// and the others are pushed farthest first; a popped entry whose entry
+// distance lies beyond the closest hit found so far is skipped.
 DEV Hit traceClosest(const SceneView& s, const TraversalRay& r, float tmin, float tmax, TraversalStack& st,
                      uint32_t* nodeVisits = nullptr, uint32_t* triTests = nullptr) {
     Hit h;
@@ -174,47 +251,23 @@ DEV Hit traceClosest(const SceneView& s, const TraversalRay& r, float tmin, floa
             st.dropped++;
             break;
         }
+        bool popNext = true;
         if (!(ref & kLeafBit)) {
             if (nodeVisits) (*nodeVisits)++;
-            const BvhNode* node = s.nodes + ref;
-            float tn[4];
-            bool hit[4];
-            boxes4(node, r, tmin, h.t, tn, hit);
-            uint4 c = node->child;
-            uint32_t refs[4] = {c.x, c.y, c.z, c.w};
-            // collect hit children, then order by entry distance (small sorting network)
-            float d[4];
-            uint32_t k[4];
-            int n = 0;
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                hit[i] = hit[i] && refs[i] != kEmptyRef;
-                d[i] = hit[i] ? tn[i] : INFINITY;
-                k[i] = hit[i] ? refs[i] : kEmptyRef;
-                n += hit[i] ? 1 : 0;
+            float tn[8];
+            uint32_t ch[8];
+            uint32_t mask = boxes8(s.nodes + ref, r, tmin, h.t, tn, ch);
+            if (mask != 0u) {
+                int ni = nearestChild(mask, tn);
+                ref = selectRef(ch, ni);
+                mask &= ~(1u << ni);
+                while (mask != 0u) {
+                    int fi = farthestChild(mask, tn);
+                    st.push(selectRef(ch, fi), selectDist(tn, fi));
+                    mask &= ~(1u << fi);
+                }
+                popNext = false;
             }
-#define MRT_CSWAP(a, b)                         \
-    if (d[b] < d[a]) {                          \
-        float td = d[a];                        \
-        d[a] = d[b];                            \
-        d[b] = td;                              \
-        uint32_t tk = k[a];                     \
-        k[a] = k[b];                            \
-        k[b] = tk;                              \
-    }
-            MRT_CSWAP(0, 1)
-            MRT_CSWAP(2, 3)
-            MRT_CSWAP(0, 2)
-            MRT_CSWAP(1, 3)
-            MRT_CSWAP(1, 2)
-#undef MRT_CSWAP
-            if (n == 0) {
-                if (st.sp == 0) break;
-                ref = st.pop();
-                continue;
-            }
-            for (int i = n - 1; i >= 1; i--) st.push(k[i]);
-            ref = k[0];
         } else {
             uint32_t first = leafFirst(ref), count = leafCount(ref);
             for (uint32_t i = 0; i < count; i++) {
@@ -228,8 +281,18 @@ DEV Hit traceClosest(const SceneView& s, const TraversalRay& r, float tmin, floa
                     h.v = v;
                 }
             }
-            if (st.sp == 0) break;
-            ref = st.pop();
+        }
+        if (popNext) {
+            bool found = false;
+            while (st.sp > 0) {
+                float d;
+                st.pop(ref, d);
+                if (d <= h.t) {
+                    found = true;
+                    break;
+                }
+            }
+            if (!found) break;
         }
     }
     return h;
@@ -246,28 +309,22 @@ DEV uint32_t traceAny(const SceneView& s, const TraversalRay& r, float tmin, flo
             st.dropped++;
             break;
         }
+        bool popNext = true;
         if (!(ref & kLeafBit)) {
-            const BvhNode* node = s.nodes + ref;
-            float tn[4];
-            bool hit[4];
-            boxes4(node, r, tmin, tmax, tn, hit);
-            uint4 c = node->child;
-            uint32_t refs[4] = {c.x, c.y, c.z, c.w};
-            uint32_t next = kEmptyRef;
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                if (!hit[i] || refs[i] == kEmptyRef) continue;
-                if (next == kEmptyRef)
-                    next = refs[i];
-                else
-                    st.push(refs[i]);
+            float tn[8];
+            uint32_t ch[8];
+            uint32_t mask = boxes8(s.nodes + ref, r, tmin, tmax, tn, ch);
+            if (mask != 0u) {
+                int ni = nearestChild(mask, tn);
+                ref = selectRef(ch, ni);
+                mask &= ~(1u << ni);
+                while (mask != 0u) {
+                    int k = __ffs(mask) - 1;
+                    st.push(selectRef(ch, k), 0.0f);
+                    mask &= mask - 1u;
+                }
+                popNext = false;
             }
-            if (next == kEmptyRef) {
-                if (st.sp == 0) break;
-                ref = st.pop();
-                continue;
-            }
-            ref = next;
         } else {
             uint32_t first = leafFirst(ref), count = leafCount(ref);
             for (uint32_t i = 0; i < count; i++) {
@@ -275,8 +332,11 @@ DEV uint32_t traceAny(const SceneView& s, const TraversalRay& r, float tmin, flo
                 float t, u, v;
                 if (intersectTriangle(tr, r.o, r.pre, tmin, tmax, t, u, v)) return __float_as_uint(tr.v0.w);
             }
+        }
+        if (popNext) {
             if (st.sp == 0) break;
-            ref = st.pop();
+            float d;
+            st.pop(ref, d);
         }
     }
     return kNoHit;

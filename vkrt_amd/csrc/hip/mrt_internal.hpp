@@ -50,6 +50,7 @@ struct PathBuffers {
     DeviceArray<float4> rays[2][2];  // [queue][origin+path | direction+tmax]
     DeviceArray<float4> hits;
     DeviceArray<float4> shadow[4];   // origin+path, direction+tmax, contribution if visible, otherwise
+    DeviceArray<float4> shadowEnv[3];  // environment NEE: origin+path, direction+tmax, contribution if visible
     DeviceArray<float4> sampleRad;
     size_t capacity = 0;
 };

@@ -31,6 +31,7 @@ constexpr uint32_t kFHeroActive = 2u;
 constexpr uint32_t kFRefractive = 4u;
 constexpr uint32_t kFAbsorption = 8u;
 constexpr uint32_t kFPendingCollapse = 16u;
+constexpr uint32_t kFPrevEnvNee = 32u;  // the previous vertex sampled the environment for NEE
 
 constexpr uint32_t kVisOccluded = 0u, kVisVisible = 1u, kVisUnsupported = 2u;
 
@@ -38,7 +39,7 @@ constexpr uint64_t kMaxChunk = 1ull << 23;
 constexpr int kBlock = 256;
 
 // counters[] layout
-constexpr int kCntQueue0 = 0, kCntQueue1 = 1, kCntShadow = 2, kCntDropped = 5,
+constexpr int kCntQueue0 = 0, kCntQueue1 = 1, kCntShadow = 2, kCntShadowEnv = 3, kCntDropped = 5,
               kCntStats = 8;  // 64-bit stats at counters+8: [0] ext rays, [1] shadow rays, [2] nodes, [3] triangles
 // Work claims of the persistent traversal kernels. The batch space of a queue
 // is split into kFetchParts interleaved partitions, each with its own counter
@@ -48,8 +49,9 @@ constexpr int kCntQueue0 = 0, kCntQueue1 = 1, kCntShadow = 2, kCntDropped = 5,
 constexpr int kFetchParts = 8;
 constexpr uint32_t kFetchStride = 64;  // uint32 slots between partition counters
 constexpr uint32_t kFetchBatch = 64;
-constexpr int kCntFetchExt = 64, kCntFetchShadow = kCntFetchExt + kFetchParts * kFetchStride;
-constexpr uint32_t kCounterWords = kCntFetchShadow + kFetchParts * kFetchStride;
+constexpr int kCntFetchExt = 64, kCntFetchShadow = kCntFetchExt + kFetchParts * kFetchStride,
+              kCntFetchShadowEnv = kCntFetchShadow + kFetchParts * kFetchStride;
+constexpr uint32_t kCounterWords = kCntFetchShadowEnv + kFetchParts * kFetchStride;
 
 struct FrameParams {
     SceneData sd;
@@ -63,6 +65,7 @@ struct FrameParams {
     uint32_t neeEnabled, bsdfOnly, neeOnly;
     uint32_t depth;
     uint32_t traceShadowAlways;
+    uint32_t envNee;  // environment importance sampling for NEE
     Spec spec;
 };
 
@@ -203,6 +206,52 @@ DEV F3 environmentRadiance(const SceneView& sv, const SceneData& sd, F3 dirIn) {
     return lerp(a, b, ty) * sd.environmentLight[3];
 }
 
+// Environment importance sampling (an addition; the reference gathers the
+// environment only on BSDF misses). Texel k is chosen with probability
+// envPdf[k] (luminance x sin(theta) plus a floor, host table) through its alias
+// table, the direction uniformly inside the texel in (u, v); the solid-angle pdf
+// is envPdf[k] * W * H / (2 pi^2 sin(theta)). The (u, v) mapping is the inverse
+// of environmentRadiance's.
+struct EnvSample {
+    F3 wi;
+    float pdf;
+    bool valid;
+};
+DEV EnvSample sampleEnvironmentDirection(const SceneView& sv, const SceneData& sd, Rng& rng) {
+    EnvSample e;
+    e.valid = false;
+    const uint32_t W = sv.envWidth, H = sv.envHeight, n = W * H;
+    float u1 = rnd(rng);
+    float u2 = rnd(rng);
+    float u3 = rnd(rng);
+    float u4 = rnd(rng);
+    uint32_t k = min((uint32_t)(u1 * (float)n), n - 1u);
+    if (!(u2 < sv.envAliasQ[k])) k = sv.envAliasIdx[k];
+    uint32_t x = k % W, y = k / W;
+    float u = ((float)x + u3) / (float)W;
+    float v = ((float)y + u4) / (float)H;
+    float phi = (u - 0.5f) * (2.0f * kPi) - sd.environmentRotation * (kPi / 180.0f);
+    float theta = v * kPi;
+    float st = sinf(theta);
+    if (!(st > 0.0f)) return e;
+    e.wi = F3{st * cosf(phi), st * sinf(phi), cosf(theta)};
+    e.pdf = sv.envPdf[k] * (float)n / (2.0f * kPi * kPi * st);
+    e.valid = e.pdf > 0.0f;
+    return e;
+}
+DEV float environmentPdf(const SceneView& sv, const SceneData& sd, F3 dirIn) {
+    F3 d = normalize(dirIn);
+    float phi = atan2f(d.y, d.x) + sd.environmentRotation * (kPi / 180.0f);
+    float z = clampf(d.z, -1.0f, 1.0f);
+    float u = fracf(phi * (0.5f * kInvPi) + 0.5f);
+    float v = acosf(z) * kInvPi;
+    const uint32_t W = sv.envWidth, H = sv.envHeight;
+    uint32_t x = min((uint32_t)(u * (float)W), W - 1u), y = min((uint32_t)(v * (float)H), H - 1u);
+    float st = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+    if (!(st > 0.0f)) return 0.0f;
+    return sv.envPdf[(size_t)y * W + x] * (float)(W * H) / (2.0f * kPi * kPi * st);
+}
+
 // src/shaders/light/direct/mis_weights.slang:6-42
 DEV float areaToSolidAngle(float pdfArea, F3 gn, F3 d, float t) {
     if (pdfArea <= 0.0f) return 0.0f;
@@ -309,12 +358,13 @@ __global__ void kPrepDepth(uint32_t* counters, int cur, int nxt) {
         stats[0] += counters[cur];
         counters[nxt] = 0u;
         counters[kCntShadow] = 0u;
+        counters[kCntShadowEnv] = 0u;
     }
-    if (t < 2u * kFetchParts) counters[kCntFetchExt + t * kFetchStride] = 0u;
+    if (t < 3u * kFetchParts) counters[kCntFetchExt + t * kFetchStride] = 0u;
 }
 __global__ void kPrepShadow(uint32_t* counters) {
     unsigned long long* stats = reinterpret_cast<unsigned long long*>(counters + kCntStats);
-    stats[1] += counters[kCntShadow];
+    stats[1] += counters[kCntShadow] + counters[kCntShadowEnv];
 }
 __global__ void kSetCount(uint32_t* counters, int slot, uint32_t value) { counters[slot] = value; }
 
@@ -324,10 +374,10 @@ template <bool COUNT>
 __global__ __launch_bounds__(kTraceBlock) void kExtend(SceneView sv, const float4* ro, const float4* rd, float4* hits,
                                                       uint32_t* counters, int countSlot, uint32_t* gstack) {
     uint32_t nodeVisits = 0, triTests = 0;
-    __shared__ uint32_t lds[kLdsStack * kTraceBlock];
+    __shared__ uint32_t lds[kLdsStackWords];
     TraversalStack st;
     st.lds = lds + threadIdx.x;
-    st.glob = gstack + (size_t)(blockIdx.x * kTraceBlock + threadIdx.x) * kGlobalStack;
+    st.glob = gstack + (size_t)(blockIdx.x * kTraceBlock + threadIdx.x) * kGlobalStackWords;
     st.dropped = 0;
     const uint32_t n = counters[countSlot];
     if (n == 0u) return;
@@ -361,36 +411,39 @@ DEV uint32_t classifyOccluder(const SceneView& sv, uint32_t tri) {
     return m.transmission > 0.0f ? kVisUnsupported : kVisOccluded;
 }
 
+// Any-hit queries of one shadow queue. Each record carries the contribution if
+// the segment is visible (a) and if it is not (b); at most one record per path
+// is in a queue, so the radiance update needs no atomics. Light rays follow the
+// reference's classification (a transmissive first occluder clears prevNee,
+// src/shaders/entry/shadow/closest_hit.slang:4-8); environment rays are visible
+// or not.
 template <int MODE>
 __global__ __launch_bounds__(kTraceBlock) void kShadow(SceneView sv, PathView pv, ShadowView sh, uint32_t* counters,
+                                                      int countSlot, int fetchSlot, uint32_t envPass,
                                                       uint32_t* gstack) {
-    __shared__ uint32_t lds[kLdsStack * kTraceBlock];
+    __shared__ uint32_t lds[kLdsStackWords];
     TraversalStack st;
     st.lds = lds + threadIdx.x;
-    st.glob = gstack + (size_t)(blockIdx.x * kTraceBlock + threadIdx.x) * kGlobalStack;
+    st.glob = gstack + (size_t)(blockIdx.x * kTraceBlock + threadIdx.x) * kGlobalStackWords;
     st.dropped = 0;
-    const uint32_t n = counters[kCntShadow];
+    const uint32_t n = counters[countSlot];
     if (n == 0u) return;
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t part = blockIdx.x % kFetchParts;
-    uint32_t base = n, round = kFetchBatch;
     while (true) {
-        if (round == kFetchBatch) {
-            base = claimBatch(counters + kCntFetchShadow, part, n);
-            round = 0;
-        }
+        const uint32_t base = claimBatch(counters + fetchSlot, part, n);
         if (base >= n) break;
-        uint32_t i = base + round + lane;
-        round += 64u;
+        uint32_t i = base + lane;
         if (i >= n) continue;
         float4 o = sh.o[i], d = sh.d[i];
         uint32_t q = __float_as_uint(o.w);
         TraversalRay r = makeTraversalRay(xyz(o), xyz(d));
-        uint32_t vis = classifyOccluder(sv, traceAny(sv, r, kRayTMin, d.w, st));
+        uint32_t tri = traceAny(sv, r, kRayTMin, d.w, st);
+        uint32_t vis = envPass ? (tri == kNoHit ? kVisVisible : kVisOccluded) : classifyOccluder(sv, tri);
+        if (vis != kVisVisible && envPass) continue;  // b is zero for environment rays
         F4 c = ld4(vis == kVisVisible ? sh.a[i] : sh.b[i]);
         uint32_t flags = pv.flags[q];
-        uint32_t newFlags = flags;
-        if (vis == kVisUnsupported) newFlags &= ~kFPrevNee;
+        if (vis == kVisUnsupported) pv.flags[q] = flags & ~kFPrevNee;
         if (MODE == kModeRgb) {
             F4 rad = ld4(pv.rad[q]);
             pv.rad[q] = st4(F3{rad.x, rad.y, rad.z} + F3{c.x, c.y, c.z}, 0.0f);
@@ -398,25 +451,13 @@ __global__ __launch_bounds__(kTraceBlock) void kShadow(SceneView sv, PathView pv
             F4 rad = ld4(pv.rad[q]);
             rad.x += c.x;
             pv.rad[q] = st4(rad);
+        } else if (flags & (kFHeroActive | kFPendingCollapse)) {
+            pv.rad[q] = st4(ld4(pv.rad[q]) + c);
         } else {
-            if (flags & kFPendingCollapse) {
-                // src/shaders/integrator/path/spectral_hero/transport.slang:77-87, deferred until this
-                // vertex's contribution is known
-                F4 rad = ld4(pv.rad[q]) + c;
-                F4 rx = ld4(pv.radXYZ[q]);
-                F3 xyzv = F3{rx.x, rx.y, rx.z} + toXYZ4(rad, ld4(pv.lam[q]), f4(470.0f));
-                pv.radXYZ[q] = make_float4(xyzv.x, xyzv.y, xyzv.z, rx.w);
-                pv.rad[q] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                newFlags &= ~kFPendingCollapse;
-            } else if (flags & kFHeroActive) {
-                pv.rad[q] = st4(ld4(pv.rad[q]) + c);
-            } else {
-                F4 rx = ld4(pv.radXYZ[q]);
-                rx.w += c.x;
-                pv.radXYZ[q] = st4(rx);
-            }
+            F4 rx = ld4(pv.radXYZ[q]);
+            rx.w += c.x;
+            pv.radXYZ[q] = st4(rx);
         }
-        if (newFlags != flags) pv.flags[q] = newFlags;
     }
     if (st.dropped) atomicAdd(&counters[kCntDropped], st.dropped);
 }
@@ -424,14 +465,14 @@ __global__ __launch_bounds__(kTraceBlock) void kShadow(SceneView sv, PathView pv
 // One path vertex for every ray of the current queue.
 template <int MODE>
 __global__ __launch_bounds__(kBlock) void kShade(FrameParams fp, SceneView sv, PathView pv, QueueView in,
-                                                 const float4* hits, QueueView out, ShadowView sh, uint32_t* counters,
-                                                 int inSlot, int outSlot) {
+                                                 const float4* hits, QueueView out, ShadowView sh, ShadowView she,
+                                                 uint32_t* counters, int inSlot, int outSlot) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t n = counters[inSlot];
-    bool cont = false, shadowEmit = false;
-    F3 nextO = f3(0.0f), nextD = f3(0.0f), shO = f3(0.0f), shD = f3(0.0f);
+    bool cont = false, shadowEmit = false, envEmit = false;
+    F3 nextO = f3(0.0f), nextD = f3(0.0f), shO = f3(0.0f), shD = f3(0.0f), seO = f3(0.0f), seD = f3(0.0f);
     float shT = 0.0f;
-    F4 shA = f4(0.0f), shB = f4(0.0f);
+    F4 shA = f4(0.0f), shB = f4(0.0f), seA = f4(0.0f);
     uint32_t q = 0;
     if (i < n) do {
         const SceneData& sd = fp.sd;
@@ -441,29 +482,48 @@ __global__ __launch_bounds__(kBlock) void kShade(FrameParams fp, SceneView sv, P
         float4 hh = hits[i];
         Hit h{hh.x, __float_as_uint(hh.y), hh.z, hh.w};
         uint32_t flags = pv.flags[q];
+        F4 lam = (MODE != kModeRgb) ? ld4(pv.lam[q]) : f4(0.0f);
+        if (MODE == kModeHero && (flags & kFPendingCollapse)) {
+            // collapse to the hero wavelength (src/shaders/integrator/path/spectral_hero/transport.slang:77-87),
+            // deferred from the previous vertex until its shadow rays had added their contributions
+            F4 rx = ld4(pv.radXYZ[q]);
+            F3 xyzv = F3{rx.x, rx.y, rx.z} + toXYZ4(ld4(pv.rad[q]), lam, f4(470.0f));
+            pv.radXYZ[q] = make_float4(xyzv.x, xyzv.y, xyzv.z, rx.w);
+            pv.rad[q] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            flags &= ~kFPendingCollapse;
+        }
         const bool absorbing = (flags & kFAbsorption) != 0u;
         const bool refractive = (flags & kFRefractive) != 0u;
         const bool heroActive = MODE == kModeHero && (flags & kFHeroActive) != 0u;
         F4 thr = ld4(pv.thr[q]);
         F4 misc = ld4(pv.misc[q]);  // prevBsdfPdf, scalar throughput, scalar wavelength, scalar 1/pdf
-        F4 lam = (MODE != kModeRgb) ? ld4(pv.lam[q]) : f4(0.0f);
 
         if (h.tri == kNoHit) {
             if (!fp.neeOnly && !(refractive || absorbing)) {
                 F3 env = environmentRadiance(sv, sd, d);
+                // MIS against environment NEE at the previous vertex (power heuristic; hero balance)
+                const bool prevEnv = (flags & kFPrevEnvNee) != 0u;
+                float envPdf = prevEnv ? environmentPdf(sv, sd, d) : 0.0f;
+                float w = 1.0f;
+                if (prevEnv && !heroActive && envPdf > 0.0f && misc.x > 0.0f) w = powerHeuristic(misc.x, envPdf);
                 if (MODE == kModeRgb) {
                     F4 rad = ld4(pv.rad[q]);
-                    pv.rad[q] = st4(F3{rad.x, rad.y, rad.z} + F3{thr.x, thr.y, thr.z} * env, 0.0f);
+                    pv.rad[q] = st4(F3{rad.x, rad.y, rad.z} + F3{thr.x, thr.y, thr.z} * (env * w), 0.0f);
                 } else if (MODE == kModeSingle) {
                     F4 rad = ld4(pv.rad[q]);
-                    rad.x += thr.x * spec1(fp.spec, env, lam.x);
+                    rad.x += thr.x * (spec1(fp.spec, env, lam.x) * w);
                     pv.rad[q] = st4(rad);
                 } else if (heroActive) {
                     F4 tpp = ld4(pv.tpp[q]);
-                    pv.rad[q] = st4(ld4(pv.rad[q]) + thr * heroBalance(tpp) * spec4(fp.spec, env, lam));
+                    float hw = heroBalance(tpp);
+                    if (prevEnv && envPdf > 0.0f) {
+                        F4 ptpp = ld4(pv.ptpp[q]), pbtp = ld4(pv.pbtp[q]);
+                        hw = spectralMis(ptpp * pbtp, ptpp * envPdf);
+                    }
+                    pv.rad[q] = st4(ld4(pv.rad[q]) + thr * hw * spec4(fp.spec, env, lam));
                 } else {
                     F4 rx = ld4(pv.radXYZ[q]);
-                    rx.w += misc.y * spec1(fp.spec, env, misc.z);
+                    rx.w += misc.y * (spec1(fp.spec, env, misc.z) * w);
                     pv.radXYZ[q] = st4(rx);
                 }
             }
@@ -539,6 +599,50 @@ __global__ __launch_bounds__(kBlock) void kShade(FrameParams fp, SceneView sv, P
             }
         }
 
+        // environment NEE (an addition): drawn after the light NEE, traced by its own shadow pass
+        bool envTried = false;
+        if (!fp.bsdfOnly && fp.envNee && curNee && !absorbing && wo.z > 0.0f) {
+            envTried = true;
+            EnvSample es = sampleEnvironmentDirection(sv, sd, rng);
+            if (es.valid) {
+                F3 wl = toLocal(es.wi, sf.basis);
+                if (!(isRefractive(mat) && wl.z <= 0.0f)) {
+                    F3 le = environmentRadiance(sv, sd, es.wi);
+                    F4 ne = f4(0.0f);
+                    if (MODE == kModeRgb) {
+                        Eval e = evalScalar(fp.spec, st, wl);
+                        if (e.pdf > 0.0f) {
+                            float mis = powerHeuristic(es.pdf, e.pdf);
+                            F3 r = (e.value * fabsf(wl.z)) * mis * le / es.pdf;
+                            ne = F4{r.x, r.y, r.z, 0.0f};
+                        }
+                    } else if (heroActive) {
+                        F4 btp;
+                        F4 fc = evalHero(fp.spec, st, wl, lam, btp) * fabsf(wl.z);
+                        float mis = spectralMis(tpp * es.pdf, tpp * btp);
+                        if (mis > 0.0f) ne = fc * mis * spec4(fp.spec, le, lam) / es.pdf;
+                    } else {
+                        Eval e = evalScalar(fp.spec, st, wl);
+                        if (e.pdf > 0.0f) {
+                            float sv1 = e.value.x * fabsf(wl.z) * spec1(fp.spec, le, stateLambda);
+                            ne.x = powerHeuristic(es.pdf, e.pdf) * sv1 / es.pdf;
+                        }
+                    }
+                    if (anyPos(ne)) {
+                        envEmit = true;
+                        // start on the side the BSDF continuation ray for this direction would
+                        // (+Ng for the shading reflection hemisphere), so both strategies see
+                        // the same visibility and their MIS weights cover the same integrand
+                        F3 off = wl.z > 0.0f ? sf.ng : -sf.ng;
+                        seO = sf.p + off * kShadowOriginOffset;
+                        seD = es.wi;
+                        F4 thrE = (MODE == kModeHero && !heroActive) ? F4{misc.y, 0.0f, 0.0f, 0.0f} : thr;
+                        seA = thrE * ne;
+                    }
+                }
+            }
+        }
+
         // emission with MIS (src/shaders/integrator/path/loop.slang:62-81, mis_weights.slang)
         F4 em = f4(0.0f);
         bool hasEm = false;
@@ -606,7 +710,7 @@ __global__ __launch_bounds__(kBlock) void kShade(FrameParams fp, SceneView sv, P
         uint32_t isT;
         if (!sampleLobe(st, rng, wl, isT)) break;
         F3 wi = toWorld(wl, sf.basis);
-        uint32_t newFlags = flags & ~(kFPrevNee | kFPendingCollapse);
+        uint32_t newFlags = flags & ~(kFPrevNee | kFPendingCollapse | kFPrevEnvNee);
         if (heroActive) {
             F4 btp;
             F4 val = evalHero(fp.spec, st, wl, lam, btp);
@@ -627,7 +731,7 @@ __global__ __launch_bounds__(kBlock) void kShade(FrameParams fp, SceneView sv, P
                 misc.w = 470.0f;
                 misc.x = btp.x;
                 newFlags &= ~kFHeroActive;
-                if (shadowEmit) {
+                if (shadowEmit || envEmit) {
                     newFlags |= kFPendingCollapse;
                 } else {
                     F4 rx = ld4(pv.radXYZ[q]);
@@ -656,6 +760,7 @@ __global__ __launch_bounds__(kBlock) void kShade(FrameParams fp, SceneView sv, P
         }
         const bool nowHero = MODE == kModeHero && (newFlags & kFHeroActive) != 0u;
         if (curNee && isT == 0u) newFlags |= kFPrevNee;
+        if (envTried && isT == 0u) newFlags |= kFPrevEnvNee;
         if (nowHero)
             updateMedium4(fp.spec, mat, sf.frontFace, isT, lam, newFlags, sigma);
         else
@@ -697,11 +802,6 @@ __global__ __launch_bounds__(kBlock) void kShade(FrameParams fp, SceneView sv, P
         if (MODE == kModeHero) pv.tpp[q] = st4(tpp);
     } while (false);
 
-    // a path that emitted a shadow ray but stopped here still needs its flags
-    // for the deferred radiance update
-    if (shadowEmit && !cont && MODE == kModeHero) {
-        // flags only matter for the pending-collapse case, which implies cont
-    }
     uint32_t slot = waveAppend(cont, &counters[outSlot]);
     if (cont) {
         out.o[slot] = make_float4(nextO.x, nextO.y, nextO.z, __uint_as_float(q));
@@ -713,6 +813,12 @@ __global__ __launch_bounds__(kBlock) void kShade(FrameParams fp, SceneView sv, P
         sh.d[sslot] = make_float4(shD.x, shD.y, shD.z, shT);
         sh.a[sslot] = st4(shA);
         sh.b[sslot] = st4(shB);
+    }
+    uint32_t eslot = waveAppend(envEmit, &counters[kCntShadowEnv]);
+    if (envEmit) {
+        she.o[eslot] = make_float4(seO.x, seO.y, seO.z, __uint_as_float(q));
+        she.d[eslot] = make_float4(seD.x, seD.y, seD.z, kRayTMax);
+        she.a[eslot] = st4(seA);
     }
 }
 
@@ -804,10 +910,10 @@ __global__ void kFill4(float4* p, size_t n, float4 v) {
 
 __global__ __launch_bounds__(kTraceBlock) void kTraceQuery(SceneView sv, const float4* rays, uint32_t n,
                                                           float4* out, uint32_t* gstack, uint32_t anyHit) {
-    __shared__ uint32_t lds[kLdsStack * kTraceBlock];
+    __shared__ uint32_t lds[kLdsStackWords];
     TraversalStack st;
     st.lds = lds + threadIdx.x;
-    st.glob = gstack + (size_t)(blockIdx.x * kTraceBlock + threadIdx.x) * kGlobalStack;
+    st.glob = gstack + (size_t)(blockIdx.x * kTraceBlock + threadIdx.x) * kGlobalStackWords;
     st.dropped = 0;
     for (uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x; i < n; i += gridDim.x * kTraceBlock) {
         float4 a = rays[2 * i], b = rays[2 * i + 1];
@@ -825,10 +931,10 @@ __global__ __launch_bounds__(kTraceBlock) void kTraceQuery(SceneView sv, const f
 __global__ __launch_bounds__(kTraceBlock) void kPrimary(SceneView sv, SceneData sd, uint32_t rngPolicy,
                                                        uint64_t rngSeed, uint32_t sampleIndex, uint32_t* outMesh,
                                                        uint32_t* outPrim, uint32_t* gstack) {
-    __shared__ uint32_t lds[kLdsStack * kTraceBlock];
+    __shared__ uint32_t lds[kLdsStackWords];
     TraversalStack st;
     st.lds = lds + threadIdx.x;
-    st.glob = gstack + (size_t)(blockIdx.x * kTraceBlock + threadIdx.x) * kGlobalStack;
+    st.glob = gstack + (size_t)(blockIdx.x * kTraceBlock + threadIdx.x) * kGlobalStackWords;
     st.dropped = 0;
     uint32_t vw = sd.viewportRect[2], vh = sd.viewportRect[3];
     uint32_t n = vw * vh;
@@ -863,17 +969,20 @@ bool ensureTraversalScratch(mrt_context* ctx) {
         ctx->traceGrid = (uint32_t)std::max(1, ctx->cuCount) * 8u;
         static_assert(8 % kFetchParts == 0, "trace grid must cover every claim partition");
     }
-    size_t need = (size_t)ctx->traceGrid * kTraceBlock * kGlobalStack;
+    size_t need = (size_t)ctx->traceGrid * kTraceBlock * kGlobalStackWords;
     return ctx->traversalStack.reserve(need);
 }
 
-static bool ensurePaths(mrt_context* ctx, size_t n, int mode) {
+static bool ensurePaths(mrt_context* ctx, size_t n, int mode, bool lightShadows, bool envShadows) {
     PathBuffers& b = ctx->paths;
     bool ok = b.rng.reserve(n) && b.flags.reserve(n) && b.thr.reserve(n) && b.rad.reserve(n) && b.misc.reserve(n) &&
               b.sigma.reserve(n) && b.hits.reserve(n) && b.sampleRad.reserve(n);
     for (int qi = 0; qi < 2; qi++)
         for (int j = 0; j < 2; j++) ok = ok && b.rays[qi][j].reserve(n);
-    for (int j = 0; j < 4; j++) ok = ok && b.shadow[j].reserve(n);
+    if (lightShadows)
+        for (int j = 0; j < 4; j++) ok = ok && b.shadow[j].reserve(n);
+    if (envShadows)
+        for (int j = 0; j < 3; j++) ok = ok && b.shadowEnv[j].reserve(n);
     if (mode != kModeRgb) ok = ok && b.lam.reserve(n);
     if (mode == kModeHero)
         ok = ok && b.radXYZ.reserve(n) && b.tpp.reserve(n) && b.ptpp.reserve(n) && b.pbtp.reserve(n);
@@ -927,6 +1036,7 @@ static bool launchChunk(mrt_context* ctx, FrameParams fp, StageEvents& ev, bool 
         qv[qi].count = cnt + qi;
     }
     ShadowView shv{b.shadow[0].ptr, b.shadow[1].ptr, b.shadow[2].ptr, b.shadow[3].ptr, cnt + kCntShadow};
+    ShadowView shev{b.shadowEnv[0].ptr, b.shadowEnv[1].ptr, b.shadowEnv[2].ptr, nullptr, cnt + kCntShadowEnv};
     hipLaunchKernelGGL(kGenerate<MODE>, dim3(blocksFor(fp.count)), dim3(kBlock), 0, s, fp, pv, qv[0], ctx->accum.ptr);
     hipLaunchKernelGGL(kSetCount, dim3(1), dim3(1), 0, s, cnt, kCntQueue0, fp.count);
     int cur = 0;
@@ -945,13 +1055,17 @@ static bool launchChunk(mrt_context* ctx, FrameParams fp, StageEvents& ev, bool 
         if (timed) ev.end(s);
         timed = ev.begin(s, 1);
         hipLaunchKernelGGL(kShade<MODE>, dim3(blocksFor(fp.count)), dim3(kBlock), 0, s, fp, sv, pv, qv[cur], b.hits.ptr,
-                           qv[nxt], shv, cnt, cur, nxt);
+                           qv[nxt], shv, shev, cnt, cur, nxt);
         if (timed) ev.end(s);
-        if (fp.neeEnabled) {  // otherwise no shadow record can exist
+        if (fp.neeEnabled || fp.envNee) {  // otherwise no shadow record can exist
             hipLaunchKernelGGL(kPrepShadow, dim3(1), dim3(1), 0, s, cnt);
             timed = ev.begin(s, 2);
-            hipLaunchKernelGGL(kShadow<MODE>, dim3(ctx->traceGrid), dim3(kTraceBlock), 0, s, sv, pv, shv, cnt,
-                               ctx->traversalStack.ptr);
+            if (fp.neeEnabled)
+                hipLaunchKernelGGL(kShadow<MODE>, dim3(ctx->traceGrid), dim3(kTraceBlock), 0, s, sv, pv, shv, cnt,
+                                   kCntShadow, kCntFetchShadow, 0u, ctx->traversalStack.ptr);
+            if (fp.envNee)
+                hipLaunchKernelGGL(kShadow<MODE>, dim3(ctx->traceGrid), dim3(kTraceBlock), 0, s, sv, pv, shev, cnt,
+                                   kCntShadowEnv, kCntFetchShadowEnv, 1u, ctx->traversalStack.ptr);
             if (timed) ev.end(s);
         }
         cur = nxt;
@@ -1002,7 +1116,7 @@ extern "C" int mrt_render_frame(mrt_context* ctx, const SceneData* scene, const 
         setError("rgb2spec table not uploaded");
         return -2;
     }
-    if (ctx->triCount > 0 && ctx->bvhMaxDepth * 3u + 1u > (uint32_t)(kLdsStack + kGlobalStack)) {
+    if (ctx->triCount > 0 && ctx->bvhMaxDepth * (kBvhWidth - 1u) + 1u > (uint32_t)(kLdsStack + kGlobalStack)) {
         setError("BVH deeper than the traversal stack supports");
         return -2;
     }
@@ -1029,8 +1143,6 @@ extern "C" int mrt_render_frame(mrt_context* ctx, const SceneData* scene, const 
     uint64_t totalPaths = numPixels * spp;
     uint64_t chunk = std::min<uint64_t>(totalPaths, kMaxChunk);
     if (numPixels > 0 && numPixels <= kMaxChunk) chunk = std::min<uint64_t>(totalPaths, (kMaxChunk / numPixels) * numPixels);
-    if (chunk > 0 && !ensurePaths(ctx, chunk, mode)) return -3;
-
     FrameParams fp;
     memset(&fp, 0, sizeof(fp));
     fp.sd = *scene;
@@ -1043,6 +1155,9 @@ extern "C" int mrt_render_frame(mrt_context* ctx, const SceneData* scene, const 
     fp.bsdfOnly = scene->debugMode == VKRT_DEBUG_MODE_BSDF_ONLY;
     fp.neeOnly = scene->debugMode == VKRT_DEBUG_MODE_NEE_ONLY;
     fp.traceShadowAlways = prm->hasTransmissive ? 1u : 0u;
+    fp.envNee = prm->envImportance != 0u && ctx->envWidth > 0u && ctx->envAliasQ.ptr &&
+                scene->environmentTextureIndex != VKRT_INVALID_INDEX;
+    if (chunk > 0 && !ensurePaths(ctx, chunk, mode, fp.neeEnabled != 0u, fp.envNee != 0u)) return -3;
     fp.spec.table = ctx->rgb2spec.ptr;
     fp.spec.res = scene->rgb2specSRGB.res;
     fp.spec.scaleOffset = scene->rgb2specSRGB.scaleOffset;

@@ -381,6 +381,11 @@ VKRT_Result VKRT_exportSceneBuffers(VKRT* vkrt, VKRT_SceneBuffers* out) {
     out->rngPolicy = vkrt->rngPolicy;
     out->rngSeed = vkrt->rngSeed;
     out->environmentImportanceSampling = vkrt->envImportance;
+    if (out->environmentPixels) {
+        out->environmentAliasQ = p->envAliasQ;
+        out->environmentAliasIdx = p->envAliasIdx;
+        out->environmentPdf = p->envPdf;
+    }
     return VKRT_SUCCESS;
 }
 

@@ -10,6 +10,9 @@ displaced_sphere -- geodesic icosphere of frequency f (20*f^2 triangles; f = 229
                  3-octave value noise of amplitude 0.1*radius, split into 64
                  meshes by material id = hash(face) mod 64.
 sky_environment -- lat-long RGBA32F sky gradient with a 0.5 degree sun disc.
+stress_scene  -- 16 instances of the displaced sphere under seeded rigid
+                 transforms plus 4M random small triangles (edge U(1e-3, 1e-2))
+                 in the unit cube, all diffuse: about 20.8M triangles.
 Scenes are returned as plain data and loaded through the C API by `load`.
 """
 
@@ -27,6 +30,7 @@ class MeshData:
     vertices: np.ndarray  # N x 20 float32 (reference Vertex layout)
     indices: np.ndarray  # M x 3 uint32
     material: int  # index into Scene.materials
+    transform: np.ndarray | None = None  # 4 x 4 world transform (row-major), identity if None
 
 
 @dataclass
@@ -275,6 +279,9 @@ def load(renderer: Renderer, scene: Scene):
     for mesh in scene.meshes:
         idx = renderer.add_mesh(mesh.vertices, mesh.indices)
         renderer.set_mesh_material(idx, mat_ids[mesh.material])
+        if mesh.transform is not None:
+            m = np.asarray(mesh.transform, dtype=np.float32)
+            renderer.set_mesh_transform_matrix(idx, np.ascontiguousarray(m.T).reshape(-1))
     pos, target, up, vfov = scene.camera
     renderer.set_camera(pos, target, up, vfov)
     if scene.environment is not None:
@@ -283,3 +290,60 @@ def load(renderer: Renderer, scene: Scene):
         renderer.set_environment_light((1.0, 1.0, 1.0), 1.0)
     else:
         renderer.set_environment_light(scene.environment_color, scene.environment_strength)
+
+
+# ---------------------------------------------------------------- stress --
+def _random_rotation(rng: np.random.Generator) -> np.ndarray:
+    """Uniform random rotation from a normalised Gaussian quaternion."""
+    q = rng.normal(size=4)
+    w, x, y, z = q / np.linalg.norm(q)
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+
+def random_triangles(count: int, seed: int = 5, edge=(1e-3, 1e-2)) -> MeshData:
+    """`count` independent triangles in the unit cube: a random centre, a random
+    orientation and edges of length U(edge) along two random directions."""
+    rng = np.random.default_rng(seed)
+    c = rng.random((count, 3))
+    e1 = rng.normal(size=(count, 3))
+    e1 /= np.linalg.norm(e1, axis=1, keepdims=True)
+    e2 = rng.normal(size=(count, 3))
+    e2 -= np.sum(e2 * e1, axis=1, keepdims=True) * e1
+    e2 /= np.linalg.norm(e2, axis=1, keepdims=True)
+    l1 = rng.uniform(edge[0], edge[1], (count, 1))
+    l2 = rng.uniform(edge[0], edge[1], (count, 1))
+    p0 = c - (l1 * e1 + l2 * e2) / 3.0
+    pos = np.stack([p0, p0 + l1 * e1, p0 + l2 * e2], axis=1).reshape(-1, 3)
+    nrm = np.repeat(np.cross(e1, e2), 3, axis=0)
+    verts = make_vertices(pos.astype(np.float32), nrm.astype(np.float32))
+    return MeshData(verts, np.arange(3 * count, dtype=np.uint32).reshape(-1, 3), 0)
+
+
+def stress_scene(instances: int = 16, freq: int = 229, random_count: int = 4_000_000, seed: int = 21) -> Scene:
+    """Incoherent-traversal stress configuration: `instances` copies of the
+    displaced sphere (one diffuse material, seeded rigid transforms within a
+    6 x 6 x 6 region around the unit cube) and `random_count` random small
+    triangles in the unit cube; the camera sits among the instances so every
+    primary ray hits geometry. freq 229 and 16 instances give 16,781,120 +
+    4,000,000 triangles."""
+    rng = np.random.default_rng(seed)
+    verts, tris = geodesic_sphere(freq)
+    disp = _value_noise(verts, 11) * 0.1
+    pos = verts * (1.0 + disp)[:, None]
+    nrm = _vertex_normals(pos, tris)
+    sphere = make_vertices(pos, nrm)
+    idx = tris.astype(np.uint32)
+    s = Scene()
+    s.materials = [lambert((0.7, 0.7, 0.7)), lambert((0.5, 0.6, 0.8))]
+    for k in range(instances):
+        m = np.eye(4)
+        m[:3, :3] = _random_rotation(rng)
+        m[:3, 3] = rng.uniform([-2.5, -2.5, -2.5], [3.5, 3.5, 3.5])
+        s.meshes.append(MeshData(sphere, idx, 0, m))
+    if random_count:
+        s.meshes.append(random_triangles(random_count, seed=seed + 1))
+        s.meshes[-1].material = 1
+    s.camera = ((0.5, -4.5, 1.2), (0.5, 0.5, 0.5), (0.0, 0.0, 1.0), 60.0)
+    return s
