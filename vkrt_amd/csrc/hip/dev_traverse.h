@@ -115,14 +115,33 @@ DEV float planeByte(uint32_t w0, uint32_t w1, int i) {
     return (float)(((i < 4 ? w0 : w1) >> ((i & 3) * 8)) & 0xFFu);
 }
 
+// The eight children of a node as registers: entry distances (misses at +inf)
+// and references. Kept as vector components, never as an indexable array, so
+// the compiler keeps them in VGPRs.
+struct Children {
+    float4 dLo, dHi;
+    uint4 rLo, rHi;
+    uint32_t mask;  // children whose box the ray overlaps
+};
+
+DEV void setComp(float4& v, int i, float x) {
+    if (i == 0) v.x = x;
+    else if (i == 1) v.y = x;
+    else if (i == 2) v.z = x;
+    else v.w = x;
+}
+
 // Slab test of the eight children of a node: decodes each child box, picks
-// near and far planes by ray octant, and returns the mask of children whose
-// box overlaps [tmin, tmax] with their entry distances in tn and their
-// references in ch. The far distance is widened by 4 ulp so that rounding in
-// the decode and slab arithmetic never rejects a box the ray touches.
-DEV uint32_t boxes8(const BvhNode* node, const TraversalRay& r, float tmin, float tmax, float tn[8], uint32_t ch[8]) {
+// near and far planes by ray octant, and reports which children overlap
+// [tmin, tmax] with their entry distances. The far distance is widened by a
+// few ulp so that rounding in the decode and slab arithmetic never rejects a
+// box the ray touches.
+DEV Children boxes8(const BvhNode* node, const TraversalRay& r, float tmin, float tmax) {
     const uint4* w = reinterpret_cast<const uint4*>(node);
-    const uint4 w0 = w[0], w1 = w[1], w2 = w[2], w3 = w[3], w4 = w[4], w5 = w[5];
+    const uint4 w0 = w[0], w1 = w[1], w2 = w[2], w3 = w[3];
+    Children c;
+    c.rLo = w[4];
+    c.rHi = w[5];
     const float ox = __uint_as_float(w0.x), oy = __uint_as_float(w0.y), oz = __uint_as_float(w0.z);
     const uint32_t packed = w0.w;
     const float sx = __uint_as_float((packed & 0xFFu) << 23);
@@ -136,15 +155,7 @@ DEV uint32_t boxes8(const BvhNode* node, const TraversalRay& r, float tmin, floa
     const uint32_t fy0 = r.negY ? w1.z : w3.x, fy1 = r.negY ? w1.w : w3.y;
     const uint32_t nz0 = r.negZ ? w3.z : w2.x, nz1 = r.negZ ? w3.w : w2.y;
     const uint32_t fz0 = r.negZ ? w2.x : w3.z, fz1 = r.negZ ? w2.y : w3.w;
-    ch[0] = w4.x;
-    ch[1] = w4.y;
-    ch[2] = w4.z;
-    ch[3] = w4.w;
-    ch[4] = w5.x;
-    ch[5] = w5.y;
-    ch[6] = w5.z;
-    ch[7] = w5.w;
-    uint32_t mask = 0;
+    c.mask = 0;
 #pragma unroll
     for (int i = 0; i < 8; i++) {
         float bnx = fmaf(planeByte(nx0, nx1, i), sx, ox), bfx = fmaf(planeByte(fx0, fx1, i), sx, ox);
@@ -155,10 +166,44 @@ DEV uint32_t boxes8(const BvhNode* node, const TraversalRay& r, float tmin, floa
         float t0z = fmaf(bnz, r.inv.z, -r.oInv.z), t1z = fmaf(bfz, r.inv.z, -r.oInv.z);
         float tnear = fmaxf(fmaxf(t0x, t0y), fmaxf(t0z, tmin));
         float tfar = fminf(fminf(t1x, t1y), fminf(t1z, tmax)) * 1.0000005f;
-        tn[i] = tnear;
-        mask |= (tnear <= tfar && (uint32_t)i < count) ? (1u << i) : 0u;
+        bool hit = tnear <= tfar && (uint32_t)i < count;
+        c.mask |= hit ? (1u << i) : 0u;
+        if (i < 4)
+            setComp(c.dLo, i, hit ? tnear : INFINITY);
+        else
+            setComp(c.dHi, i - 4, hit ? tnear : INFINITY);
     }
-    return mask;
+    return c;
+}
+
+// Sorts the children by entry distance (misses last) with a 19-comparator
+// network for 8 inputs (Knuth, TAOCP 5.3.4).
+DEV void sortChildren(Children& c) {
+    float d0 = c.dLo.x, d1 = c.dLo.y, d2 = c.dLo.z, d3 = c.dLo.w, d4 = c.dHi.x, d5 = c.dHi.y, d6 = c.dHi.z,
+          d7 = c.dHi.w;
+    uint32_t r0 = c.rLo.x, r1 = c.rLo.y, r2 = c.rLo.z, r3 = c.rLo.w, r4 = c.rHi.x, r5 = c.rHi.y, r6 = c.rHi.z,
+             r7 = c.rHi.w;
+#define MRT_CE(a, b)                           \
+    {                                          \
+        bool sw = d##b < d##a;                 \
+        float td = sw ? d##b : d##a;           \
+        d##b = sw ? d##a : d##b;               \
+        d##a = td;                             \
+        uint32_t tr = sw ? r##b : r##a;        \
+        r##b = sw ? r##a : r##b;               \
+        r##a = tr;                             \
+    }
+    MRT_CE(0, 2) MRT_CE(1, 3) MRT_CE(4, 6) MRT_CE(5, 7)
+    MRT_CE(0, 4) MRT_CE(1, 5) MRT_CE(2, 6) MRT_CE(3, 7)
+    MRT_CE(0, 1) MRT_CE(2, 3) MRT_CE(4, 5) MRT_CE(6, 7)
+    MRT_CE(2, 4) MRT_CE(3, 5)
+    MRT_CE(1, 4) MRT_CE(3, 6)
+    MRT_CE(1, 2) MRT_CE(3, 4) MRT_CE(5, 6)
+#undef MRT_CE
+    c.dLo = make_float4(d0, d1, d2, d3);
+    c.dHi = make_float4(d4, d5, d6, d7);
+    c.rLo = make_uint4(r0, r1, r2, r3);
+    c.rHi = make_uint4(r4, r5, r6, r7);
 }
 
 // Per-lane traversal stack of (reference, entry distance) pairs: kLdsStack
@@ -197,105 +242,84 @@ struct TraversalStack {
 constexpr int kLdsStackWords = 2 * kLdsStack * kTraceBlock;
 constexpr int kGlobalStackWords = 2 * kGlobalStack;
 
-// Child i of the mask with the smallest (largest) entry distance, chosen with
-// unrolled selects so tn and ch stay in registers.
-DEV int nearestChild(uint32_t mask, const float tn[8]) {
-    float best = INFINITY;
-    int bi = 0;
-#pragma unroll
-    for (int i = 0; i < 8; i++) {
-        bool take = ((mask >> i) & 1u) && tn[i] < best;
-        best = take ? tn[i] : best;
-        bi = take ? i : bi;
-    }
-    return bi;
-}
-DEV int farthestChild(uint32_t mask, const float tn[8]) {
-    float best = -INFINITY;
-    int bi = 0;
-#pragma unroll
-    for (int i = 0; i < 8; i++) {
-        bool take = ((mask >> i) & 1u) && tn[i] >= best;
-        best = take ? tn[i] : best;
-        bi = take ? i : bi;
-    }
-    return bi;
-}
-DEV uint32_t selectRef(const uint32_t ch[8], int k) {
-    uint32_t r = ch[0];
-#pragma unroll
-    for (int i = 1; i < 8; i++) r = k == i ? ch[i] : r;
-    return r;
-}
-DEV float selectDist(const float tn[8], int k) {
-    float d = tn[0];
-#pragma unroll
-    for (int i = 1; i < 8; i++) d = k == i ? tn[i] : d;
-    return d;
+// Closest-hit traversal state of one ray.
+struct ClosestState {
+    Hit h;
+    uint32_t ref;    // next node or leaf to visit
+    uint32_t steps;  // visits so far (bounded by kMaxTraversalSteps)
+};
+
+DEV void closestBegin(ClosestState& cs, TraversalStack& st, float tmax) {
+    cs.h.t = tmax;
+    cs.h.tri = kNoHit;
+    cs.h.u = cs.h.v = 0.0f;
+    cs.ref = 0;
+    cs.steps = 0;
+    st.sp = 0;
 }
 
-// Closest hit in (tmin, tmax). The nearest overlapping child is visited next
-// and the others are pushed farthest first; a popped entry whose entry
-// distance lies beyond the closest hit found so far is skipped.
-DEV Hit traceClosest(const SceneView& s, const TraversalRay& r, float tmin, float tmax, TraversalStack& st,
+// One node or leaf visit of a closest-hit traversal in (tmin, h.t). The
+// nearest overlapping child is visited next and the others are pushed farthest
+// first; a popped entry whose entry distance lies beyond the closest hit found
+// so far is skipped. Returns false once the traversal has finished.
+DEV bool closestStep(const SceneView& s, const TraversalRay& r, float tmin, ClosestState& cs, TraversalStack& st,
                      uint32_t* nodeVisits = nullptr, uint32_t* triTests = nullptr) {
-    Hit h;
-    h.t = tmax;
-    h.tri = kNoHit;
-    h.u = h.v = 0.0f;
-    if (s.nodeCount == 0) return h;
-    st.sp = 0;
-    uint32_t ref = 0;
-    for (uint32_t step = 0;; step++) {
-        if (step >= kMaxTraversalSteps) {
-            st.dropped++;
-            break;
+    if (++cs.steps > kMaxTraversalSteps) {
+        st.dropped++;
+        return false;
+    }
+    bool popNext = true;
+    if (!(cs.ref & kLeafBit)) {
+        if (nodeVisits) (*nodeVisits)++;
+        Children c = boxes8(s.nodes + cs.ref, r, tmin, cs.h.t);
+        const int hits = __popc(c.mask);
+        if (hits > 0) {
+            sortChildren(c);
+            // visit the nearest next; push the others farthest first
+            cs.ref = c.rLo.x;
+            if (hits > 7) st.push(c.rHi.w, c.dHi.w);
+            if (hits > 6) st.push(c.rHi.z, c.dHi.z);
+            if (hits > 5) st.push(c.rHi.y, c.dHi.y);
+            if (hits > 4) st.push(c.rHi.x, c.dHi.x);
+            if (hits > 3) st.push(c.rLo.w, c.dLo.w);
+            if (hits > 2) st.push(c.rLo.z, c.dLo.z);
+            if (hits > 1) st.push(c.rLo.y, c.dLo.y);
+            popNext = false;
         }
-        bool popNext = true;
-        if (!(ref & kLeafBit)) {
-            if (nodeVisits) (*nodeVisits)++;
-            float tn[8];
-            uint32_t ch[8];
-            uint32_t mask = boxes8(s.nodes + ref, r, tmin, h.t, tn, ch);
-            if (mask != 0u) {
-                int ni = nearestChild(mask, tn);
-                ref = selectRef(ch, ni);
-                mask &= ~(1u << ni);
-                while (mask != 0u) {
-                    int fi = farthestChild(mask, tn);
-                    st.push(selectRef(ch, fi), selectDist(tn, fi));
-                    mask &= ~(1u << fi);
-                }
-                popNext = false;
+    } else {
+        uint32_t first = leafFirst(cs.ref), count = leafCount(cs.ref);
+        for (uint32_t i = 0; i < count; i++) {
+            if (triTests) (*triTests)++;
+            const TriRecord tr = s.tris[first + i];
+            float t, u, v;
+            if (intersectTriangle(tr, r.o, r.pre, tmin, cs.h.t, t, u, v)) {
+                cs.h.t = t;
+                cs.h.tri = __float_as_uint(tr.v0.w);
+                cs.h.u = u;
+                cs.h.v = v;
             }
-        } else {
-            uint32_t first = leafFirst(ref), count = leafCount(ref);
-            for (uint32_t i = 0; i < count; i++) {
-                if (triTests) (*triTests)++;
-                const TriRecord tr = s.tris[first + i];
-                float t, u, v;
-                if (intersectTriangle(tr, r.o, r.pre, tmin, h.t, t, u, v)) {
-                    h.t = t;
-                    h.tri = __float_as_uint(tr.v0.w);
-                    h.u = u;
-                    h.v = v;
-                }
-            }
-        }
-        if (popNext) {
-            bool found = false;
-            while (st.sp > 0) {
-                float d;
-                st.pop(ref, d);
-                if (d <= h.t) {
-                    found = true;
-                    break;
-                }
-            }
-            if (!found) break;
         }
     }
-    return h;
+    if (popNext) {
+        while (st.sp > 0) {
+            float d;
+            st.pop(cs.ref, d);
+            if (d <= cs.h.t) return true;
+        }
+        return false;
+    }
+    return true;
+}
+
+// Closest hit in (tmin, tmax).
+DEV Hit traceClosest(const SceneView& s, const TraversalRay& r, float tmin, float tmax, TraversalStack& st,
+                     uint32_t* nodeVisits = nullptr, uint32_t* triTests = nullptr) {
+    ClosestState cs;
+    closestBegin(cs, st, tmax);
+    if (s.nodeCount == 0) return cs.h;
+    while (closestStep(s, r, tmin, cs, st, nodeVisits, triTests)) {
+    }
+    return cs.h;
 }
 
 // Any hit in (tmin, tmax): the first accepted intersection ends the search, as
@@ -311,18 +335,21 @@ DEV uint32_t traceAny(const SceneView& s, const TraversalRay& r, float tmin, flo
         }
         bool popNext = true;
         if (!(ref & kLeafBit)) {
-            float tn[8];
-            uint32_t ch[8];
-            uint32_t mask = boxes8(s.nodes + ref, r, tmin, tmax, tn, ch);
-            if (mask != 0u) {
-                int ni = nearestChild(mask, tn);
-                ref = selectRef(ch, ni);
-                mask &= ~(1u << ni);
-                while (mask != 0u) {
-                    int k = __ffs(mask) - 1;
-                    st.push(selectRef(ch, k), 0.0f);
-                    mask &= mask - 1u;
-                }
+            const Children c = boxes8(s.nodes + ref, r, tmin, tmax);
+            if (c.mask != 0u) {
+                // visit the lowest overlapping slot next, push the others
+                const uint32_t m = c.mask;
+                const int first = __ffs(m) - 1;
+                ref = first == 0 ? c.rLo.x : first == 1 ? c.rLo.y : first == 2 ? c.rLo.z : first == 3 ? c.rLo.w
+                    : first == 4 ? c.rHi.x : first == 5 ? c.rHi.y : first == 6 ? c.rHi.z : c.rHi.w;
+                const uint32_t rest = m & (m - 1u);
+                if (rest & 0x02u) st.push(c.rLo.y, 0.0f);
+                if (rest & 0x04u) st.push(c.rLo.z, 0.0f);
+                if (rest & 0x08u) st.push(c.rLo.w, 0.0f);
+                if (rest & 0x10u) st.push(c.rHi.x, 0.0f);
+                if (rest & 0x20u) st.push(c.rHi.y, 0.0f);
+                if (rest & 0x40u) st.push(c.rHi.z, 0.0f);
+                if (rest & 0x80u) st.push(c.rHi.w, 0.0f);
                 popNext = false;
             }
         } else {

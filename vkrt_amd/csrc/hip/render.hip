@@ -368,8 +368,17 @@ __global__ void kPrepShadow(uint32_t* counters) {
 }
 __global__ void kSetCount(uint32_t* counters, int slot, uint32_t value) { counters[slot] = value; }
 
-// Closest-hit traversal, persistent: each wavefront claims 64 consecutive rays at a time.
+// Closest-hit traversal, persistent with lane refill. Every loop iteration
+// advances each busy lane by one node or leaf visit; once kRefillLanes lanes of
+// a wavefront are idle, the wavefront claims that many new rays with one atomic,
+// so lanes do not wait for the slowest ray of a batch. The queue is split into
+// kFetchParts contiguous partitions, each with its own claim counter (its own
+// 256-byte line); a wavefront starts in the partition of its block and moves on
+// to the next partition once one is exhausted.
 // COUNT adds the number of BVH nodes visited and triangles tested to the frame statistics.
+constexpr uint32_t kRefillLanes = 16;
+constexpr uint32_t kNoRay = 0xFFFFFFFFu;
+
 template <bool COUNT>
 __global__ __launch_bounds__(kTraceBlock) void kExtend(SceneView sv, const float4* ro, const float4* rd, float4* hits,
                                                       uint32_t* counters, int countSlot, uint32_t* gstack) {
@@ -379,23 +388,60 @@ __global__ __launch_bounds__(kTraceBlock) void kExtend(SceneView sv, const float
     st.lds = lds + threadIdx.x;
     st.glob = gstack + (size_t)(blockIdx.x * kTraceBlock + threadIdx.x) * kGlobalStackWords;
     st.dropped = 0;
+    st.sp = 0;
     const uint32_t n = counters[countSlot];
-    if (n == 0u) return;
+    if (n == 0u || sv.nodeCount == 0u) {
+        // empty queue, or no geometry: every ray misses
+        for (uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x; i < n; i += gridDim.x * kTraceBlock)
+            hits[i] = make_float4(rd[i].w, __uint_as_float(kNoHit), 0.0f, 0.0f);
+        return;
+    }
     const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t part = blockIdx.x % kFetchParts;
+    const uint64_t lanesBelow = (1ull << lane) - 1ull;
+    const uint32_t partSize = (n + kFetchParts - 1u) / kFetchParts;
+    uint32_t part = blockIdx.x % kFetchParts;
+    uint32_t partsTried = 0;  // partitions found exhausted by this wavefront
+    uint32_t rayId = kNoRay;
+    TraversalRay r;
+    ClosestState cs;
+    cs.ref = 0;
+    cs.steps = 0;
+    cs.h.t = 0.0f;
+    cs.h.tri = kNoHit;
+    cs.h.u = cs.h.v = 0.0f;
     while (true) {
-        const uint32_t base = claimBatch(counters + kCntFetchExt, part, n);
-        if (base >= n) break;
-        for (uint32_t r0 = 0; r0 < kFetchBatch; r0 += 64u) {
-            uint32_t i = base + r0 + lane;
-            if (i < n) {
-                float4 a = ro[i], b = rd[i];
-                TraversalRay r = makeTraversalRay(xyz(a), xyz(b));
-                Hit h = traceClosest(sv, r, kRayTMin, b.w, st, COUNT ? &nodeVisits : nullptr,
-                                     COUNT ? &triTests : nullptr);
-                hits[i] = make_float4(h.t, __uint_as_float(h.tri), h.u, h.v);
+        const uint64_t idleMask = __ballot(rayId == kNoRay);
+        const uint32_t idle = (uint32_t)__popcll(idleMask);
+        if (partsTried < kFetchParts && (idle >= kRefillLanes || idle == 64u)) {
+            // claim `idle` rays of the current partition; lanes beyond its end stay idle
+            uint32_t base = 0;
+            if (lane == 0u) base = atomicAdd(&counters[kCntFetchExt + part * kFetchStride], idle);
+            base = (uint32_t)__shfl((int)base, 0);
+            const uint32_t begin = part * partSize, end = min(n, begin + partSize);
+            if (rayId == kNoRay) {
+                const uint32_t k = begin + base + (uint32_t)__popcll(idleMask & lanesBelow);
+                if (base < partSize && k < end) {
+                    float4 a = ro[k], b = rd[k];
+                    r = makeTraversalRay(xyz(a), xyz(b));
+                    closestBegin(cs, st, b.w);
+                    rayId = k;
+                }
+            }
+            if (base + idle >= partSize) {  // wave-uniform: this partition is used up
+                part = (part + 1u) % kFetchParts;
+                partsTried++;
             }
         }
+        if (rayId != kNoRay) {
+            if (!closestStep(sv, r, kRayTMin, cs, st, COUNT ? &nodeVisits : nullptr, COUNT ? &triTests : nullptr)) {
+                hits[rayId] = make_float4(cs.h.t, __uint_as_float(cs.h.tri), cs.h.u, cs.h.v);
+                rayId = kNoRay;
+            }
+        } else if (partsTried >= kFetchParts) {
+            // nothing left to claim; this lane is done, but the wavefront keeps
+            // iterating until every lane has finished its ray
+        }
+        if (partsTried >= kFetchParts && __ballot(rayId != kNoRay) == 0ull) break;
     }
     if (st.dropped) atomicAdd(&counters[kCntDropped], st.dropped);
     if (COUNT) {
