@@ -508,9 +508,12 @@ __global__ __launch_bounds__(kTraceBlock) void kShadow(SceneView sv, PathView pv
     if (st.dropped) atomicAdd(&counters[kCntDropped], st.dropped);
 }
 
-// One path vertex for every ray of the current queue.
+// One path vertex for every ray of the current queue. The hero-wavelength
+// variant needs about 180 registers unconstrained (two waves per SIMD); asking
+// for three waves per SIMD costs a handful of spilled registers and raises the
+// latency hiding of this memory-bound stage by half.
 template <int MODE>
-__global__ __launch_bounds__(kBlock) void kShade(FrameParams fp, SceneView sv, PathView pv, QueueView in,
+__global__ __launch_bounds__(kBlock, 3) void kShade(FrameParams fp, SceneView sv, PathView pv, QueueView in,
                                                  const float4* hits, QueueView out, ShadowView sh, ShadowView she,
                                                  uint32_t* counters, int inSlot, int outSlot) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
