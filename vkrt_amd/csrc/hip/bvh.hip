@@ -4,16 +4,20 @@
 //  1. per-triangle bounds and the centroid bounds of the scene
 //  2. 63-bit Morton codes (21 bits per axis) of triangle centroids
 //  3. radix sort of (code, triangle) pairs (rocPRIM)
-//  4. binary radix tree over the sorted codes (Karras, HPG 2012), each node
-//     covering a contiguous range of sorted triangles
-//  5. bottom-up bounds with one atomic arrival flag per node
-//  6. top-down collapse into 8-wide nodes: a node is opened greedily by
+//  4. binary tree by parallel locally-ordered clustering over the Morton order
+//     (PLOC, Meister and Bittner, TVCG 2018): every cluster finds the
+//     neighbour within kPlocRadius positions whose merged box has the least
+//     surface area, mutual nearest neighbours merge, survivors are compacted;
+//     repeated until one cluster remains. This refines the LBVH order into a
+//     tree of surface-area-heuristic quality.
+//  5. top-down collapse into 8-wide nodes: a node is opened greedily by
 //     splitting its largest-area child until eight children remain; subtrees
-//     of at most kLeafMaxTris triangles become leaves (level-synchronous);
-//     child boxes are quantised to 8 bits per plane, conservatively
-//  7. triangles reordered into leaf order as 48-byte records
+//     of at most kLeafMaxTris triangles become leaves whose triangles are
+//     written contiguously (depth-first order) as 48-byte records; child
+//     boxes are quantised to 8 bits per plane, conservatively
 #include <cstring>
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_select.hpp>
 
 #include <algorithm>
 #include <chrono>
@@ -104,89 +108,9 @@ __global__ __launch_bounds__(kB) void kMorton(const Aabb* boxes, uint32_t n, con
     values[i] = i;
 }
 
-// Binary radix tree (Karras 2012). Internal node i in [0, n-1); leaves are
-// encoded as (n - 1 + sortedIndex). range[node] = first/last sorted index.
-__device__ __forceinline__ int delta(const uint64_t* keys, int n, int i, int j) {
-    if (j < 0 || j >= n) return -1;
-    uint64_t a = keys[i], b = keys[j];
-    if (a == b) return 64 + __clz((uint32_t)(i ^ j));
-    return __clzll(a ^ b);
-}
-
-__global__ __launch_bounds__(kB) void kRadixTree(const uint64_t* keys, int n, int2* children, int* parent, int2* range) {
-    int i = blockIdx.x * kB + threadIdx.x;
-    if (i >= n - 1) return;
-    int d = (delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
-    int dMin = delta(keys, n, i, i - d);
-    int lMax = 2;
-    while (delta(keys, n, i, i + lMax * d) > dMin) lMax <<= 1;
-    int l = 0;
-    for (int t = lMax >> 1; t >= 1; t >>= 1)
-        if (delta(keys, n, i, i + (l + t) * d) > dMin) l += t;
-    int j = i + l * d;
-    int dNode = delta(keys, n, i, j);
-    int s = 0;
-    int div = 2;
-    int t = (l + div - 1) / div;
-    while (true) {
-        if (delta(keys, n, i, i + (s + t) * d) > dNode) s += t;
-        if (t == 1) break;
-        div <<= 1;
-        t = (l + div - 1) / div;
-    }
-    int gamma = i + s * d + min(d, 0);
-    int first = min(i, j), last = max(i, j);
-    int left = (first == gamma) ? (n - 1 + gamma) : gamma;
-    int right = (last == gamma + 1) ? (n - 1 + gamma + 1) : gamma + 1;
-    children[i] = make_int2(left, right);
-    range[i] = make_int2(first, last);
-    parent[left] = i;
-    parent[right] = i;
-}
-
-// bottom-up bounds: leaves start, the second arrival at a node merges
-__global__ __launch_bounds__(kB) void kRefit(const Aabb* triBoxes, const uint32_t* sortedTri, int n, const int2* children,
-                                            const int* parent, Aabb* nodeBoxes, uint32_t* arrival) {
-    int i = blockIdx.x * kB + threadIdx.x;
-    if (i >= n) return;
-    int node = n - 1 + i;
-    nodeBoxes[node] = triBoxes[sortedTri[i]];
-    int p = parent[node];
-    while (p >= 0) {
-        __threadfence();
-        if (atomicAdd(&arrival[p], 1u) == 0u) return;
-        __threadfence();
-        int2 c = children[p];
-        // the sibling's box was written by another thread: read it through L2 (coherent loads)
-        const volatile Aabb* va = &nodeBoxes[c.x];
-        const volatile Aabb* vb = &nodeBoxes[c.y];
-        Aabb m;
-        for (int k = 0; k < 3; k++) {
-            m.lo[k] = fminf(va->lo[k], vb->lo[k]);
-            m.hi[k] = fmaxf(va->hi[k], vb->hi[k]);
-        }
-        nodeBoxes[p] = m;
-        p = parent[p];
-    }
-}
-
 __device__ __forceinline__ float halfArea(const Aabb& b) {
     float dx = fmaxf(b.hi[0] - b.lo[0], 0.0f), dy = fmaxf(b.hi[1] - b.lo[1], 0.0f), dz = fmaxf(b.hi[2] - b.lo[2], 0.0f);
     return dx * dy + dy * dz + dz * dx;
-}
-
-struct CollapseItem {
-    int binNode;   // binary node (internal index or n-1+leaf)
-    uint32_t wide;  // wide node slot to fill
-};
-
-__device__ __forceinline__ int rangeCount(int node, int n, const int2* range) {
-    if (node >= n - 1) return 1;
-    int2 r = range[node];
-    return r.y - r.x + 1;
-}
-__device__ __forceinline__ int rangeFirst(int node, int n, const int2* range) {
-    return node >= n - 1 ? node - (n - 1) : range[node].x;
 }
 
 // Quantisation of one axis of a node: the smallest power-of-two step with
@@ -214,41 +138,138 @@ __device__ void quantizePlanes(float origin, float step, float clo, float chi, u
     qhi = (uint32_t)h;
 }
 
+// ---------------------------------------------------------------- PLOC --
+constexpr int kPlocRadius = 16;
+
+__device__ __forceinline__ Aabb merge(const Aabb& a, const Aabb& b) {
+    Aabb m;
+    for (int k = 0; k < 3; k++) {
+        m.lo[k] = fminf(a.lo[k], b.lo[k]);
+        m.hi[k] = fmaxf(a.hi[k], b.hi[k]);
+    }
+    return m;
+}
+
+// Leaf i is sorted triangle i; nodes n.. are created by merges.
+__global__ __launch_bounds__(kB) void kPlocInit(const Aabb* triBoxes, const uint32_t* sortedTri, uint32_t n,
+                                               Aabb* boxes, uint32_t* counts, uint32_t* clusters) {
+    uint32_t i = blockIdx.x * kB + threadIdx.x;
+    if (i >= n) return;
+    boxes[i] = triBoxes[sortedTri[i]];
+    counts[i] = 1u;
+    clusters[i] = i;
+}
+
+// Pairs are ordered by (merged surface area, lower index, higher index), so the
+// globally smallest pair is always a mutual nearest-neighbour pair and every
+// pass merges at least once.
+__device__ __forceinline__ bool pairBefore(float c0, int i, int j0, float c1, int j1) {
+    if (c0 != c1) return c0 < c1;
+    int a0 = min(i, j0), b0 = max(i, j0), a1 = min(i, j1), b1 = max(i, j1);
+    return a0 != a1 ? a0 < a1 : b0 < b1;
+}
+
+__global__ __launch_bounds__(kB) void kPlocNearest(const uint32_t* clusters, uint32_t m, const Aabb* boxes, int* nn) {
+    __shared__ Aabb tile[kB + 2 * kPlocRadius];
+    const int b0 = (int)(blockIdx.x * kB);
+    for (int t = threadIdx.x; t < kB + 2 * kPlocRadius; t += kB) {
+        int g = b0 - kPlocRadius + t;
+        if (g >= 0 && g < (int)m) tile[t] = boxes[clusters[g]];
+    }
+    __syncthreads();
+    const int i = b0 + (int)threadIdx.x;
+    if (i >= (int)m) return;
+    const Aabb a = tile[threadIdx.x + kPlocRadius];
+    float best = INFINITY;
+    int bj = -1;
+    for (int k = -kPlocRadius; k <= kPlocRadius; k++) {
+        int j = i + k;
+        if (k == 0 || j < 0 || j >= (int)m) continue;
+        float c = halfArea(merge(a, tile[threadIdx.x + kPlocRadius + k]));
+        if (bj < 0 || pairBefore(c, i, j, best, bj)) {
+            best = c;
+            bj = j;
+        }
+    }
+    nn[i] = bj;
+}
+
+__global__ __launch_bounds__(kB) void kPlocMerge(uint32_t* clusters, uint32_t m, const int* nn, Aabb* boxes,
+                                                uint32_t* counts, int2* children, uint32_t n, uint32_t* nodeCounter,
+                                                uint8_t* keep) {
+    const int i = (int)(blockIdx.x * kB + threadIdx.x);
+    if (i >= (int)m) return;
+    const int j = nn[i];
+    const bool mutual = j >= 0 && nn[j] == i;
+    if (mutual && i < j) {
+        const uint32_t a = clusters[i], b = clusters[j];
+        const uint32_t id = atomicAdd(nodeCounter, 1u);
+        children[id - n] = make_int2((int)a, (int)b);
+        boxes[id] = merge(boxes[a], boxes[b]);
+        counts[id] = counts[a] + counts[b];
+        clusters[i] = id;
+    }
+    keep[i] = (mutual && i > j) ? 0u : 1u;
+}
+
+// ------------------------------------------------------------- collapse --
+struct CollapseItem {
+    uint32_t node;   // binary node: leaf (< n) or merged node (>= n)
+    uint32_t wide;   // wide node slot to fill
+    uint32_t first;  // first triangle slot of the subtree in leaf order
+    uint32_t pad;
+};
+
+__device__ __forceinline__ void writeTriangle(const float* tris9, uint32_t t, TriRecord* out) {
+    const float* p = tris9 + (size_t)t * 9u;
+    TriRecord r;
+    r.v0 = make_float4(p[0], p[1], p[2], __uint_as_float(t));
+    r.v1 = make_float4(p[3], p[4], p[5], 0.0f);
+    r.v2 = make_float4(p[6], p[7], p[8], 0.0f);
+    *out = r;
+}
+
 __global__ __launch_bounds__(kB) void kCollapse(const CollapseItem* in, const uint32_t* inCount, CollapseItem* out,
-                                               uint32_t* outCount, uint32_t* nodeAlloc, int n, const int2* children,
-                                               const int2* range, const Aabb* nodeBoxes, BvhNode* nodes,
-                                               uint32_t nodeCapacity, uint32_t* overflow) {
+                                               uint32_t* outCount, uint32_t* nodeAlloc, uint32_t n,
+                                               const int2* children, const uint32_t* counts, const Aabb* boxes,
+                                               const uint32_t* sortedTri, const float* tris9, TriRecord* triOut,
+                                               BvhNode* nodes, uint32_t nodeCapacity, uint32_t* overflow) {
     uint32_t w = blockIdx.x * kB + threadIdx.x;
     if (w >= *inCount) return;
-    CollapseItem it = in[w];
-    int slot[kBvhWidth];
+    const CollapseItem it = in[w];
+    uint32_t slot[kBvhWidth], first[kBvhWidth];
     int cnt = 0;
-    if (it.binNode >= n - 1) {
-        slot[cnt++] = it.binNode;
+    if (it.node < n || counts[it.node] <= kLeafMaxTris) {
+        slot[cnt] = it.node;  // only the root of a tiny scene
+        first[cnt++] = it.first;
     } else {
-        int2 c = children[it.binNode];
-        slot[cnt++] = c.x;
-        slot[cnt++] = c.y;
+        int2 c = children[it.node - n];
+        slot[0] = (uint32_t)c.x;
+        first[0] = it.first;
+        slot[1] = (uint32_t)c.y;
+        first[1] = it.first + counts[c.x];
+        cnt = 2;
         while (cnt < kBvhWidth) {
-            // open the largest child that is an internal node with too many triangles for a leaf
+            // open the largest child that has too many triangles for a leaf
             int best = -1;
             float bestArea = -1.0f;
             for (int k = 0; k < cnt; k++) {
-                int nd = slot[k];
-                if (nd >= n - 1 || rangeCount(nd, n, range) <= (int)kLeafMaxTris) continue;
-                float a = halfArea(nodeBoxes[nd]);
+                if (counts[slot[k]] <= kLeafMaxTris) continue;
+                float a = halfArea(boxes[slot[k]]);
                 if (a > bestArea) {
                     bestArea = a;
                     best = k;
                 }
             }
             if (best < 0) break;
-            int2 c2 = children[slot[best]];
-            slot[best] = c2.x;
-            slot[cnt++] = c2.y;
+            int2 c2 = children[slot[best] - n];
+            uint32_t f = first[best];
+            slot[best] = (uint32_t)c2.x;
+            slot[cnt] = (uint32_t)c2.y;
+            first[cnt++] = f + counts[c2.x];
         }
     }
-    const Aabb P = nodeBoxes[it.binNode];
+    const Aabb P = boxes[it.node];
     BvhNode node;
     node.ox = P.lo[0];
     node.oy = P.lo[1];
@@ -262,11 +283,27 @@ __global__ __launch_bounds__(kB) void kCollapse(const CollapseItem* in, const ui
     for (int k = 0; k < 6; k++) node.qlo[k] = node.qhi[k] = 0u;
     uint32_t used = 0;
     for (int k = 0; k < cnt; k++) {
-        int nd = slot[k];
+        const uint32_t nd = slot[k];
+        const uint32_t count = counts[nd];
         uint32_t ref;
-        int count = rangeCount(nd, n, range);
-        if (count <= (int)kLeafMaxTris) {
-            ref = kLeafBit | ((uint32_t)(count - 1) << 27) | (uint32_t)rangeFirst(nd, n, range);
+        if (count <= kLeafMaxTris) {
+            // the subtree's triangles, depth first, into [first, first + count)
+            uint32_t stack[kLeafMaxTris];
+            int sp = 0;
+            uint32_t written = 0;
+            stack[sp++] = nd;
+            while (sp > 0) {
+                uint32_t x = stack[--sp];
+                if (x < n) {
+                    writeTriangle(tris9, sortedTri[x], &triOut[first[k] + written]);
+                    written++;
+                } else {
+                    int2 c = children[x - n];
+                    stack[sp++] = (uint32_t)c.y;
+                    stack[sp++] = (uint32_t)c.x;
+                }
+            }
+            ref = kLeafBit | ((count - 1u) << 27) | first[k];
         } else {
             uint32_t child = atomicAdd(nodeAlloc, 1u);
             if (child >= nodeCapacity) {
@@ -275,9 +312,9 @@ __global__ __launch_bounds__(kB) void kCollapse(const CollapseItem* in, const ui
             }
             ref = child;
             uint32_t o = atomicAdd(outCount, 1u);
-            out[o] = CollapseItem{nd, child};
+            out[o] = CollapseItem{nd, child, first[k], 0u};
         }
-        const Aabb b = nodeBoxes[nd];
+        const Aabb b = boxes[nd];
         for (int a = 0; a < 3; a++) {
             uint32_t ql, qh;
             quantizePlanes(P.lo[a], step[a], b.lo[a], b.hi[a], ql, qh);
@@ -291,21 +328,8 @@ __global__ __launch_bounds__(kB) void kCollapse(const CollapseItem* in, const ui
     nodes[it.wide] = node;
 }
 
-__global__ __launch_bounds__(kB) void kReorderTris(const float* tris9, const uint32_t* sortedTri, uint32_t n,
-                                                  TriRecord* out) {
-    uint32_t i = blockIdx.x * kB + threadIdx.x;
-    if (i >= n) return;
-    uint32_t t = sortedTri[i];
-    const float* p = tris9 + (size_t)t * 9u;
-    TriRecord r;
-    r.v0 = make_float4(p[0], p[1], p[2], __uint_as_float(t));
-    r.v1 = make_float4(p[3], p[4], p[5], 0.0f);
-    r.v2 = make_float4(p[6], p[7], p[8], 0.0f);
-    out[i] = r;
-}
-
-__global__ void kInitCollapse(CollapseItem* items, uint32_t* counts, int n) {
-    items[0] = CollapseItem{0, 0u};  // binary root: internal node 0, or the single leaf when n == 1
+__global__ void kInitCollapse(CollapseItem* items, uint32_t* counts, const uint32_t* root) {
+    items[0] = CollapseItem{*root, 0u, 0u, 0u};
     counts[0] = 1u;  // items in the current frontier
     counts[1] = 0u;  // items in the next frontier
     counts[2] = 1u;  // wide nodes allocated
@@ -336,33 +360,39 @@ extern "C" int mrt_build_bvh(mrt_context* ctx, mrt_bvh_stats* stats) {
         return 0;
     }
     hipEventRecord(ctx->evStage[0], s);
-    DeviceArray<Aabb> triBoxes, nodeBoxes;
-    DeviceArray<uint32_t> cb, vals, valsSorted, arrival, counts;
+    const size_t binNodes = (size_t)2 * n - 1;
+    const size_t wideCap = n;  // every wide node but the root is a distinct binary node with > kLeafMaxTris triangles
+    DeviceArray<Aabb> triBoxes, boxes;
+    DeviceArray<uint32_t> cb, vals, valsSorted, counts, nodeCounts, clusters[2], selected;
     DeviceArray<uint64_t> keys, keysSorted;
-    DeviceArray<int2> children, range;
-    DeviceArray<int> parent;
+    DeviceArray<int2> children;
+    DeviceArray<int> nn;
+    DeviceArray<uint8_t> keep;
     DeviceArray<CollapseItem> front[2];
-    int nn = (int)n;
-    size_t binNodes = (size_t)2 * n - 1;
-    size_t wideCap = n;  // every wide node but the root is a distinct binary node with > kLeafMaxTris triangles
-    bool ok = triBoxes.reserve(n) && nodeBoxes.reserve(binNodes) && cb.reserve(6) && vals.reserve(n) &&
-              valsSorted.reserve(n) && arrival.reserve(n) && counts.reserve(4) && keys.reserve(n) &&
-              keysSorted.reserve(n) && children.reserve(n) && range.reserve(n) && parent.reserve(binNodes) &&
-              front[0].reserve(wideCap) && front[1].reserve(wideCap) && ctx->nodes.reserve(wideCap) &&
-              ctx->tris.reserve(n);
+    bool ok = triBoxes.reserve(n) && boxes.reserve(binNodes) && cb.reserve(6) && vals.reserve(n) &&
+              valsSorted.reserve(n) && counts.reserve(4) && nodeCounts.reserve(binNodes) && clusters[0].reserve(n) &&
+              clusters[1].reserve(n) && selected.reserve(2) && keys.reserve(n) && keysSorted.reserve(n) &&
+              children.reserve(n) && nn.reserve(n) && keep.reserve(n) && front[0].reserve(wideCap) &&
+              front[1].reserve(wideCap) && ctx->nodes.reserve(wideCap) && ctx->tris.reserve(n);
+    void* temp = nullptr;
     auto cleanup = [&]() {
+        if (temp) hipFree(temp);
+        temp = nullptr;
         triBoxes.release();
-        nodeBoxes.release();
+        boxes.release();
         cb.release();
         vals.release();
         valsSorted.release();
-        arrival.release();
         counts.release();
+        nodeCounts.release();
+        clusters[0].release();
+        clusters[1].release();
+        selected.release();
         keys.release();
         keysSorted.release();
         children.release();
-        range.release();
-        parent.release();
+        nn.release();
+        keep.release();
         front[0].release();
         front[1].release();
     };
@@ -370,54 +400,76 @@ extern "C" int mrt_build_bvh(mrt_context* ctx, mrt_bvh_stats* stats) {
         cleanup();
         return -3;
     }
+    // 1-3: bounds, Morton codes, sort
     uint32_t initCb[6] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u};
     hipMemcpyAsync(cb.ptr, initCb, sizeof(initCb), hipMemcpyHostToDevice, s);
     hipLaunchKernelGGL(kTriBounds, dim3(blocks(n)), dim3(kB), 0, s, ctx->worldTris.ptr, n, triBoxes.ptr, cb.ptr);
     hipLaunchKernelGGL(kMorton, dim3(blocks(n)), dim3(kB), 0, s, triBoxes.ptr, n, cb.ptr, keys.ptr, vals.ptr);
-    size_t tempBytes = 0;
-    rocprim::radix_sort_pairs(nullptr, tempBytes, keys.ptr, keysSorted.ptr, vals.ptr, valsSorted.ptr, n, 0, 63, s);
-    void* temp = nullptr;
-    if (!check(hipMalloc(&temp, tempBytes ? tempBytes : 1), "hipMalloc sort")) {
+    size_t sortBytes = 0, selectBytes = 0;
+    rocprim::radix_sort_pairs(nullptr, sortBytes, keys.ptr, keysSorted.ptr, vals.ptr, valsSorted.ptr, n, 0, 63, s);
+    rocprim::select(nullptr, selectBytes, clusters[0].ptr, keep.ptr, clusters[1].ptr, selected.ptr, (size_t)n, s);
+    if (!check(hipMalloc(&temp, std::max<size_t>(std::max(sortBytes, selectBytes), 1)), "hipMalloc build scratch")) {
         cleanup();
         return -3;
     }
-    rocprim::radix_sort_pairs(temp, tempBytes, keys.ptr, keysSorted.ptr, vals.ptr, valsSorted.ptr, n, 0, 63, s);
-    hipLaunchKernelGGL(kFillU32, dim3(blocks(binNodes)), dim3(kB), 0, s, reinterpret_cast<uint32_t*>(parent.ptr),
-                       binNodes, 0xffffffffu);
-    if (n > 1)
-        hipLaunchKernelGGL(kRadixTree, dim3(blocks(n - 1)), dim3(kB), 0, s, keysSorted.ptr, nn, children.ptr,
-                           parent.ptr, range.ptr);
-    hipLaunchKernelGGL(kFillU32, dim3(blocks(n)), dim3(kB), 0, s, arrival.ptr, (size_t)n, 0u);
-    hipLaunchKernelGGL(kRefit, dim3(blocks(n)), dim3(kB), 0, s, triBoxes.ptr, valsSorted.ptr, nn, children.ptr,
-                       parent.ptr, nodeBoxes.ptr, arrival.ptr);
-    hipLaunchKernelGGL(kReorderTris, dim3(blocks(n)), dim3(kB), 0, s, ctx->worldTris.ptr, valsSorted.ptr, n,
-                       ctx->tris.ptr);
-    hipLaunchKernelGGL(kInitCollapse, dim3(1), dim3(1), 0, s, front[0].ptr, counts.ptr, nn);
-    uint32_t depth = 0;
+    rocprim::radix_sort_pairs(temp, sortBytes, keys.ptr, keysSorted.ptr, vals.ptr, valsSorted.ptr, n, 0, 63, s);
+
+    // 4: PLOC
+    hipLaunchKernelGGL(kPlocInit, dim3(blocks(n)), dim3(kB), 0, s, triBoxes.ptr, valsSorted.ptr, n, boxes.ptr,
+                       nodeCounts.ptr, clusters[0].ptr);
+    uint32_t nodeCounterInit = n;
+    hipMemcpyAsync(counts.ptr + 3, &nodeCounterInit, sizeof(uint32_t), hipMemcpyHostToDevice, s);
+    uint32_t m = n;
     int cur = 0;
+    uint32_t passes = 0;
+    while (m > 1) {
+        hipLaunchKernelGGL(kPlocNearest, dim3(blocks(m)), dim3(kB), 0, s, clusters[cur].ptr, m, boxes.ptr, nn.ptr);
+        hipLaunchKernelGGL(kPlocMerge, dim3(blocks(m)), dim3(kB), 0, s, clusters[cur].ptr, m, nn.ptr, boxes.ptr,
+                           nodeCounts.ptr, children.ptr, n, counts.ptr + 3, keep.ptr);
+        size_t bytes = selectBytes;
+        rocprim::select(temp, bytes, clusters[cur].ptr, keep.ptr, clusters[1 - cur].ptr, selected.ptr, (size_t)m, s);
+        uint32_t next = 0;
+        if (!check(hipMemcpyAsync(&next, selected.ptr, sizeof(uint32_t), hipMemcpyDeviceToHost, s), "ploc count") ||
+            !check(hipStreamSynchronize(s), "ploc sync")) {
+            cleanup();
+            return -4;
+        }
+        cur = 1 - cur;
+        passes++;
+        if (next >= m || passes > 100000u) {  // every pass merges at least one pair
+            cleanup();
+            setError("BVH clustering made no progress");
+            return -5;
+        }
+        m = next;
+    }
+
+    // 5: collapse into 8-wide nodes, writing triangle records in leaf order
+    hipLaunchKernelGGL(kInitCollapse, dim3(1), dim3(1), 0, s, front[0].ptr, counts.ptr, clusters[cur].ptr);
+    uint32_t depth = 0;
+    int fc = 0;
     uint32_t hostCounts[4] = {1, 0, 1, 0};
     while (true) {
         uint32_t frontier = hostCounts[0];
         if (frontier == 0) break;
         depth++;
         hipMemsetAsync(counts.ptr + 1, 0, sizeof(uint32_t), s);
-        hipLaunchKernelGGL(kCollapse, dim3(blocks(frontier)), dim3(kB), 0, s, front[cur].ptr, counts.ptr,
-                           front[1 - cur].ptr, counts.ptr + 1, counts.ptr + 2, nn, children.ptr, range.ptr,
-                           nodeBoxes.ptr, ctx->nodes.ptr, (uint32_t)wideCap, counts.ptr + 3);
+        hipLaunchKernelGGL(kCollapse, dim3(blocks(frontier)), dim3(kB), 0, s, front[fc].ptr, counts.ptr,
+                           front[1 - fc].ptr, counts.ptr + 1, counts.ptr + 2, n, children.ptr, nodeCounts.ptr,
+                           boxes.ptr, valsSorted.ptr, ctx->worldTris.ptr, ctx->tris.ptr, ctx->nodes.ptr,
+                           (uint32_t)wideCap, counts.ptr + 3);
         // the next frontier becomes the current one
         hipMemcpyAsync(counts.ptr, counts.ptr + 1, sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
         if (!check(hipMemcpyAsync(hostCounts, counts.ptr, sizeof(hostCounts), hipMemcpyDeviceToHost, s), "collapse") ||
             !check(hipStreamSynchronize(s), "collapse sync")) {
-            hipFree(temp);
             cleanup();
             return -4;
         }
-        cur = 1 - cur;
+        fc = 1 - fc;
         if (depth > 4096) break;
     }
     hipEventRecord(ctx->evStage[1], s);
     hipStreamSynchronize(s);
-    hipFree(temp);
     if (hostCounts[3] != 0) {
         cleanup();
         setError("BVH collapse exceeded node capacity");
