@@ -36,9 +36,10 @@ constexpr uint32_t kFPrevEnvNee = 32u;  // the previous vertex sampled the envir
 constexpr uint32_t kVisOccluded = 0u, kVisVisible = 1u, kVisUnsupported = 2u;
 
 // Paths in flight per chunk. Large enough that the queues of late bounces
-// (after most paths have ended) still fill the chip: 2^26 paths keep about
-// 19 GB of path state and queues resident, well inside 288 GB of HBM.
-constexpr uint64_t kMaxChunk = 1ull << 26;
+// (after most paths have ended) still fill the chip: 2^27 paths keep about
+// 40 GB of path state and queues resident, well inside 288 GB of HBM (a
+// 1080p frame of 64 samples is one chunk).
+constexpr uint64_t kMaxChunk = 1ull << 27;
 constexpr int kBlock = 256;
 
 // counters[] layout
