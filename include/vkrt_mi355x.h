@@ -58,6 +58,7 @@ typedef struct VKRT_TraceStats {
     uint32_t shadowLaunches;
     uint64_t nodeVisits;    /* closest-hit traversal only, when statistics are enabled */
     uint64_t triangleTests; /* closest-hit traversal only, when statistics are enabled */
+    float backwardMs;       /* device time of the last VKRT_renderBackward (primal + replay) */
 } VKRT_TraceStats;
 VKRT_Result VKRT_getTraceStats(const VKRT* vkrt, VKRT_TraceStats* outStats);
 
@@ -69,6 +70,29 @@ VKRT_Result VKRT_deviceSynchronize(VKRT* vkrt);
 /* Counts BVH nodes visited and triangles tested by every closest-hit query of
  * the following frames (an instrumented traversal; slower, for measurement). */
 VKRT_Result VKRT_setTraversalStatistics(VKRT* vkrt, uint8_t enabled);
+
+/* Gradients returned by VKRT_renderBackward. Arrays are caller-owned; a NULL
+ * array is not computed. materialCount must equal VKRT_getMaterialCount. */
+typedef struct VKRT_Gradients {
+    float* baseColor;          /* materialCount x 3: d loss / d Material.baseColor */
+    float* roughness;          /* materialCount: d loss / d Material.roughness */
+    float* emission;           /* materialCount x 3: d loss / d (Material.emissionColor * emissionLuminance) */
+    uint32_t materialCount;
+    float* environment;        /* environmentWidth x environmentHeight x 3: d loss / d environment texel RGB */
+    uint32_t environmentWidth; /* must match the environment texture, else environment is skipped */
+    uint32_t environmentHeight;
+} VKRT_Gradients;
+
+/* Path-replay backpropagation (render_backward): gradients of
+ * sum_px dLossdImage[px] . I[px] with respect to material base colour,
+ * roughness and emission and to environment texels, where I is the frame of
+ * samplesPerPixel samples that the next VKRT_draw would add with the current
+ * settings (same random streams, so finite differences of VKRT_draw images
+ * share random numbers with it). dLossdImage holds renderWidth x renderHeight
+ * x 3 floats. RGB render mode only. Paths are replayed, not stored; visibility
+ * edges are not differentiated. Does not advance the frame. No counterpart in
+ * the reference (src/core/api/vkrt.h has no differentiable rendering). */
+VKRT_Result VKRT_renderBackward(VKRT* vkrt, const float* dLossdImage, VKRT_Gradients* outGradients);
 
 /* Host copies of the packed scene buffers exactly as the kernels consume them
  * (the reference's descriptor bindings 10-24, src/shaders/scene/resources.slang:7-59).

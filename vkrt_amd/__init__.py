@@ -276,6 +276,24 @@ class TraceStats(C.Structure):
         ("shadowLaunches", C.c_uint32),
         ("nodeVisits", C.c_uint64),
         ("triangleTests", C.c_uint64),
+        ("backwardMs", C.c_float),
+    ]
+
+
+class MaterialSnapshot(C.Structure):
+    # include/vkrt_types.h: 16-byte aligned like Material's vector members, 544 bytes
+    _fields_ = [("material", Material), ("useCount", C.c_uint32), ("name", C.c_char * 256), ("_pad", C.c_uint8 * 12)]
+
+
+class Gradients(C.Structure):
+    _fields_ = [
+        ("baseColor", C.POINTER(C.c_float)),
+        ("roughness", C.POINTER(C.c_float)),
+        ("emission", C.POINTER(C.c_float)),
+        ("materialCount", C.c_uint32),
+        ("environment", C.POINTER(C.c_float)),
+        ("environmentWidth", C.c_uint32),
+        ("environmentHeight", C.c_uint32),
     ]
 
 
@@ -321,6 +339,7 @@ _EXPECTED_SIZES = {
     EmissiveMesh: 32,
     EmissiveTriangle: 48,
     SceneData: 240,
+    MaterialSnapshot: 544,
 }
 for _cls, _size in _EXPECTED_SIZES.items():
     assert C.sizeof(_cls) == _size, f"{_cls.__name__} is {C.sizeof(_cls)} bytes, expected {_size}"
@@ -402,6 +421,8 @@ def library() -> C.CDLL:
         "VKRT_tracePrimaryHits": (C.c_int, [P, u32, up, up]),
         "VKRT_setTraversalStatistics": (C.c_int, [P, C.c_uint8]),
         "VKRT_deviceSynchronize": (C.c_int, [P]),
+        "VKRT_renderBackward": (C.c_int, [P, fp, C.POINTER(Gradients)]),
+        "VKRT_getMaterialSnapshot": (C.c_int, [P, u32, C.POINTER(MaterialSnapshot)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -585,6 +606,39 @@ class Renderer:
         out = np.zeros((self.height, self.width, 4), dtype=np.uint16)
         _ok(self.lib.VKRT_readOutputImage(self.h, out.ctypes.data_as(C.POINTER(C.c_uint16)), out.size),
             "VKRT_readOutputImage")
+        return out
+
+    def material(self, index: int) -> Material:
+        snap = MaterialSnapshot()
+        _ok(self.lib.VKRT_getMaterialSnapshot(self.h, index, C.byref(snap)), "VKRT_getMaterialSnapshot")
+        return snap.material
+
+    def set_material(self, index: int, material: Material):
+        _ok(self.lib.VKRT_setMaterial(self.h, index, C.byref(material)), "VKRT_setMaterial")
+
+    def material_count(self) -> int:
+        c = C.c_uint32()
+        _ok(self.lib.VKRT_getMaterialCount(self.h, C.byref(c)), "VKRT_getMaterialCount")
+        return c.value
+
+    def render_backward(self, d_image: np.ndarray, environment_shape=None) -> dict:
+        """Gradients of sum(d_image * I) for the frame the next draw() renders
+        (VKRT_renderBackward). d_image: H x W x 3. Returns numpy arrays."""
+        d = np.ascontiguousarray(d_image, dtype=np.float32)
+        m = self.material_count()
+        out = {"baseColor": np.zeros((m, 3), np.float32), "roughness": np.zeros(m, np.float32),
+               "emission": np.zeros((m, 3), np.float32)}
+        g = Gradients()
+        g.baseColor = fptr(out["baseColor"])
+        g.roughness = fptr(out["roughness"])
+        g.emission = fptr(out["emission"])
+        g.materialCount = m
+        if environment_shape is not None:
+            h, w = environment_shape
+            out["environment"] = np.zeros((h, w, 3), np.float32)
+            g.environment = fptr(out["environment"])
+            g.environmentWidth, g.environmentHeight = w, h
+        _ok(self.lib.VKRT_renderBackward(self.h, fptr(d), C.byref(g)), "VKRT_renderBackward")
         return out
 
     def synchronize(self):

@@ -82,6 +82,7 @@ extern "C" void mrt_destroy(mrt_context* ctx) {
     ctx->meshAliasIdx.release();
     ctx->triAliasQ.release();
     ctx->triAliasIdx.release();
+    ctx->emissiveMeshMaterial.release();
     ctx->env.release();
     ctx->envAliasQ.release();
     ctx->envAliasIdx.release();
@@ -90,6 +91,8 @@ extern "C" void mrt_destroy(mrt_context* ctx) {
     ctx->accum.release();
     ctx->output.release();
     ctx->frameSum.release();
+    ctx->adjImage.release();
+    ctx->adjGrad.release();
     PathBuffers& b = ctx->paths;
     b.rng.release();
     b.flags.release();
@@ -177,6 +180,7 @@ extern "C" int mrt_upload_lights(mrt_context* ctx, const mrt_light_desc* l) {
               ctx->emissiveTriangles.upload(l->emissiveTriangles, t, s) && ctx->meshAliasQ.upload(l->meshAliasQ, m, s) &&
               ctx->meshAliasIdx.upload(l->meshAliasIdx, m, s) && ctx->triAliasQ.upload(l->triAliasQ, t, s) &&
               ctx->triAliasIdx.upload(l->triAliasIdx, t, s);
+    if (ok && l->emissiveMeshMaterial) ok = ctx->emissiveMeshMaterial.upload(l->emissiveMeshMaterial, m, s);
     if (!ok) return -3;
     return sync(ctx, "upload lights");
 }

@@ -80,6 +80,7 @@ struct mrt_context {
     mrt::DeviceArray<EmissiveTriangle> emissiveTriangles;
     mrt::DeviceArray<float> meshAliasQ, triAliasQ;
     mrt::DeviceArray<uint32_t> meshAliasIdx, triAliasIdx;
+    mrt::DeviceArray<uint32_t> emissiveMeshMaterial;
     mrt::DeviceArray<float4> env;
     uint32_t envWidth = 0, envHeight = 0;
     mrt::DeviceArray<float> envAliasQ, envPdf;
@@ -93,6 +94,10 @@ struct mrt_context {
     mrt::DeviceArray<ushort4> output;
     mrt::DeviceArray<float4> frameSum;
     uint32_t width = 0, height = 0;
+
+    // adjoint (render_backward): loss gradient image and parameter gradients
+    mrt::DeviceArray<float> adjImage;
+    mrt::DeviceArray<float> adjGrad;
 
     // wavefront + traversal scratch
     mrt::PathBuffers paths;

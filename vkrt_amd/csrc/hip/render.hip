@@ -141,6 +141,7 @@ struct LightSurfaceSample {
     F3 wi;
     F3 emission;
     float pdfSolidAngle, shadowDistance;
+    uint32_t light;  // emissive mesh index
     bool valid;
 };
 
@@ -152,6 +153,7 @@ DEV LightSurfaceSample sampleLightSurface(const SceneView& sv, const SceneData& 
     if (meshCount == 0u) return s;
     uint32_t mi = sampleAlias(rnd(rng), meshCount, 0u, sv.meshAliasQ, sv.meshAliasIdx);
     const EmissiveMesh em = sv.emissiveMeshes[mi];
+    s.light = mi;
     if (em.triCount == 0u) return s;
     uint32_t lt = sampleAlias(rnd(rng), em.triCount, em.triOffset, sv.triAliasQ, sv.triAliasIdx);
     const EmissiveTriangle tri = sv.emissiveTriangles[em.triOffset + lt];
@@ -1011,6 +1013,364 @@ __global__ __launch_bounds__(kTraceBlock) void kPrimary(SceneView sv, SceneData 
     }
 }
 
+// ------------------------------------------------------------- adjoint --
+// Path-replay backpropagation (Vicini, Speierer and Jakob, SIGGRAPH 2021) of
+// the RGB integrator. A primal pass stores each path's radiance L; the replay
+// regenerates the same path from the same random streams (nothing per vertex
+// is stored) and walks L_rem, the radiance still to come, down the path:
+//  * every contribution C = throughput * (NEE + emission) the forward pass
+//    adds is recomputed and subtracted from L_rem; its own parameter
+//    derivatives (light or surface emission, environment texels, the BSDF
+//    value at the NEE direction) are accumulated directly;
+//  * at a BSDF-sampled continuation the rest of the path scales with the
+//    BSDF value f, so d/dtheta adds dI * L_rem * (df/dtheta) / f (sampling and
+//    MIS weights are treated as constants, the usual detached estimator).
+// Derivatives of f: base colour exactly (the RGB principled BSDF is affine in
+// it per channel, so df/db = f(b=1) - f(b=0), times the vertex colour);
+// roughness by a central difference of step kRoughnessStep. Shadow rays of the
+// replay are traced inline. Gradients are reduced per block in LDS when the
+// material table fits, then added to global memory with atomics.
+constexpr float kRoughnessStep = 1e-3f;
+
+struct AdjointView {
+    const float* dImage;  // accumulation extent, 3 floats per pixel
+    float* gBase;         // materialCount * 3
+    float* gRough;        // materialCount
+    float* gEmission;     // materialCount * 3
+    float* gEnv;          // envWidth * envHeight * 3, or null
+    const uint32_t* emissiveMeshMaterial;
+    uint32_t materialCount;
+    uint32_t ldsMaterials;  // material gradients reduced in LDS first
+    float invSpp;
+};
+
+// slots of one material in the LDS / global gradient layout
+struct GradSink {
+    float* lds;  // 7 floats per material: base rgb, roughness, emission rgb; null -> global
+    const AdjointView* av;
+    DEV void add(uint32_t mat, int slot, float v) {
+        if (v == 0.0f || mat >= av->materialCount) return;
+        if (lds) {
+            atomicAdd(&lds[mat * 7u + slot], v);
+        } else if (slot < 3) {
+            atomicAdd(&av->gBase[mat * 3u + slot], v);
+        } else if (slot == 3) {
+            atomicAdd(&av->gRough[mat], v);
+        } else {
+            atomicAdd(&av->gEmission[mat * 3u + (slot - 4)], v);
+        }
+    }
+};
+
+// Texel weights of environmentRadiance's bilinear lookup times the strength:
+// adds g (the loss gradient of the returned radiance) to the texels.
+DEV void environmentGradient(const SceneView& sv, const SceneData& sd, F3 dirIn, F3 g, float* gEnv) {
+    if (!gEnv || sd.environmentTextureIndex == VKRT_INVALID_INDEX || !sv.env) return;
+    F3 d = normalize(dirIn);
+    float phi = atan2f(d.y, d.x) + sd.environmentRotation * (kPi / 180.0f);
+    float theta = acosf(clampf(d.z, -1.0f, 1.0f));
+    float u = fracf(phi * (0.5f * kInvPi) + 0.5f);
+    float v = theta * kInvPi;
+    int W = (int)sv.envWidth, H = (int)sv.envHeight;
+    float fx = u * (float)W - 0.5f, fy = v * (float)H - 0.5f;
+    float x0f = floorf(fx), y0f = floorf(fy);
+    float tx = fx - x0f, ty = fy - y0f;
+    int x0 = (int)x0f, y0 = (int)y0f;
+    int x1 = x0 + 1, y1 = y0 + 1;
+    x0 = ((x0 % W) + W) % W;
+    x1 = ((x1 % W) + W) % W;
+    y0 = min(max(y0, 0), H - 1);
+    y1 = min(max(y1, 0), H - 1);
+    const float s = sd.environmentLight[3];
+    const int xs[4] = {x0, x1, x0, x1}, ys[4] = {y0, y0, y1, y1};
+    const float ws[4] = {(1.0f - tx) * (1.0f - ty), tx * (1.0f - ty), (1.0f - tx) * ty, tx * ty};
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        float w = ws[k] * s;
+        if (w == 0.0f) continue;
+        float* t = gEnv + ((size_t)ys[k] * W + xs[k]) * 3u;
+        if (g.x != 0.0f) atomicAdd(&t[0], g.x * w);
+        if (g.y != 0.0f) atomicAdd(&t[1], g.y * w);
+        if (g.z != 0.0f) atomicAdd(&t[2], g.z * w);
+    }
+}
+
+// BSDF value derivatives at wl: base colour (per channel) and roughness.
+struct BsdfDerivatives {
+    F3 dBase, dRough;
+};
+DEV BsdfDerivatives bsdfDerivatives(const Spec& sp, const BMat& mat, F3 vertexColor, F3 wo, uint32_t ff,
+                                    float roughnessParam, F3 wl) {
+    BsdfDerivatives d;
+    BMat m0 = mat, m1 = mat;
+    m0.baseColor = f3(0.0f);
+    m1.baseColor = f3(1.0f);
+    F3 f0 = evalScalar(sp, makeState(m0, wo, ff, 0.0f, 0u), wl).value;
+    F3 f1 = evalScalar(sp, makeState(m1, wo, ff, 0.0f, 0u), wl).value;
+    d.dBase = (f1 - f0) * vertexColor;
+    float rp = sat(roughnessParam + kRoughnessStep), rm = sat(roughnessParam - kRoughnessStep);
+    d.dRough = f3(0.0f);
+    if (rp > rm) {
+        BMat mp = mat, mm = mat;
+        mp.roughness = rp;
+        mm.roughness = rm;
+        F3 fp = evalScalar(sp, makeState(mp, wo, ff, 0.0f, 0u), wl).value;
+        F3 fm = evalScalar(sp, makeState(mm, wo, ff, 0.0f, 0u), wl).value;
+        d.dRough = (fp - fm) / ((roughnessParam + kRoughnessStep) - (roughnessParam - kRoughnessStep));
+    }
+    return d;
+}
+
+// L_rem of every path starts at the path's primal radiance.
+__global__ __launch_bounds__(kBlock) void kAdjointInit(uint32_t count, PathView pv, const float4* sampleRad) {
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= count) return;
+    pv.rad[i] = sampleRad[i];
+}
+
+// One replayed path vertex for every ray of the current queue (RGB mode). The
+// forward estimator's operations, random draws and continuation decisions are
+// those of kShade<kModeRgb> followed by the shadow passes.
+__global__ __launch_bounds__(kTraceBlock) void kShadeAdjoint(FrameParams fp, SceneView sv, PathView pv, QueueView in,
+                                                            const float4* hits, QueueView out, uint32_t* counters,
+                                                            int inSlot, int outSlot, uint32_t* gstack,
+                                                            AdjointView av) {
+    __shared__ uint32_t lds[kLdsStackWords];
+    extern __shared__ float gradLds[];
+    GradSink sink{av.ldsMaterials ? gradLds : nullptr, &av};
+    if (av.ldsMaterials) {
+        for (uint32_t k = threadIdx.x; k < av.materialCount * 7u; k += kTraceBlock) gradLds[k] = 0.0f;
+        __syncthreads();
+    }
+    TraversalStack st;
+    st.lds = lds + threadIdx.x;
+    st.glob = gstack + (size_t)(blockIdx.x * kTraceBlock + threadIdx.x) * kGlobalStackWords;
+    st.dropped = 0;
+    st.sp = 0;
+    const uint32_t n = counters[inSlot];
+    const uint32_t lane = threadIdx.x & 63u;
+    const SceneData& sd = fp.sd;
+    for (uint32_t base = blockIdx.x * kTraceBlock + (threadIdx.x & ~63u); base < n; base += gridDim.x * kTraceBlock) {
+        const uint32_t i = base + lane;
+        bool cont = false;
+        F3 nextO = f3(0.0f), nextD = f3(0.0f);
+        uint32_t q = 0;
+        if (i < n) do {
+            float4 ro = in.o[i], rdir = in.d[i];
+            q = __float_as_uint(ro.w);
+            F3 o = xyz(ro), d = xyz(rdir);
+            float4 hh = hits[i];
+            Hit h{hh.x, __float_as_uint(hh.y), hh.z, hh.w};
+            uint32_t flags = pv.flags[q];
+            const bool absorbing = (flags & kFAbsorption) != 0u;
+            const bool refractive = (flags & kFRefractive) != 0u;
+            F4 thr = ld4(pv.thr[q]);
+            F4 misc = ld4(pv.misc[q]);
+            F4 lr = ld4(pv.rad[q]);
+            F3 lrem = F3{lr.x, lr.y, lr.z};
+            // dLoss/d(sample radiance) of this path's pixel
+            const uint64_t pg = fp.pathBegin + q;
+            const uint32_t k = (uint32_t)(pg % fp.numPixels);
+            const uint32_t px = sd.viewportRect[0] + k % fp.vpW, py = sd.viewportRect[1] + k / fp.vpW;
+            const float* di = av.dImage + ((size_t)py * fp.accumWidth + px) * 3u;
+            const F3 dI = F3{di[0], di[1], di[2]} * av.invSpp;
+            const F3 t3 = F3{thr.x, thr.y, thr.z};
+
+            if (h.tri == kNoHit) {
+                if (!fp.neeOnly && !(refractive || absorbing)) {
+                    F3 env = environmentRadiance(sv, sd, d);
+                    const bool prevEnv = (flags & kFPrevEnvNee) != 0u;
+                    float envPdf = prevEnv ? environmentPdf(sv, sd, d) : 0.0f;
+                    float w = 1.0f;
+                    if (prevEnv && envPdf > 0.0f && misc.x > 0.0f) w = powerHeuristic(misc.x, envPdf);
+                    environmentGradient(sv, sd, d, dI * t3 * w, av.gEnv);
+                }
+                break;
+            }
+            F4 sigma = ld4(pv.sigma[q]);
+            if (absorbing) {
+                F3 t = transmittance3(sigma, true, h.t);
+                thr = F4{thr.x * t.x, thr.y * t.y, thr.z * t.z, thr.w};
+            }
+            if (!anyPos(F3{thr.x, thr.y, thr.z})) break;
+            const F3 thr3 = F3{thr.x, thr.y, thr.z};
+
+            Surface sf = reconstruct(sv, o, d, h);
+            const uint32_t mi = sf.materialIndex;
+            BMat mat = loadMaterial(sv.materials, mi, sf.vertexColor);
+            const float roughParam = sv.materials[mi].roughness;
+            F3 wo = toLocal(-d, sf.basis);
+            BState bst = makeState(mat, wo, sf.frontFace, 0.0f, 0u);
+            const bool curNee = !refractive;
+            Rng rng = loadRng(pv.rng[q], fp.rngPolicy);
+            bool clearPrevNee = false;
+
+            // light NEE, visibility traced inline
+            if (!fp.bsdfOnly && fp.neeEnabled && curNee && wo.z > 0.0f) {
+                LightSurfaceSample ls = sampleLightSurface(sv, sd, sf.p, rng);
+                if (ls.valid) {
+                    F3 off = dot(ls.wi, sf.ng) >= 0.0f ? sf.ng : -sf.ng;
+                    F3 wl = toLocal(ls.wi, sf.basis);
+                    F3 nee = f3(0.0f);
+                    float mis = 0.0f;
+                    F3 tr = f3(1.0f);
+                    Eval e{};
+                    bool evaluated = false;
+                    if (!(isRefractive(mat) && wl.z <= 0.0f)) {
+                        e = evalScalar(fp.spec, bst, wl);
+                        if (e.pdf > 0.0f) {
+                            evaluated = true;
+                            mis = powerHeuristic(ls.pdfSolidAngle, e.pdf);
+                            tr = transmittance3(sigma, absorbing, ls.shadowDistance);
+                            F3 fc = e.value * fabsf(wl.z);
+                            nee = tr * mis * fc * ls.emission / ls.pdfSolidAngle;
+                        }
+                    }
+                    if (fp.traceShadowAlways || anyPos(nee)) {
+                        TraversalRay sr = makeTraversalRay(sf.p + off * kShadowOriginOffset, ls.wi);
+                        uint32_t vis = classifyOccluder(sv, traceAny(sv, sr, kRayTMin, ls.shadowDistance, st));
+                        if (vis == kVisUnsupported) clearPrevNee = true;
+                        if (vis == kVisVisible && evaluated) {
+                            lrem = lrem - thr3 * nee;
+                            // d/d(light emission): thr * tr * mis * f cos / pdf
+                            F3 ge = dI * thr3 * tr * (e.value * fabsf(wl.z)) * (mis / ls.pdfSolidAngle);
+                            const uint32_t lm = av.emissiveMeshMaterial ? av.emissiveMeshMaterial[ls.light] : 0xFFFFFFFFu;
+                            sink.add(lm, 4, ge.x);
+                            sink.add(lm, 5, ge.y);
+                            sink.add(lm, 6, ge.z);
+                            // d/d(surface parameters) through f at the light direction
+                            BsdfDerivatives bd = bsdfDerivatives(fp.spec, mat, sf.vertexColor, wo, sf.frontFace,
+                                                                 roughParam, wl);
+                            F3 g = dI * thr3 * tr * ls.emission * (mis * fabsf(wl.z) / ls.pdfSolidAngle);
+                            sink.add(mi, 0, g.x * bd.dBase.x);
+                            sink.add(mi, 1, g.y * bd.dBase.y);
+                            sink.add(mi, 2, g.z * bd.dBase.z);
+                            sink.add(mi, 3, g.x * bd.dRough.x + g.y * bd.dRough.y + g.z * bd.dRough.z);
+                        }
+                    }
+                }
+            }
+
+            // environment NEE
+            bool envTried = false;
+            if (!fp.bsdfOnly && fp.envNee && curNee && !absorbing && wo.z > 0.0f) {
+                envTried = true;
+                EnvSample es = sampleEnvironmentDirection(sv, sd, rng);
+                if (es.valid) {
+                    F3 wl = toLocal(es.wi, sf.basis);
+                    if (!(isRefractive(mat) && wl.z <= 0.0f)) {
+                        F3 le = environmentRadiance(sv, sd, es.wi);
+                        Eval e = evalScalar(fp.spec, bst, wl);
+                        if (e.pdf > 0.0f) {
+                            float mis = powerHeuristic(es.pdf, e.pdf);
+                            F3 ne = (e.value * fabsf(wl.z)) * mis * le / es.pdf;
+                            if (anyPos(ne)) {
+                                F3 off = wl.z > 0.0f ? sf.ng : -sf.ng;
+                                TraversalRay sr = makeTraversalRay(sf.p + off * kShadowOriginOffset, es.wi);
+                                if (traceAny(sv, sr, kRayTMin, kRayTMax, st) == kNoHit) {
+                                    lrem = lrem - thr3 * ne;
+                                    environmentGradient(sv, sd, es.wi,
+                                                        dI * thr3 * (e.value * fabsf(wl.z)) * (mis / es.pdf), av.gEnv);
+                                    BsdfDerivatives bd = bsdfDerivatives(fp.spec, mat, sf.vertexColor, wo,
+                                                                         sf.frontFace, roughParam, wl);
+                                    F3 g = dI * thr3 * le * (mis * fabsf(wl.z) / es.pdf);
+                                    sink.add(mi, 0, g.x * bd.dBase.x);
+                                    sink.add(mi, 1, g.y * bd.dBase.y);
+                                    sink.add(mi, 2, g.z * bd.dBase.z);
+                                    sink.add(mi, 3, g.x * bd.dRough.x + g.y * bd.dRough.y + g.z * bd.dRough.z);
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+
+            // emission with MIS
+            if (!fp.neeOnly && anyPos(mat.emission)) {
+                const bool prevNee = (flags & kFPrevNee) != 0u;
+                float w = 1.0f;
+                if (fp.neeEnabled && prevNee && fp.depth != 0u && misc.x > 0.0f) {
+                    float lp = areaToSolidAngle(sv.meshes[sf.mesh].lightPdfArea, sf.ng, d, h.t);
+                    w = lp <= 0.0f ? 1.0f : powerHeuristic(misc.x, lp);
+                }
+                F3 em = mat.emission * w;
+                lrem = lrem - thr3 * em;
+                F3 ge = dI * thr3 * w;
+                sink.add(mi, 4, ge.x);
+                sink.add(mi, 5, ge.y);
+                sink.add(mi, 6, ge.z);
+            }
+
+            if (fp.depth + 1u >= sd.rrMaxDepth) break;
+            if (wo.z <= 0.0f) break;
+            F3 wl;
+            uint32_t isT;
+            if (!sampleLobe(bst, rng, wl, isT)) break;
+            F3 wi = toWorld(wl, sf.basis);
+            Eval e = evalScalar(fp.spec, bst, wl);
+            if (e.pdf <= 0.0f) break;
+            F3 weight = e.value * fabsf(wl.z) / e.pdf;
+            if (!(e.pdf > 0.0f && anyPos(weight))) break;
+            // the rest of the path (L_rem) scales with f at the sampled direction
+            {
+                BsdfDerivatives bd = bsdfDerivatives(fp.spec, mat, sf.vertexColor, wo, sf.frontFace, roughParam, wl);
+                F3 r = F3{e.value.x > 0.0f ? dI.x * lrem.x / e.value.x : 0.0f,
+                          e.value.y > 0.0f ? dI.y * lrem.y / e.value.y : 0.0f,
+                          e.value.z > 0.0f ? dI.z * lrem.z / e.value.z : 0.0f};
+                sink.add(mi, 0, r.x * bd.dBase.x);
+                sink.add(mi, 1, r.y * bd.dBase.y);
+                sink.add(mi, 2, r.z * bd.dBase.z);
+                sink.add(mi, 3, r.x * bd.dRough.x + r.y * bd.dRough.y + r.z * bd.dRough.z);
+            }
+            F3 t = thr3 * weight;
+            thr = F4{t.x, t.y, t.z, 0.0f};
+            if (!anyPos(t)) break;
+            misc.x = e.pdf;
+            uint32_t newFlags = flags & ~(kFPrevNee | kFPendingCollapse | kFPrevEnvNee);
+            if (curNee && isT == 0u) newFlags |= kFPrevNee;
+            if (clearPrevNee) newFlags &= ~kFPrevNee;
+            if (envTried && isT == 0u) newFlags |= kFPrevEnvNee;
+            updateMedium(fp.spec, mat, sf.frontFace, isT, 0.0f, 0u, newFlags, sigma);
+            if (fp.depth + 1u >= sd.rrMinDepth) {
+                float pr = clampf(maxc(F3{thr.x, thr.y, thr.z}), kRRMin, kRRMax);
+                if (rnd(rng) > pr) break;
+                F3 t2 = F3{thr.x, thr.y, thr.z} / pr;
+                thr = F4{t2.x, t2.y, t2.z, 0.0f};
+            }
+            F3 offc = isT ? -sf.ng : sf.ng;
+            nextO = sf.p + offc * kShadowOriginOffset;
+            nextD = wi;
+            cont = true;
+            pv.rng[q] = storeRng(rng);
+            pv.thr[q] = st4(thr);
+            pv.misc[q] = st4(misc);
+            pv.flags[q] = newFlags;
+            pv.sigma[q] = st4(sigma);
+            pv.rad[q] = make_float4(lrem.x, lrem.y, lrem.z, 0.0f);
+        } while (false);
+        uint32_t slot = waveAppend(cont, &counters[outSlot]);
+        if (cont) {
+            out.o[slot] = make_float4(nextO.x, nextO.y, nextO.z, __uint_as_float(q));
+            out.d[slot] = make_float4(nextD.x, nextD.y, nextD.z, kRayTMax);
+        }
+    }
+    if (st.dropped) atomicAdd(&counters[kCntDropped], st.dropped);
+    if (av.ldsMaterials) {
+        __syncthreads();
+        for (uint32_t k = threadIdx.x; k < av.materialCount * 7u; k += kTraceBlock) {
+            float v = gradLds[k];
+            if (v == 0.0f) continue;
+            uint32_t m = k / 7u, slot = k % 7u;
+            if (slot < 3u)
+                atomicAdd(&av.gBase[m * 3u + slot], v);
+            else if (slot == 3u)
+                atomicAdd(&av.gRough[m], v);
+            else
+                atomicAdd(&av.gEmission[m * 3u + (slot - 4u)], v);
+        }
+    }
+}
+
 // ------------------------------------------------------------- host side --
 static uint32_t blocksFor(uint64_t n) { return (uint32_t)((n + kBlock - 1) / kBlock); }
 
@@ -1074,6 +1434,15 @@ struct StageEvents {
         used += 2;
     }
 };
+
+static void queueViews(mrt_context* ctx, QueueView qv[2]) {
+    PathBuffers& b = ctx->paths;
+    for (int qi = 0; qi < 2; qi++) {
+        qv[qi].o = b.rays[qi][0].ptr;
+        qv[qi].d = b.rays[qi][1].ptr;
+        qv[qi].count = ctx->counters.ptr + qi;
+    }
+}
 
 template <int MODE>
 static bool launchChunk(mrt_context* ctx, FrameParams fp, StageEvents& ev, bool countTraversal) {
@@ -1157,12 +1526,16 @@ extern "C" int mrt_clear_accumulation(mrt_context* ctx) {
     return check(hipGetLastError(), "clear accumulation") ? 0 : -2;
 }
 
-extern "C" int mrt_render_frame(mrt_context* ctx, const SceneData* scene, const mrt_render_params* prm,
-                                mrt_render_stats* stats) {
-    if (!ctx || !scene || !prm) {
-        setError("mrt_render_frame: null argument");
-        return -1;
-    }
+// Validation, buffers and per-frame parameters shared by the forward and the
+// adjoint passes.
+struct FrameSetup {
+    FrameParams fp;
+    uint64_t totalPaths = 0, chunk = 0;
+    int mode = kModeRgb;
+    uint32_t renderMode = 0, vw = 0, vh = 0;
+};
+
+static int setupFrame(mrt_context* ctx, const SceneData* scene, const mrt_render_params* prm, FrameSetup& fs) {
     if (!check(hipSetDevice(ctx->device), "hipSetDevice")) return -2;
     if (mrt_resize_accumulation(ctx, prm->width, prm->height) != 0) return -2;
     if (!ctx->rgb2spec.ptr) {
@@ -1173,7 +1546,6 @@ extern "C" int mrt_render_frame(mrt_context* ctx, const SceneData* scene, const 
         setError("BVH deeper than the traversal stack supports");
         return -2;
     }
-    auto wall0 = std::chrono::steady_clock::now();
     hipStream_t s = ctx->stream;
     uint32_t vx = scene->viewportRect[0], vy = scene->viewportRect[1];
     uint32_t vw = scene->viewportRect[2], vh = scene->viewportRect[3];
@@ -1196,7 +1568,7 @@ extern "C" int mrt_render_frame(mrt_context* ctx, const SceneData* scene, const 
     uint64_t totalPaths = numPixels * spp;
     uint64_t chunk = std::min<uint64_t>(totalPaths, kMaxChunk);
     if (numPixels > 0 && numPixels <= kMaxChunk) chunk = std::min<uint64_t>(totalPaths, (kMaxChunk / numPixels) * numPixels);
-    FrameParams fp;
+    FrameParams& fp = fs.fp;
     memset(&fp, 0, sizeof(fp));
     fp.sd = *scene;
     fp.numPixels = (uint32_t)numPixels;
@@ -1215,6 +1587,30 @@ extern "C" int mrt_render_frame(mrt_context* ctx, const SceneData* scene, const 
     fp.spec.res = scene->rgb2specSRGB.res;
     fp.spec.scaleOffset = scene->rgb2specSRGB.scaleOffset;
     fp.spec.dataOffset = scene->rgb2specSRGB.dataOffset;
+    fs.totalPaths = totalPaths;
+    fs.chunk = chunk;
+    fs.mode = mode;
+    fs.renderMode = renderMode;
+    fs.vw = vw;
+    fs.vh = vh;
+    return 0;
+}
+
+extern "C" int mrt_render_frame(mrt_context* ctx, const SceneData* scene, const mrt_render_params* prm,
+                                mrt_render_stats* stats) {
+    if (!ctx || !scene || !prm) {
+        setError("mrt_render_frame: null argument");
+        return -1;
+    }
+    auto wall0 = std::chrono::steady_clock::now();
+    FrameSetup fs;
+    int rc = setupFrame(ctx, scene, prm, fs);
+    if (rc != 0) return rc;
+    hipStream_t s = ctx->stream;
+    FrameParams& fp = fs.fp;
+    const uint64_t totalPaths = fs.totalPaths, chunk = fs.chunk;
+    const int mode = fs.mode;
+    const uint32_t renderMode = fs.renderMode, vw = fs.vw, vh = fs.vh;
 
     static StageEvents ev;
     if (ev.pool.empty()) {
@@ -1274,6 +1670,113 @@ extern "C" int mrt_render_frame(mrt_context* ctx, const SceneData* scene, const 
         stats->shadeLaunches = stageN[1];
         stats->shadowLaunches = stageN[2];
         (void)wall0;
+    }
+    return 0;
+}
+
+extern "C" int mrt_render_backward(mrt_context* ctx, const SceneData* scene, const mrt_render_params* prm,
+                                   const float* dImage, mrt_gradients* g, mrt_render_stats* stats) {
+    if (!ctx || !scene || !prm || !dImage || !g) {
+        setError("mrt_render_backward: null argument");
+        return -1;
+    }
+    mrt_render_params p2 = *prm;
+    p2.resetAccumulation = 0;
+    FrameSetup fs;
+    int rc = setupFrame(ctx, scene, &p2, fs);
+    if (rc != 0) return rc;
+    if (fs.mode != kModeRgb) {
+        setError("render_backward supports the RGB render mode");
+        return -1;
+    }
+    if (g->materialCount != (uint32_t)ctx->materials.count) {
+        setError("render_backward: gradient material count differs from the scene");
+        return -1;
+    }
+    hipStream_t s = ctx->stream;
+    FrameParams& fp = fs.fp;
+    const uint32_t M = g->materialCount;
+    const bool envGrad = g->environment && ctx->envWidth > 0u && g->environmentWidth == ctx->envWidth &&
+                         g->environmentHeight == ctx->envHeight &&
+                         scene->environmentTextureIndex != VKRT_INVALID_INDEX;
+    const size_t envN = envGrad ? (size_t)ctx->envWidth * ctx->envHeight * 3u : 0u;
+    const size_t pix = (size_t)prm->width * prm->height;
+    if (!ctx->adjImage.upload(dImage, pix * 3u, s) || !ctx->adjGrad.reserve((size_t)M * 7u + envN + 1u)) return -3;
+    hipMemsetAsync(ctx->adjGrad.ptr, 0, ((size_t)M * 7u + envN) * sizeof(float), s);
+    AdjointView av;
+    av.dImage = ctx->adjImage.ptr;
+    av.gBase = ctx->adjGrad.ptr;
+    av.gRough = ctx->adjGrad.ptr + (size_t)M * 3u;
+    av.gEmission = ctx->adjGrad.ptr + (size_t)M * 4u;
+    av.gEnv = envGrad ? ctx->adjGrad.ptr + (size_t)M * 7u : nullptr;
+    av.emissiveMeshMaterial = scene->emissiveMeshCount ? ctx->emissiveMeshMaterial.ptr : nullptr;
+    av.materialCount = M;
+    const size_t gradLdsBytes = (size_t)M * 7u * sizeof(float);
+    av.ldsMaterials = gradLdsBytes <= 16384u ? 1u : 0u;
+    uint32_t spp = scene->samplesPerPixel > 1u ? scene->samplesPerPixel : 1u;
+    av.invSpp = 1.0f / (float)spp;
+
+    PathBuffers& b = ctx->paths;
+    PathView pv = pathView(ctx);
+    SceneView sv = ctx->view();
+    uint32_t* cnt = ctx->counters.ptr;
+    QueueView qv[2];
+    queueViews(ctx, qv);
+    StageEvents untimed;
+    hipEventRecord(ctx->evBegin, s);
+    for (uint64_t begin = 0; begin < fs.totalPaths; begin += fs.chunk) {
+        fp.pathBegin = begin;
+        fp.count = (uint32_t)std::min<uint64_t>(fs.chunk, fs.totalPaths - begin);
+        // primal: the radiance of every path of the chunk (sampleRad)
+        if (!launchChunk<kModeRgb>(ctx, fp, untimed, false)) return -4;
+        // replay with the same streams
+        hipLaunchKernelGGL(kGenerate<kModeRgb>, dim3(blocksFor(fp.count)), dim3(kBlock), 0, s, fp, pv, qv[0],
+                           ctx->accum.ptr);
+        hipLaunchKernelGGL(kSetCount, dim3(1), dim3(1), 0, s, cnt, kCntQueue0, fp.count);
+        hipLaunchKernelGGL(kAdjointInit, dim3(blocksFor(fp.count)), dim3(kBlock), 0, s, fp.count, pv,
+                           b.sampleRad.ptr);
+        int cur = 0;
+        for (uint32_t depth = 0; depth < fp.sd.rrMaxDepth; depth++) {
+            int nxt = 1 - cur;
+            fp.depth = depth;
+            hipLaunchKernelGGL(kPrepDepth, dim3(1), dim3(64), 0, s, cnt, cur, nxt);
+            hipLaunchKernelGGL(kExtend<false>, dim3(ctx->traceGrid), dim3(kTraceBlock), 0, s, sv, qv[cur].o, qv[cur].d,
+                               b.hits.ptr, cnt, cur, ctx->traversalStack.ptr);
+            hipLaunchKernelGGL(kShadeAdjoint, dim3(ctx->traceGrid), dim3(kTraceBlock),
+                               av.ldsMaterials ? (uint32_t)gradLdsBytes : 0u, s, fp, sv, pv, qv[cur], b.hits.ptr,
+                               qv[nxt], cnt, cur, nxt, ctx->traversalStack.ptr, av);
+            cur = nxt;
+        }
+        if (!check(hipGetLastError(), "adjoint launch")) return -4;
+    }
+    hipEventRecord(ctx->evEnd, s);
+    std::vector<float> host((size_t)M * 7u + envN);
+    if (!host.empty() &&
+        !check(hipMemcpyAsync(host.data(), ctx->adjGrad.ptr, host.size() * sizeof(float), hipMemcpyDeviceToHost, s),
+               "gradients"))
+        return -4;
+    uint32_t hostCounters[64];
+    if (!check(hipMemcpyAsync(hostCounters, cnt, sizeof(hostCounters), hipMemcpyDeviceToHost, s), "counters") ||
+        !check(hipStreamSynchronize(s), "render backward"))
+        return -4;
+    if (hostCounters[kCntDropped]) {
+        setError("traversal stack overflow");
+        return -5;
+    }
+    if (g->baseColor) memcpy(g->baseColor, host.data(), (size_t)M * 3u * sizeof(float));
+    if (g->roughness) memcpy(g->roughness, host.data() + (size_t)M * 3u, (size_t)M * sizeof(float));
+    if (g->emission) memcpy(g->emission, host.data() + (size_t)M * 4u, (size_t)M * 3u * sizeof(float));
+    if (envGrad) memcpy(g->environment, host.data() + (size_t)M * 7u, envN * sizeof(float));
+    if (stats) {
+        memset(stats, 0, sizeof(*stats));
+        unsigned long long st64[2];
+        memcpy(st64, hostCounters + kCntStats, sizeof(st64));
+        stats->extensionRays = st64[0];
+        stats->shadowRays = st64[1];
+        stats->paths = fs.totalPaths;
+        float ms = 0.0f;
+        hipEventElapsedTime(&ms, ctx->evBegin, ctx->evEnd);
+        stats->traceMs = ms;
     }
     return 0;
 }

@@ -156,8 +156,9 @@ static VKRT_Result syncScene(VKRT* vkrt, int requireDevice) {
     }
     if (mrt_upload_materials(vkrt->device, p->materials, p->materialCount) != 0) return deviceError();
     if (mrt_upload_meshes(vkrt->device, p->meshInfos, vkrt->meshCount) != 0) return deviceError();
-    mrt_light_desc l = {p->emissiveMeshes,    p->emissiveMeshCount, p->emissiveTriangles, p->emissiveTriangleCount,
-                        p->meshAliasQ,        p->meshAliasIdx,      p->triAliasQ,         p->triAliasIdx};
+    mrt_light_desc l = {p->emissiveMeshes, p->emissiveMeshCount, p->emissiveTriangles, p->emissiveTriangleCount,
+                        p->meshAliasQ,     p->meshAliasIdx,      p->triAliasQ,         p->triAliasIdx,
+                        p->emissiveMeshMaterial};
     if (mrt_upload_lights(vkrt->device, &l) != 0) return deviceError();
     if (envDirty) {
         mrt_environment_desc e;
@@ -318,6 +319,36 @@ VKRT_Result VKRT_getTraceStats(const VKRT* vkrt, VKRT_TraceStats* out) {
     return VKRT_SUCCESS;
 }
 
+static VKRT_Result requireTraceable(VKRT* vkrt) {
+    if (!vkrt) return VKRT_ERROR_INVALID_ARGUMENT;
+    if (!vkrt->device) return VKRT_ERROR_DEVICE_LOST;
+    return syncScene(vkrt, 1);
+}
+
+VKRT_Result VKRT_renderBackward(VKRT* vkrt, const float* dLossdImage, VKRT_Gradients* out) {
+    if (!vkrt || !dLossdImage || !out) return VKRT_ERROR_INVALID_ARGUMENT;
+    VKRT_Result r = requireTraceable(vkrt);
+    if (r != VKRT_SUCCESS) return r;
+    if (vkrt->settings.renderMode != VKRT_RENDER_MODE_RGB) {
+        VKRT_LOG_ERROR("VKRT_renderBackward supports the RGB render mode");
+        return VKRT_ERROR_INVALID_ARGUMENT;
+    }
+    if (out->materialCount != vkrt->packed.materialCount) return VKRT_ERROR_INVALID_ARGUMENT;
+    mrt_render_params rp;
+    fillRenderParams(vkrt, &rp);
+    if (mrt_resize_accumulation(vkrt->device, vkrt->renderWidth, vkrt->renderHeight) != 0) return deviceError();
+    /* the next frame starts from an empty accumulation when a reset is pending:
+     * replay the samples of that frame */
+    if (vkrt->accumulationNeedsReset && mrt_clear_accumulation(vkrt->device) != 0) return deviceError();
+    mrt_gradients g = {out->baseColor,   out->roughness,        out->emission,         out->materialCount,
+                       out->environment, out->environmentWidth, out->environmentHeight};
+    mrt_render_stats rs;
+    memset(&rs, 0, sizeof(rs));
+    if (mrt_render_backward(vkrt->device, &vkrt->sceneData, &rp, dLossdImage, &g, &rs) != 0) return deviceError();
+    vkrt->stats.backwardMs = rs.traceMs;
+    return VKRT_SUCCESS;
+}
+
 VKRT_Result VKRT_deviceSynchronize(VKRT* vkrt) {
     if (!vkrt) return VKRT_ERROR_INVALID_ARGUMENT;
     if (!vkrt->device) return VKRT_ERROR_INITIALIZATION_FAILED;
@@ -389,11 +420,6 @@ VKRT_Result VKRT_exportSceneBuffers(VKRT* vkrt, VKRT_SceneBuffers* out) {
     return VKRT_SUCCESS;
 }
 
-static VKRT_Result requireTraceable(VKRT* vkrt) {
-    if (!vkrt) return VKRT_ERROR_INVALID_ARGUMENT;
-    if (!vkrt->device) return VKRT_ERROR_DEVICE_LOST;
-    return syncScene(vkrt, 1);
-}
 
 VKRT_Result VKRT_traceRays(VKRT* vkrt, const float* rays, size_t count, uint32_t* outMesh, uint32_t* outPrimitive,
                            float* outT, float* outBarycentrics) {

@@ -61,6 +61,7 @@ typedef struct PackedScene {
     uint32_t* meshAliasIdx;
     float* triAliasQ;
     uint32_t* triAliasIdx;
+    uint32_t* emissiveMeshMaterial; /* material of each emissive mesh (for emission gradients) */
     float* worldTriangles;
     uint32_t* triangleMesh;
     uint32_t* meshTriangleBase;

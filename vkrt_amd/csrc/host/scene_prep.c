@@ -158,6 +158,7 @@ void vkrtFreePackedScene(PackedScene* p) {
     free(p->meshAliasIdx);
     free(p->triAliasQ);
     free(p->triAliasIdx);
+    free(p->emissiveMeshMaterial);
     free(p->worldTriangles);
     free(p->triangleMesh);
     free(p->meshTriangleBase);
@@ -216,8 +217,9 @@ static VKRT_Result buildLights(VKRT* vkrt, PackedScene* p) {
     float* meshWeights = (float*)calloc(allocMeshes, sizeof(float));
     float* pmfScratch = (float*)calloc(allocTris > allocMeshes ? allocTris : allocMeshes, sizeof(float));
     uint32_t* sourceMesh = (uint32_t*)calloc(allocMeshes, sizeof(uint32_t));
+    p->emissiveMeshMaterial = (uint32_t*)calloc(allocMeshes, sizeof(uint32_t));
     if (!p->emissiveMeshes || !p->emissiveTriangles || !p->meshAliasQ || !p->meshAliasIdx || !p->triAliasQ ||
-        !p->triAliasIdx || !meshWeights || !pmfScratch || !sourceMesh) {
+        !p->triAliasIdx || !meshWeights || !pmfScratch || !sourceMesh || !p->emissiveMeshMaterial) {
         free(meshWeights);
         free(pmfScratch);
         free(sourceMesh);
@@ -283,6 +285,7 @@ static VKRT_Result buildLights(VKRT* vkrt, PackedScene* p) {
         for (int k = 0; k < 3; k++) em->emission[k] = m->emissionColor[k] * m->emissionLuminance;
         meshWeights[nMesh] = selection;
         sourceMesh[nMesh] = mi;
+        p->emissiveMeshMaterial[nMesh] = mesh->info.materialIndex;
         totalSelection += selection;
         nMesh++;
     }

@@ -63,6 +63,7 @@ typedef struct mrt_light_desc {
     const uint32_t* meshAliasIdx;
     const float* triAliasQ;
     const uint32_t* triAliasIdx;
+    const uint32_t* emissiveMeshMaterial; /* material index of each emissive mesh */
 } mrt_light_desc;
 int mrt_upload_lights(mrt_context* ctx, const mrt_light_desc* desc);
 
@@ -112,6 +113,24 @@ int mrt_resize_accumulation(mrt_context* ctx, uint32_t width, uint32_t height);
 int mrt_clear_accumulation(mrt_context* ctx);
 int mrt_read_accumulation(mrt_context* ctx, float* out, size_t floatCount);
 int mrt_read_output(mrt_context* ctx, uint16_t* out, size_t count);
+
+/* Path-replay backpropagation of one frame (RGB mode): gradients of
+ * sum_px dImage[px] . I[px], where I is the frame the same parameters and
+ * frame number render, with respect to material base colour, roughness and
+ * emission and to environment texels. dImage holds width*height*3 floats of
+ * the accumulation extent; gradient arrays (materialCount*3, materialCount,
+ * materialCount*3, envWidth*envHeight*3) may be NULL. */
+typedef struct mrt_gradients {
+    float* baseColor;
+    float* roughness;
+    float* emission;
+    uint32_t materialCount;
+    float* environment;
+    uint32_t environmentWidth;
+    uint32_t environmentHeight;
+} mrt_gradients;
+int mrt_render_backward(mrt_context* ctx, const SceneData* scene, const mrt_render_params* params,
+                        const float* dImage, mrt_gradients* grads, mrt_render_stats* stats);
 
 int mrt_trace_rays(mrt_context* ctx, const float* rays, size_t count, uint32_t* outMesh, uint32_t* outPrim,
                    float* outT, float* outBary);
