@@ -74,6 +74,7 @@ extern "C" void mrt_destroy(mrt_context* ctx) {
     ctx->meshTriBase.release();
     ctx->tris.release();
     ctx->nodes.release();
+    ctx->shadeTris.release();
     ctx->meshes.release();
     ctx->materials.release();
     ctx->emissiveMeshes.release();
@@ -153,6 +154,7 @@ extern "C" int mrt_upload_geometry(mrt_context* ctx, const mrt_geometry_desc* g)
               ctx->meshTriBase.upload(g->meshTriangleBase, g->meshCount, s);
     if (!ok) return -3;
     ctx->triCount = g->triangleCount;
+    ctx->shadeTrisDirty = true;
     return sync(ctx, "upload geometry");
 }
 
@@ -160,6 +162,7 @@ extern "C" int mrt_upload_meshes(mrt_context* ctx, const MeshInfo* meshInfos, ui
     if (!ctx) return -1;
     if (!check(hipSetDevice(ctx->device), "hipSetDevice")) return -2;
     if (!ctx->meshes.upload(meshInfos, meshCount, ctx->stream)) return -3;
+    ctx->shadeTrisDirty = true;
     return sync(ctx, "upload meshes");
 }
 

@@ -111,6 +111,21 @@ struct alignas(16) TriRecord {
     float4 v2;
 };
 
+// Per-triangle shading record in global triangle order, 64 bytes (one cache
+// line): everything surface reconstruction needs besides the mesh transform,
+// so a hit costs one line instead of gathers of the index triple and three
+// 48-byte vertices. The geometric normal is computed once per scene change
+// with exactly the arithmetic reconstruction would use.
+struct alignas(16) ShadeTri {
+    uint32_t mesh;
+    float gn[3];        // world geometric normal (normalised, flipped for mirroring transforms)
+    uint32_t normal[3];  // packed octahedral shading normals of the three vertices
+    uint32_t tangent[3];
+    uint32_t color[3];
+    uint32_t pad[3];
+};
+static_assert(sizeof(ShadeTri) == 64, "ShadeTri layout");
+
 // What the kernels read of the scene. Passed by value as a kernel argument.
 struct SceneView {
     const BvhNode* nodes;
@@ -138,6 +153,7 @@ struct SceneView {
     const uint32_t* envAliasIdx;
     const float* envPdf;
     const float* rgb2spec;
+    const ShadeTri* shadeTris;
 };
 
 struct Hit {

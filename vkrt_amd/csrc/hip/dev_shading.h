@@ -210,28 +210,54 @@ DEV F4 ldVertexPos(const ShaderVertex& v) { return F4{v.position[0], v.position[
 
 // src/shaders/integrator/path/surface_state.slang:14-30 with
 // src/shaders/geometry/surface/reconstruct.slang:8-49 (no normal texture)
-DEV Surface reconstruct(const SceneView& s, F3 o, F3 d, const Hit& h) {
-    Surface sf;
-    uint32_t mesh = s.triMesh[h.tri];
+// The per-triangle part of surface reconstruction (src/shaders/geometry/surface/
+// reconstruct.slang:8-49): mesh and the world geometric normal from the object
+// edges through the mesh transform, plus the packed vertex attributes.
+DEV ShadeTri buildShadeTri(const SceneView& s, uint32_t tri) {
+    ShadeTri r;
+    uint32_t mesh = s.triMesh[tri];
     const MeshInfo m = s.meshes[mesh];
-    uint32_t prim = h.tri - s.meshTriBase[mesh];
+    uint32_t prim = tri - s.meshTriBase[mesh];
     uint32_t tb = m.indexBase + prim * 3u;
     const ShaderVertex v0 = s.vertices[s.indices[tb + 0] + m.vertexBase];
     const ShaderVertex v1 = s.vertices[s.indices[tb + 1] + m.vertexBase];
     const ShaderVertex v2 = s.vertices[s.indices[tb + 2] + m.vertexBase];
-    sf.p = o + d * h.t;
-    F3 objN = safeNormalize(
-        bary3(unpackOctNormal(v0.packedNormal), unpackOctNormal(v1.packedNormal), unpackOctNormal(v2.packedNormal), h.u, h.v));
-    F4 objT = bary4(unpackOctTangent(v0.packedTangent), unpackOctTangent(v1.packedTangent),
-                    unpackOctTangent(v2.packedTangent), h.u, h.v);
-    float hand = objT.w < 0.0f ? -1.0f : 1.0f;
-    F3 snU = meshNormal(m, objN);
-    float facing = dot(snU, d) > 0.0f ? -1.0f : 1.0f;
     F4 p0 = ldVertexPos(v0), p1 = ldVertexPos(v1), p2 = ldVertexPos(v2);
     F3 e1 = meshVector(m, F3{p1.x - p0.x, p1.y - p0.y, p1.z - p0.z});
     F3 e2 = meshVector(m, F3{p2.x - p0.x, p2.y - p0.y, p2.z - p0.z});
     F3 gn = safeNormalize(cross(e1, e2));
     if (m.scale[0] * m.scale[1] * m.scale[2] < 0.0f) gn = -gn;
+    r.mesh = mesh;
+    r.gn[0] = gn.x;
+    r.gn[1] = gn.y;
+    r.gn[2] = gn.z;
+    r.normal[0] = v0.packedNormal;
+    r.normal[1] = v1.packedNormal;
+    r.normal[2] = v2.packedNormal;
+    r.tangent[0] = v0.packedTangent;
+    r.tangent[1] = v1.packedTangent;
+    r.tangent[2] = v2.packedTangent;
+    r.color[0] = v0.packedColor;
+    r.color[1] = v1.packedColor;
+    r.color[2] = v2.packedColor;
+    r.pad[0] = r.pad[1] = r.pad[2] = 0u;
+    return r;
+}
+
+DEV Surface reconstruct(const SceneView& s, F3 o, F3 d, const Hit& h) {
+    Surface sf;
+    const uint4* w = reinterpret_cast<const uint4*>(s.shadeTris + h.tri);
+    const uint4 w0 = w[0], w1 = w[1], w2 = w[2], w3 = w[3];
+    const uint32_t mesh = w0.x;
+    const F3 gn = F3{__uint_as_float(w0.y), __uint_as_float(w0.z), __uint_as_float(w0.w)};
+    // normals w1.xyz, tangents w1.w w2.xy, colours w2.zw w3.x
+    const MeshInfo m = s.meshes[mesh];
+    sf.p = o + d * h.t;
+    F3 objN = safeNormalize(bary3(unpackOctNormal(w1.x), unpackOctNormal(w1.y), unpackOctNormal(w1.z), h.u, h.v));
+    F4 objT = bary4(unpackOctTangent(w1.w), unpackOctTangent(w2.x), unpackOctTangent(w2.y), h.u, h.v);
+    float hand = objT.w < 0.0f ? -1.0f : 1.0f;
+    F3 snU = meshNormal(m, objN);
+    float facing = dot(snU, d) > 0.0f ? -1.0f : 1.0f;
     sf.mesh = mesh;
     sf.materialIndex = m.materialIndex;
     sf.frontFace = dot(gn, d) < 0.0f ? 1u : 0u;
@@ -239,7 +265,7 @@ DEV Surface reconstruct(const SceneView& s, F3 o, F3 d, const Hit& h) {
     sf.ng = sf.frontFace ? gn : -gn;
     F3 tw = safeNormalize(meshVector(m, F3{objT.x, objT.y, objT.z})) * facing;
     F4 tangent = F4{tw.x, tw.y, tw.z, hand};
-    F4 col = bary4(unpackColor(v0.packedColor), unpackColor(v1.packedColor), unpackColor(v2.packedColor), h.u, h.v);
+    F4 col = bary4(unpackColor(w2.z), unpackColor(w2.w), unpackColor(w3.x), h.u, h.v);
     sf.vertexColor = F3{col.x, col.y, col.z};
     Basis unperturbed = makeBasis(sn, tangent);
     sn = unperturbed.n;
@@ -259,10 +285,14 @@ struct Spec {
     uint32_t res, scaleOffset, dataOffset;
 };
 
-// src/shaders/utility/rgb2spec.slang:14-90
+// src/shaders/utility/rgb2spec.slang:14-90. The table resolution is fixed at
+// compile time (the shipped table is 64^3; the host rejects others): constant
+// strides let the 24 corner loads share two base addresses, which keeps the
+// hero shading kernel within its register budget.
+constexpr uint32_t kSpecRes = 64u;
 DEV uint32_t specInterval(const Spec& sp, float x) {
     int left = 0;
-    int size = (int)sp.res - 2;
+    int size = (int)kSpecRes - 2;
     while (size > 0) {
         int half = size >> 1;
         int middle = left + half + 1;
@@ -273,7 +303,7 @@ DEV uint32_t specInterval(const Spec& sp, float x) {
             size = half;
         }
     }
-    return min((uint32_t)left, sp.res - 2u);
+    return min((uint32_t)left, kSpecRes - 2u);
 }
 
 DEV F3 specFetch(const Spec& sp, F3 rgb) {
@@ -283,14 +313,14 @@ DEV F3 specFetch(const Spec& sp, F3 rgb) {
     uint32_t dom = 0;
     if (ch[1] >= ch[dom]) dom = 1;
     if (ch[2] >= ch[dom]) dom = 2;
-    float xyScale = (float)(sp.res - 1u) / z;
+    float xyScale = (float)(kSpecRes - 1u) / z;
     float x = ch[dom == 2 ? 0 : dom + 1] * xyScale;
     float y = ch[dom == 0 ? 2 : dom - 1] * xyScale;
-    uint32_t xi = min((uint32_t)x, sp.res - 2u);
-    uint32_t yi = min((uint32_t)y, sp.res - 2u);
+    uint32_t xi = min((uint32_t)x, kSpecRes - 2u);
+    uint32_t yi = min((uint32_t)y, kSpecRes - 2u);
     uint32_t zi = specInterval(sp, z);
-    uint32_t off = (((dom * sp.res + zi) * sp.res + yi) * sp.res + xi) * 3u;
-    uint32_t dx = 3u, dy = 3u * sp.res, dz = dy * sp.res;
+    uint32_t off = (((dom * kSpecRes + zi) * kSpecRes + yi) * kSpecRes + xi) * 3u;
+    uint32_t dx = 3u, dy = 3u * kSpecRes, dz = dy * kSpecRes;
     float x1 = x - (float)xi, y1 = y - (float)yi, x0 = 1.0f - x1, y0 = 1.0f - y1;
     float s0 = sp.table[sp.scaleOffset + zi], s1 = sp.table[sp.scaleOffset + zi + 1u];
     float z1 = (z - s0) / fmaxf(s1 - s0, 1e-8f);

@@ -74,6 +74,8 @@ struct mrt_context {
     mrt::DeviceArray<uint32_t> meshTriBase;
     mrt::DeviceArray<mrt::TriRecord> tris;
     mrt::DeviceArray<mrt::BvhNode> nodes;
+    mrt::DeviceArray<mrt::ShadeTri> shadeTris;
+    bool shadeTrisDirty = true;  // geometry or mesh transforms changed since the records were built
     mrt::DeviceArray<MeshInfo> meshes;
     mrt::DeviceArray<Material> materials;
     mrt::DeviceArray<EmissiveMesh> emissiveMeshes;
@@ -132,6 +134,7 @@ struct mrt_context {
         v.envAliasIdx = envAliasIdx.ptr;
         v.envPdf = envPdf.ptr;
         v.rgb2spec = rgb2spec.ptr;
+        v.shadeTris = shadeTris.ptr;
         return v;
     }
 };

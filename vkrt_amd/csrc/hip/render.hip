@@ -343,16 +343,17 @@ __global__ __launch_bounds__(kBlock) void kGenerate(FrameParams fp, PathView pv,
                                 360.0f + fracf(u + 0.5f) * 470.0f, 360.0f + fracf(u + 0.75f) * 470.0f);
         pv.radXYZ[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         pv.tpp[i] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
-        pv.ptpp[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        pv.pbtp[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        // ptpp / pbtp are read only after a BSDF sample has written them
         flags |= kFHeroActive;
     }
     pv.rng[i] = storeRng(rng);
     pv.flags[i] = flags;
     pv.thr[i] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
     pv.rad[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    pv.misc[i] = make_float4(0.0f, 1.0f, 0.0f, 0.0f);
-    pv.sigma[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    // hero paths write every field of misc when they collapse to one wavelength,
+    // before anything reads it
+    if (MODE != kModeHero) pv.misc[i] = make_float4(0.0f, 1.0f, 0.0f, 0.0f);
+    // sigma is read only inside an absorbing medium (kFAbsorption), after a transmission wrote it
     q.o[i] = make_float4(o.x, o.y, o.z, __uint_as_float(i));
     q.d[i] = make_float4(d.x, d.y, d.z, kRayTMax);
 }
@@ -514,10 +515,13 @@ __global__ __launch_bounds__(kTraceBlock) void kShadow(SceneView sv, PathView pv
     if (st.dropped) atomicAdd(&counters[kCntDropped], st.dropped);
 }
 
-// One path vertex for every ray of the current queue. The hero-wavelength
-// variant needs about 180 registers unconstrained (two waves per SIMD); asking
-// for three waves per SIMD costs a handful of spilled registers and raises the
-// latency hiding of this memory-bound stage by half.
+// One path vertex for every ray of the current queue, at three waves per SIMD
+// (at most 168 registers). Every variant must fit that budget without spilling
+// registers to scratch: a hero build that spilled about 60 registers gave wrong
+// NEE results while the same source at two waves matched the oracle bit for
+// bit. What keeps the hero variant inside the budget: the rgb2spec resolution is
+// a compile-time constant, medium coefficients and the scalar-wavelength state
+// are read only where used, and shadow rays are queued before BSDF sampling.
 template <int MODE>
 __global__ __launch_bounds__(kBlock, 3) void kShade(FrameParams fp, SceneView sv, PathView pv, QueueView in,
                                                  const float4* hits, QueueView out, ShadowView sh, ShadowView she,
@@ -551,7 +555,9 @@ __global__ __launch_bounds__(kBlock, 3) void kShade(FrameParams fp, SceneView sv
         const bool refractive = (flags & kFRefractive) != 0u;
         const bool heroActive = MODE == kModeHero && (flags & kFHeroActive) != 0u;
         F4 thr = ld4(pv.thr[q]);
-        F4 misc = ld4(pv.misc[q]);  // prevBsdfPdf, scalar throughput, scalar wavelength, scalar 1/pdf
+        // prevBsdfPdf, scalar throughput, scalar wavelength, scalar 1/pdf; unused while
+        // all four hero wavelengths are alive
+        F4 misc = heroActive ? f4(0.0f) : ld4(pv.misc[q]);
 
         if (h.tri == kNoHit) {
             if (!fp.neeOnly && !(refractive || absorbing)) {
@@ -585,8 +591,11 @@ __global__ __launch_bounds__(kBlock, 3) void kShade(FrameParams fp, SceneView sv
             break;
         }
 
-        F4 sigma = ld4(pv.sigma[q]);
+        // the medium coefficients exist only inside an absorbing medium; they are
+        // re-read where needed rather than held in registers through the vertex
+        auto mediumSigma = [&]() { return absorbing ? ld4(pv.sigma[q]) : f4(0.0f); };
         if (absorbing) {
+            const F4 sigma = mediumSigma();
             if (MODE == kModeRgb) {
                 F3 t = transmittance3(sigma, true, h.t);
                 thr = F4{thr.x * t.x, thr.y * t.y, thr.z * t.z, thr.w};
@@ -629,7 +638,7 @@ __global__ __launch_bounds__(kBlock, 3) void kShade(FrameParams fp, SceneView sv
                         Eval e = evalScalar(fp.spec, st, wl);
                         if (e.pdf > 0.0f) {
                             float mis = powerHeuristic(ls.pdfSolidAngle, e.pdf);
-                            F3 tr = transmittance3(sigma, absorbing, ls.shadowDistance);
+                            F3 tr = transmittance3(mediumSigma(), absorbing, ls.shadowDistance);
                             F3 fc = e.value * fabsf(wl.z);
                             F3 r = tr * mis * fc * ls.emission / ls.pdfSolidAngle;
                             nee = F4{r.x, r.y, r.z, 0.0f};
@@ -639,7 +648,7 @@ __global__ __launch_bounds__(kBlock, 3) void kShade(FrameParams fp, SceneView sv
                         F4 fc = evalHero(fp.spec, st, wl, lam, btp) * fabsf(wl.z);
                         float mis = spectralMis(tpp * ls.pdfSolidAngle, tpp * btp);
                         if (mis > 0.0f) {
-                            F4 tr = transmittance4(sigma, absorbing, ls.shadowDistance);
+                            F4 tr = transmittance4(mediumSigma(), absorbing, ls.shadowDistance);
                             nee = tr * mis * fc * spec4(fp.spec, ls.emission, lam) / ls.pdfSolidAngle;
                         }
                     } else {
@@ -647,7 +656,7 @@ __global__ __launch_bounds__(kBlock, 3) void kShade(FrameParams fp, SceneView sv
                         if (e.pdf > 0.0f) {
                             float sv1 = e.value.x * fabsf(wl.z) * spec1(fp.spec, ls.emission, stateLambda);
                             float mis = powerHeuristic(ls.pdfSolidAngle, e.pdf);
-                            nee.x = mis * transmittance3(sigma, absorbing, ls.shadowDistance).x * sv1 / ls.pdfSolidAngle;
+                            nee.x = mis * transmittance3(mediumSigma(), absorbing, ls.shadowDistance).x * sv1 / ls.pdfSolidAngle;
                         }
                     }
                 }
@@ -756,6 +765,25 @@ __global__ __launch_bounds__(kBlock, 3) void kShade(FrameParams fp, SceneView sv
             }
         }
 
+        // The shadow payloads are final here: queue them now rather than at the end
+        // so they are not held in registers through BSDF sampling (waveAppend works
+        // on whichever lanes are still active).
+        {
+            const uint32_t sslot = waveAppend(shadowEmit, &counters[kCntShadow]);
+            if (shadowEmit) {
+                sh.o[sslot] = make_float4(shO.x, shO.y, shO.z, __uint_as_float(q));
+                sh.d[sslot] = make_float4(shD.x, shD.y, shD.z, shT);
+                sh.a[sslot] = st4(shA);
+                sh.b[sslot] = st4(shB);
+            }
+            const uint32_t eslot = waveAppend(envEmit, &counters[kCntShadowEnv]);
+            if (envEmit) {
+                she.o[eslot] = make_float4(seO.x, seO.y, seO.z, __uint_as_float(q));
+                she.d[eslot] = make_float4(seD.x, seD.y, seD.z, kRayTMax);
+                she.a[eslot] = st4(seA);
+            }
+        }
+
         // the last vertex of the loop never traces its continuation
         if (fp.depth + 1u >= sd.rrMaxDepth) break;
 
@@ -814,6 +842,7 @@ __global__ __launch_bounds__(kBlock, 3) void kShade(FrameParams fp, SceneView sv
             misc.x = e.pdf;
         }
         const bool nowHero = MODE == kModeHero && (newFlags & kFHeroActive) != 0u;
+        F4 sigma = mediumSigma();
         if (curNee && isT == 0u) newFlags |= kFPrevNee;
         if (envTried && isT == 0u) newFlags |= kFPrevEnvNee;
         if (nowHero)
@@ -851,9 +880,9 @@ __global__ __launch_bounds__(kBlock, 3) void kShade(FrameParams fp, SceneView sv
         cont = true;
         pv.rng[q] = storeRng(rng);
         pv.thr[q] = st4(thr);
-        pv.misc[q] = st4(misc);
+        if (!nowHero) pv.misc[q] = st4(misc);
         pv.flags[q] = newFlags;
-        pv.sigma[q] = st4(sigma);
+        if (newFlags & kFAbsorption) pv.sigma[q] = st4(sigma);
         if (MODE == kModeHero) pv.tpp[q] = st4(tpp);
     } while (false);
 
@@ -861,19 +890,6 @@ __global__ __launch_bounds__(kBlock, 3) void kShade(FrameParams fp, SceneView sv
     if (cont) {
         out.o[slot] = make_float4(nextO.x, nextO.y, nextO.z, __uint_as_float(q));
         out.d[slot] = make_float4(nextD.x, nextD.y, nextD.z, kRayTMax);
-    }
-    uint32_t sslot = waveAppend(shadowEmit, &counters[kCntShadow]);
-    if (shadowEmit) {
-        sh.o[sslot] = make_float4(shO.x, shO.y, shO.z, __uint_as_float(q));
-        sh.d[sslot] = make_float4(shD.x, shD.y, shD.z, shT);
-        sh.a[sslot] = st4(shA);
-        sh.b[sslot] = st4(shB);
-    }
-    uint32_t eslot = waveAppend(envEmit, &counters[kCntShadowEnv]);
-    if (envEmit) {
-        she.o[eslot] = make_float4(seO.x, seO.y, seO.z, __uint_as_float(q));
-        she.d[eslot] = make_float4(seD.x, seD.y, seD.z, kRayTMax);
-        she.a[eslot] = st4(seA);
     }
 }
 
@@ -1187,7 +1203,7 @@ __global__ __launch_bounds__(kTraceBlock) void kShadeAdjoint(FrameParams fp, Sce
                 }
                 break;
             }
-            F4 sigma = ld4(pv.sigma[q]);
+            F4 sigma = absorbing ? ld4(pv.sigma[q]) : f4(0.0f);
             if (absorbing) {
                 F3 t = transmittance3(sigma, true, h.t);
                 thr = F4{thr.x * t.x, thr.y * t.y, thr.z * t.z, thr.w};
@@ -1345,7 +1361,7 @@ __global__ __launch_bounds__(kTraceBlock) void kShadeAdjoint(FrameParams fp, Sce
             pv.thr[q] = st4(thr);
             pv.misc[q] = st4(misc);
             pv.flags[q] = newFlags;
-            pv.sigma[q] = st4(sigma);
+            if (newFlags & kFAbsorption) pv.sigma[q] = st4(sigma);
             pv.rad[q] = make_float4(lrem.x, lrem.y, lrem.z, 0.0f);
         } while (false);
         uint32_t slot = waveAppend(cont, &counters[outSlot]);
@@ -1371,8 +1387,28 @@ __global__ __launch_bounds__(kTraceBlock) void kShadeAdjoint(FrameParams fp, Sce
     }
 }
 
+__global__ __launch_bounds__(kBlock) void kBuildShadeTris(SceneView sv, uint32_t n, ShadeTri* out) {
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) out[i] = buildShadeTri(sv, i);
+}
+
 // ------------------------------------------------------------- host side --
 static uint32_t blocksFor(uint64_t n) { return (uint32_t)((n + kBlock - 1) / kBlock); }
+
+// Per-triangle shading records, rebuilt after a geometry or mesh upload.
+bool ensureShadeTris(mrt_context* ctx) {
+    if (!ctx->shadeTrisDirty) return true;
+    uint32_t n = ctx->triCount;
+    if (n > 0) {
+        if (!ctx->shadeTris.reserve(n)) return false;
+        ctx->shadeTris.count = n;
+        hipLaunchKernelGGL(kBuildShadeTris, dim3(blocksFor(n)), dim3(kBlock), 0, ctx->stream, ctx->view(), n,
+                           ctx->shadeTris.ptr);
+        if (!check(hipGetLastError(), "shading records")) return false;
+    }
+    ctx->shadeTrisDirty = false;
+    return true;
+}
 
 bool ensureTraversalScratch(mrt_context* ctx) {
     if (!ctx->counters.reserve(kCounterWords)) return false;
@@ -1542,6 +1578,10 @@ static int setupFrame(mrt_context* ctx, const SceneData* scene, const mrt_render
         setError("rgb2spec table not uploaded");
         return -2;
     }
+    if (scene->rgb2specSRGB.res != kSpecRes) {
+        setError("the shading kernels are built for a 64^3 rgb2spec table");
+        return -2;
+    }
     if (ctx->triCount > 0 && ctx->bvhMaxDepth * (kBvhWidth - 1u) + 1u > (uint32_t)(kLdsStack + kGlobalStack)) {
         setError("BVH deeper than the traversal stack supports");
         return -2;
@@ -1559,7 +1599,7 @@ static int setupFrame(mrt_context* ctx, const SceneData* scene, const mrt_render
     uint32_t spp = scene->samplesPerPixel > 1u ? scene->samplesPerPixel : 1u;
     uint64_t numPixels = (uint64_t)vw * vh;
     if (prm->resetAccumulation && mrt_clear_accumulation(ctx) != 0) return -2;
-    if (!ensureTraversalScratch(ctx)) return -3;
+    if (!ensureTraversalScratch(ctx) || !ensureShadeTris(ctx)) return -3;
     if (!ctx->frameSum.reserve(numPixels ? numPixels : 1)) return -3;
     hipLaunchKernelGGL(kFill4, dim3(blocksFor(numPixels ? numPixels : 1)), dim3(kBlock), 0, s, ctx->frameSum.ptr,
                        (size_t)numPixels, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
