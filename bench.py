@@ -79,6 +79,9 @@ def parse_args():
     p.add_argument("--cpu-rows", type=int, default=CPU_ROWS)
     p.add_argument("--workload", choices=sorted(WORKLOADS), default="spectral",
                    help="spectral: the headline 1080p spectral workload; stress: the 20.8M-triangle traversal test")
+    p.add_argument("--prb-iters", type=int, default=200,
+                   help="Adam iterations of the inverse-rendering variant (0: forward/backward timings only, "
+                        "-1: no PRB variant)")
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                    help="only run device-to-device copies of MIB mebibytes (reference kernel for HBM counters)")
     return p.parse_args()
@@ -148,6 +151,92 @@ def cpu_baseline(renderer, rows: int, spp: int, depth: int):
                   f"{depth}, same scene, mode and seeds; FP32 C oracle with OpenMP; "
                   f"{bounces} bounces in {dt:.2f} s; oracle BVH build ({build_s:.1f} s) excluded",
     }
+
+
+def prb_variant(iters: int):
+    """Differentiable configuration: the Cornell geometry with 64 materials (one
+    per face patch) and a 512x256 environment, 512x512, 32 spp forward and 32 spp
+    backward from 3 seeded views. Times VKRT_draw and VKRT_renderBackward per view
+    (wall clock around the C calls, device synchronised), then runs `iters` Adam
+    iterations against targets rendered with parameters perturbed by +-20 %."""
+    import torch
+
+    import vkrt_amd as V
+    from vkrt_amd import autograd as A
+    from vkrt_amd import scenes
+
+    size, spp = 512, 32
+    r = V.Renderer(size, size)
+    scene = scenes.differentiable_cornell()
+    scenes.load(r, scene)
+    r.configure(spp=spp, max_depth=8, min_depth=MIN_DEPTH, mode=V.RENDER_MODE_RGB, rng=V.RNG_PCG32, seed=1)
+    views = scenes.prb_views()
+    env_shape = scene.environment.shape[:2]
+    d_image = np.random.default_rng(0).normal(size=(size, size, 3)).astype(np.float32)
+    fwd, bwd, bwd_dev = [], [], []
+    for rep in range(3):
+        for v in views:
+            r.set_camera(*v)
+            r.start(size, size, 0)
+            r.draw()  # the first frame after a camera change also re-packs the scene
+            r.start(size, size, 0)
+            r.synchronize()
+            t0 = time.perf_counter()
+            r.draw()
+            r.synchronize()
+            t1 = time.perf_counter()
+            r.start(size, size, 0)
+            r.synchronize()
+            t2 = time.perf_counter()
+            r.render_backward(d_image, environment_shape=env_shape)
+            t3 = time.perf_counter()
+            if rep > 0:
+                fwd.append((t1 - t0) * 1e3)
+                bwd.append((t3 - t2) * 1e3)
+                bwd_dev.append(r.stats().backwardMs)
+    fwd_ms, bwd_ms = float(np.mean(fwd)), float(np.mean(bwd))
+    log(f"PRB: forward {fwd_ms:.2f} ms, backward {bwd_ms:.2f} ms per view (512x512, 32 spp) -> {bwd_ms / fwd_ms:.2f}x")
+    out = {"fwd_ms": round(fwd_ms, 3), "bwd_ms": round(bwd_ms, 3), "bwd_device_ms": round(float(np.mean(bwd_dev)), 3),
+           "bwd_over_fwd": round(bwd_ms / fwd_ms, 3), "width": size, "height": size, "spp": spp, "views": len(views),
+           "parameters": 64 * 7 + env_shape[0] * env_shape[1] * 3,
+           "scene": "Cornell geometry, 68 face patches + light, 64 materials, 512x256 sky; RGB, depth 8, PCG32 seed 1"}
+    if iters > 0:
+        start = A.SceneParameters.from_renderer(r, scene.environment)
+        rng = np.random.default_rng(4)
+        target = A.SceneParameters(*[t * torch.tensor(rng.uniform(0.8, 1.2, t.shape), dtype=torch.float32)
+                                     for t in start.tensors()])
+        target.clamp_()
+        with torch.no_grad():
+            ref = [A.render(r, target, v) for v in views]
+        p = A.SceneParameters(*[t.clone().requires_grad_(True) for t in start.tensors()])
+        # step sizes per parameter kind: albedo and roughness live in [0, 1]; emission
+        # of non-emitting patches starts at 0 and is only weakly observed
+        opt = torch.optim.Adam([{"params": [p.base_color, p.roughness], "lr": 0.01},
+                                {"params": [p.emission], "lr": 0.002}, {"params": [p.environment], "lr": 0.002}])
+        losses = []
+        seen = None
+        t0 = time.perf_counter()
+        for it in range(iters):
+            opt.zero_grad()
+            loss = sum(((A.render(r, p, v) - t) ** 2).mean() for v, t in zip(views, ref))
+            loss.backward()
+            if it == 0:  # materials the views see: those with a non-negligible albedo gradient
+                gm = p.base_color.grad.abs().amax(dim=1)
+                seen = gm > 1e-3 * float(gm.max())
+                err0 = float((p.base_color - target.base_color).detach().abs()[seen].mean())
+            opt.step()
+            p.clamp_()
+            losses.append(float(loss.detach()))
+        el = time.perf_counter() - t0
+        err1 = float((p.base_color - target.base_color).detach().abs()[seen].mean())
+        log(f"PRB: {iters} Adam iterations in {el:.1f} s, loss {losses[0]:.3e} -> {losses[-1]:.3e}, "
+            f"albedo error of {int(seen.sum())} seen materials {err0:.4f} -> {err1:.4f}")
+        out["optimisation"] = {"iterations": iters, "seconds": round(el, 2), "loss_first": losses[0],
+                               "loss_last": losses[-1], "seen_materials": int(seen.sum()),
+                               "albedo_error_first": round(err0, 5), "albedo_error_last": round(err1, 5),
+                               "parameters_optimised": "albedo, roughness, emission of 64 materials + 512x256 sky"}
+    r.close()
+    return out
 
 
 def main():
@@ -271,6 +360,9 @@ def main():
                                "ms_per_step": round(vel * 1e3 / vsteps, 3),
                                "shadow_rays_per_step": vs // vsteps}
         log(f"env NEE variant: {vb / vel / 1e9:.3f} Gbounces/s, {vs // vsteps} shadow rays per frame")
+
+    if args.workload == "spectral" and args.prb_iters >= 0 and args.steps > 0:
+        variants["prb"] = prb_variant(args.prb_iters)
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -81,6 +81,7 @@ VKRT_Result VKRT_getMaterialSnapshot(const VKRT* vkrt, uint32_t materialIndex,
 VKRT_Result VKRT_getTextureCount(const VKRT* vkrt, uint32_t* outTextureCount);                        /* vkrt.h:77 */
 VKRT_Result VKRT_addTextureFromPixels(VKRT* vkrt, const VKRT_TextureUpload* upload,
                                       uint32_t* outTextureIndex);                                     /* vkrt.h:79 */
+VKRT_Result VKRT_removeTexture(VKRT* vkrt, uint32_t textureIndex);                                  /* vkrt.h:87 */
 VKRT_Result VKRT_addMaterial(VKRT* vkrt, const Material* material, const char* name,
                              uint32_t* outMaterialIndex);                                             /* vkrt.h:95 */
 VKRT_Result VKRT_setMaterial(VKRT* vkrt, uint32_t materialIndex, const Material* material);           /* vkrt.h:98 */

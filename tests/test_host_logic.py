@@ -170,3 +170,52 @@ def test_sky_environment_sun():
     assert env[..., :3].max() == pytest.approx(1.0e4)
     # a 0.5 degree disc covers about pi/4 * (0.5 / 0.176)^2 texels at 2048x1024, more near the poles
     assert 4 <= (env[..., 0] >= 1.0e4).sum() <= 20
+
+
+def test_remove_texture_shifts_indices_and_clears_the_environment():
+    """VKRT_removeTexture (src/core/scene/textures.c, vkrtSceneRemoveTexture)."""
+    r = V.Renderer(init_device=False)
+    px = np.ones((2, 4, 4), np.float32)
+    a, b, c = (r.add_texture_rgba32f(px * k) for k in (1, 2, 3))
+    r.set_environment_texture(c)
+    r.remove_texture(a)
+    assert r.texture_count() == 2
+    assert r.scene_settings().environmentTextureIndex == 1  # texture c moved down one index
+    mi = r.add_material(V.default_material())
+    m = r.material(mi)
+    m.baseColorTextureIndex = 0  # texture b, now at index 0
+    r.set_material(mi, m)
+    with pytest.raises(V.VkrtError):
+        r.remove_texture(0)  # still bound to a material
+    r.remove_texture(1)  # the environment's texture: clears the environment first
+    assert r.scene_settings().environmentTextureIndex == V.VKRT_INVALID_INDEX
+    assert r.texture_count() == 1
+    with pytest.raises(V.VkrtError):
+        r.remove_texture(5)
+    r.close()
+
+
+def test_differentiable_cornell_and_parameter_upload():
+    torch = pytest.importorskip("torch")
+    from vkrt_amd import autograd as A
+
+    r = V.Renderer(init_device=False)
+    s = scenes.differentiable_cornell()
+    scenes.load(r, s)
+    assert r.material_count() == 65 and len(s.meshes) == 69 and s.triangle_count == 138
+    assert s.environment.shape == (256, 512, 4)
+    p = A.SceneParameters.from_renderer(r, s.environment)
+    assert p.base_color.shape == (65, 3) and p.environment.shape == (256, 512, 3)
+    assert float(p.emission[64].max()) == pytest.approx(15.0)
+    base = p.base_color.clone()
+    base[7] = torch.tensor([0.1, 0.2, 0.3])
+    emis = p.emission.clone()
+    emis[7] = torch.tensor([0.0, 2.0, 1.0])
+    A._apply(r, base.numpy(), p.roughness.numpy(), emis.numpy(), p.environment.numpy() * 0.5)
+    m = r.material(7)
+    assert np.allclose(m.baseColor[:], [0.1, 0.2, 0.3])
+    assert np.allclose(np.asarray(m.emissionColor[:]) * m.emissionLuminance, [0.0, 2.0, 1.0])
+    assert r.texture_count() == 1  # the replaced environment texture was removed
+    assert r.scene_settings().environmentTextureIndex == 0
+    A.forget(r)
+    r.close()

@@ -399,6 +399,7 @@ def library() -> C.CDLL:
         "VKRT_getMaterialCount": (C.c_int, [P, up]),
         "VKRT_getTextureCount": (C.c_int, [P, up]),
         "VKRT_addTextureFromPixels": (C.c_int, [P, C.POINTER(TextureUpload), up]),
+        "VKRT_removeTexture": (C.c_int, [P, u32]),
         "VKRT_addMaterial": (C.c_int, [P, C.POINTER(Material), C.c_char_p, up]),
         "VKRT_setMaterial": (C.c_int, [P, u32, C.POINTER(Material)]),
         "VKRT_setMeshMaterialIndex": (C.c_int, [P, u32, u32]),
@@ -559,6 +560,33 @@ class Renderer:
 
     def set_environment_texture(self, index: int):
         _ok(self.lib.VKRT_setEnvironmentTexture(self.h, index), "VKRT_setEnvironmentTexture")
+
+    def remove_texture(self, index: int):
+        _ok(self.lib.VKRT_removeTexture(self.h, index), "VKRT_removeTexture")
+
+    def texture_count(self) -> int:
+        c = C.c_uint32()
+        _ok(self.lib.VKRT_getTextureCount(self.h, C.byref(c)), "VKRT_getTextureCount")
+        return c.value
+
+    def scene_settings(self) -> SceneSettingsSnapshot:
+        out = SceneSettingsSnapshot()
+        _ok(self.lib.VKRT_getSceneSettings(self.h, C.byref(out)), "VKRT_getSceneSettings")
+        return out
+
+    def replace_environment(self, pixels: np.ndarray) -> int:
+        """Uploads `pixels` (H x W x 3 or 4, linear) as the environment texture and
+        removes the texture it replaces."""
+        px = np.asarray(pixels, dtype=np.float32)
+        if px.shape[2] == 3:
+            px = np.concatenate([px, np.ones(px.shape[:2] + (1,), np.float32)], axis=2)
+        old = self.scene_settings().environmentTextureIndex
+        idx = self.add_texture_rgba32f(px)
+        self.set_environment_texture(idx)
+        if old != VKRT_INVALID_INDEX:
+            self.remove_texture(old)
+            idx = self.scene_settings().environmentTextureIndex
+        return idx
 
     def set_environment_light(self, color, strength: float):
         c = np.asarray(color, dtype=np.float32)

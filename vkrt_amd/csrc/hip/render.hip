@@ -1301,16 +1301,18 @@ __global__ __launch_bounds__(kTraceBlock) void kShadeAdjoint(FrameParams fp, Sce
                 }
             }
 
-            // emission with MIS
-            if (!fp.neeOnly && anyPos(mat.emission)) {
+            // emission with MIS. The emission derivative exists on every surface the path
+            // hits: a surface that emits nothing is no light source of the current
+            // estimator, so only the BSDF strategy reaches it and its weight is 1.
+            if (!fp.neeOnly) {
+                const bool emissive = anyPos(mat.emission);
                 const bool prevNee = (flags & kFPrevNee) != 0u;
                 float w = 1.0f;
-                if (fp.neeEnabled && prevNee && fp.depth != 0u && misc.x > 0.0f) {
+                if (emissive && fp.neeEnabled && prevNee && fp.depth != 0u && misc.x > 0.0f) {
                     float lp = areaToSolidAngle(sv.meshes[sf.mesh].lightPdfArea, sf.ng, d, h.t);
                     w = lp <= 0.0f ? 1.0f : powerHeuristic(misc.x, lp);
                 }
-                F3 em = mat.emission * w;
-                lrem = lrem - thr3 * em;
+                if (emissive) lrem = lrem - thr3 * (mat.emission * w);
                 F3 ge = dI * thr3 * w;
                 sink.add(mi, 4, ge.x);
                 sink.add(mi, 5, ge.y);

@@ -343,6 +343,48 @@ VKRT_Result VKRT_getTextureCount(const VKRT* vkrt, uint32_t* outTextureCount) {
     return VKRT_SUCCESS;
 }
 
+/* src/core/scene/textures.c (vkrtSceneRemoveTexture): removing the environment's
+ * texture first clears the environment; a texture still bound to a material slot
+ * cannot be removed; later textures move down one index, in the material slots
+ * and in the environment setting alike. */
+VKRT_Result VKRT_removeTexture(VKRT* vkrt, uint32_t textureIndex) {
+    if (!vkrt || textureIndex >= vkrt->textureCount) return VKRT_ERROR_INVALID_ARGUMENT;
+    for (uint32_t m = 0; m < vkrt->materialCount; m++) {
+        const Material* mat = &vkrt->materials[m].material;
+        if (mat->baseColorTextureIndex == textureIndex || mat->metallicRoughnessTextureIndex == textureIndex ||
+            mat->normalTextureIndex == textureIndex || mat->emissiveTextureIndex == textureIndex)
+            return VKRT_ERROR_OPERATION_FAILED;
+    }
+    if (vkrt->settings.environmentTextureIndex == textureIndex) {
+        VKRT_Result r = VKRT_setEnvironmentTexture(vkrt, VKRT_INVALID_INDEX);
+        if (r != VKRT_SUCCESS) return r;
+    }
+    free(vkrt->textures[textureIndex].rgba);
+    memmove(&vkrt->textures[textureIndex], &vkrt->textures[textureIndex + 1u],
+            sizeof(HostTexture) * (vkrt->textureCount - textureIndex - 1u));
+    vkrt->textureCount--;
+    uint8_t materialsMoved = 0;
+    for (uint32_t m = 0; m < vkrt->materialCount; m++) {
+        Material* mat = &vkrt->materials[m].material;
+        uint32_t* slots[4] = {&mat->baseColorTextureIndex, &mat->metallicRoughnessTextureIndex, &mat->normalTextureIndex,
+                              &mat->emissiveTextureIndex};
+        for (int k = 0; k < 4; k++) {
+            if (*slots[k] != VKRT_INVALID_INDEX && *slots[k] > textureIndex) {
+                (*slots[k])--;
+                materialsMoved = 1;
+            }
+        }
+    }
+    if (materialsMoved) vkrt->materialRevision++;
+    if (vkrt->settings.environmentTextureIndex != VKRT_INVALID_INDEX &&
+        vkrt->settings.environmentTextureIndex > textureIndex) {
+        vkrt->settings.environmentTextureIndex--;
+        vkrt->environmentRevision++;
+    }
+    vkrtResetSceneData(vkrt);
+    return VKRT_SUCCESS;
+}
+
 /* src/core/scene/environment.c:11-41: only linear textures may light the scene. */
 VKRT_Result VKRT_setEnvironmentTexture(VKRT* vkrt, uint32_t textureIndex) {
     if (!vkrt) return VKRT_ERROR_INVALID_ARGUMENT;

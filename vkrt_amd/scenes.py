@@ -107,6 +107,73 @@ def cornell_box() -> Scene:
     return s
 
 
+# ---------------------------------------------------- differentiable Cornell --
+def _subdivide_quad(pos, nrm, n):
+    """Splits one quad (corners p0..p3 in order) into n x n sub-quads."""
+    p0, p1, p2, p3 = (np.asarray(p, dtype=np.float64) for p in pos)
+    out = []
+    for i in range(n):
+        for j in range(n):
+            def at(a, b):
+                return (p0 * (1 - a) * (1 - b) + p1 * a * (1 - b) + p2 * a * b + p3 * (1 - a) * b)
+            a0, a1, b0, b1 = i / n, (i + 1) / n, j / n, (j + 1) / n
+            out.append(_quad(at(a0, b0), at(a1, b0), at(a1, b1), at(a0, b1), nrm[0]))
+    return out
+
+
+PRB_MATERIALS = 64
+
+
+def differentiable_cornell(seed: int = 0, env_width: int = 512, env_height: int = 256) -> Scene:
+    """The Cornell box geometry with one material per face patch for inverse rendering.
+
+    Every wall and box face is split into 2 x 2 patches (17 quads -> 68 patches, one
+    mesh each); patch k gets material k mod 63 and the ceiling light material 63, so
+    64 materials carry (albedo RGB, roughness, emission RGB). Materials 0..62 are
+    coated diffuse (specular 0.5, so roughness matters) with albedo U(0.2, 0.8) and
+    roughness U(0.2, 0.8); the light emits 15. The box is lit by the light and by a
+    512 x 256 sky (no sun) through its open side. Material 0 of the renderer is its
+    lazily created default, so scene material m is renderer material m + 1.
+    """
+    rng = np.random.default_rng(seed)
+    s = Scene()
+    for _ in range(PRB_MATERIALS - 1):
+        m = default_material(baseColor=tuple(rng.uniform(0.2, 0.8, 3)), roughness=float(rng.uniform(0.2, 0.8)),
+                             specular=0.5, metallic=0.0)
+        m.emissionLuminance = 0.0
+        s.materials.append(m)
+    s.materials.append(lambert((0.73, 0.73, 0.73), emission=15.0))
+    quads = [
+        _quad((0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1)),
+        _quad((0, 0, 1), (0, 1, 1), (1, 1, 1), (1, 0, 1), (0, 0, -1)),
+        _quad((0, 1, 0), (1, 1, 0), (1, 1, 1), (0, 1, 1), (0, -1, 0)),
+        _quad((0, 0, 0), (0, 1, 0), (0, 1, 1), (0, 0, 1), (1, 0, 0)),
+        _quad((1, 0, 0), (1, 0, 1), (1, 1, 1), (1, 1, 0), (-1, 0, 0)),
+    ] + _box(0.53, 0.83, 0.12, 0.42, 0.3) + _box(0.17, 0.47, 0.48, 0.78, 0.6)
+    k = 0
+    for pos, nrm, _ in quads:
+        for sub in _subdivide_quad(pos, nrm, 2):
+            s.meshes.append(MeshData(*_merge([sub]), k % (PRB_MATERIALS - 1)))
+            k += 1
+    zl = 1.0 - 1e-3
+    light = _quad((0.37, 0.37, zl), (0.63, 0.37, zl), (0.63, 0.63, zl), (0.37, 0.63, zl), (0, 0, -1))
+    s.meshes.append(MeshData(*_merge([light]), PRB_MATERIALS - 1))
+    s.camera = ((0.5, -1.35, 0.5), (0.5, 0.5, 0.5), (0.0, 0.0, 1.0), 40.0)
+    s.environment = sky_environment(env_width, env_height, sun_radiance=0.0)
+    return s
+
+
+def prb_views(seed: int = 0, count: int = 3):
+    """Seeded camera views into the open side of the box (position, target, up, vfov)."""
+    rng = np.random.default_rng(1000 + seed)
+    views = []
+    for _ in range(count):
+        pos = (0.5 + rng.uniform(-0.25, 0.25), -1.35 + rng.uniform(-0.15, 0.15), 0.5 + rng.uniform(-0.2, 0.2))
+        target = (0.5 + rng.uniform(-0.1, 0.1), 0.5, 0.5 + rng.uniform(-0.1, 0.1))
+        views.append((pos, target, (0.0, 0.0, 1.0), 40.0))
+    return views
+
+
 # ---------------------------------------------------------------- sphere --
 _ICO_T = (1.0 + 5.0 ** 0.5) / 2.0
 _ICO_V = np.array(
