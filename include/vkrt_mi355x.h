@@ -94,6 +94,55 @@ typedef struct VKRT_Gradients {
  * the reference (src/core/api/vkrt.h has no differentiable rendering). */
 VKRT_Result VKRT_renderBackward(VKRT* vkrt, const float* dLossdImage, VKRT_Gradients* outGradients);
 
+/* Neural radiance cache (an addition; the reference has no neural cache).
+ * RGB render mode only. When enabled, VKRT_draw shades the first
+ * prefixBounces bounces of every path as usual and then replaces the rest of
+ * the path with the cache's prediction of the radiance scattered at the next
+ * surface (emission there is still added); trainingPaths extra paths per frame
+ * through random pixels are traced in full, without the cache, and every
+ * surface vertex on them becomes a training record (the radiance the rest of
+ * that path carried back, divided by its throughput). After the frame the cache
+ * takes trainingSteps Adam steps of batchSize records each (relative L2 loss).
+ * The network: 12-frequency encoding of position, direction and normal plus
+ * roughness and albedo (192 inputs), 5 hidden layers of 64 ReLU units, 3
+ * outputs; fp16 weights on MFMA with fp32 accumulation. */
+typedef struct VKRT_NeuralCacheSettings {
+    uint32_t enabled;
+    uint32_t prefixBounces;   /* bounces traced before the cache is queried (2) */
+    uint32_t trainingPaths;   /* full paths per frame that produce training records (65536) */
+    uint32_t trainingSteps;   /* optimiser steps per frame (4) */
+    uint32_t batchSize;       /* records per step (16384) */
+    float learningRate;       /* Adam step size (1e-3) */
+    uint32_t train;           /* 0 keeps the weights fixed */
+} VKRT_NeuralCacheSettings;
+
+typedef struct VKRT_NeuralCacheStats {
+    float lastLoss;            /* mean relative L2 loss of the last frame's training steps */
+    uint32_t queries;          /* cache queries of the last frame */
+    uint32_t trainingRecords;  /* training records of the last frame */
+    float inferMs;             /* device time of the last frame's inference */
+    float trainMs;             /* device time of the last frame's training */
+} VKRT_NeuralCacheStats;
+
+void VKRT_defaultNeuralCacheSettings(VKRT_NeuralCacheSettings* outSettings);
+VKRT_Result VKRT_setNeuralCache(VKRT* vkrt, const VKRT_NeuralCacheSettings* settings);
+VKRT_Result VKRT_getNeuralCacheStats(const VKRT* vkrt, VKRT_NeuralCacheStats* outStats);
+
+/* Direct access to the cache network (tests and tools). Inputs: 11 floats per
+ * sample in [0, 1] -- position in the scene box (3), outgoing direction as
+ * theta/pi and phi/(2 pi) (2), shading normal likewise (2), roughness, albedo
+ * (3); outputs: 3 floats. Weights: the fp32 master parameters in the order
+ * W0 [64][192], W1..W4 [64][64], W5 [16][64] (rows 3..15 are zero). */
+uint32_t VKRT_neuralCacheParameterCount(void);
+VKRT_Result VKRT_neuralCacheReset(VKRT* vkrt, uint32_t seed);
+VKRT_Result VKRT_neuralCacheGetWeights(VKRT* vkrt, float* outWeights, size_t count);
+VKRT_Result VKRT_neuralCacheSetWeights(VKRT* vkrt, const float* weights, size_t count);
+VKRT_Result VKRT_neuralCacheInfer(VKRT* vkrt, const float* inputs, uint32_t count, float* outputs);
+VKRT_Result VKRT_neuralCacheTrain(VKRT* vkrt, const float* inputs, const float* targets, uint32_t count,
+                                  uint32_t steps, uint32_t batchSize, float learningRate, float* outLoss);
+VKRT_Result VKRT_neuralCacheGradient(VKRT* vkrt, const float* inputs, const float* targets, uint32_t count,
+                                     float* outGradient, float* outLoss);
+
 /* Host copies of the packed scene buffers exactly as the kernels consume them
  * (the reference's descriptor bindings 10-24, src/shaders/scene/resources.slang:7-59).
  * Pointers stay valid until the next scene change or VKRT_destroy. */

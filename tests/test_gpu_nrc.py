@@ -1,0 +1,138 @@
+"""Neural radiance cache network on the GPU against a numpy reference of the
+same operations: fp16 encoded inputs, fp16 weights and activations, fp32
+accumulation (the reference accumulates in float64 on fp16-rounded values)."""
+
+import math
+
+import numpy as np
+import pytest
+
+import vkrt_amd as V
+
+pytestmark = pytest.mark.gpu
+
+F16 = np.float16
+OFF_HIDDEN, OFF_OUT, PARAMS = 12288, 28672, 29696
+
+
+def encode(v):
+    """12-frequency sin/cos of the 7 periodic inputs, then roughness and albedo, a 1, zero padding."""
+    v = np.asarray(v, np.float32)
+    x = np.zeros((len(v), 192), np.float32)
+    for d in range(7):
+        for f in range(12):
+            s = np.float32(0.5 if f == 0 else 2.0 ** (f - 1))
+            a = v[:, d] * s
+            a = (a - np.floor(a)).astype(np.float64) * 2 * math.pi
+            x[:, d * 24 + 2 * f] = np.sin(a)
+            x[:, d * 24 + 2 * f + 1] = np.cos(a)
+    x[:, 168:172] = v[:, 7:11]
+    x[:, 172] = 1.0
+    return x
+
+
+def layers(w):
+    w = np.asarray(w, np.float32)
+    mats = [w[:OFF_HIDDEN].reshape(64, 192)]
+    mats += [w[OFF_HIDDEN + k * 4096:OFF_HIDDEN + (k + 1) * 4096].reshape(64, 64) for k in range(4)]
+    mats.append(w[OFF_OUT:].reshape(16, 64)[:3])
+    return [m.astype(F16).astype(np.float64) for m in mats]
+
+
+def forward(w, v):
+    mats = layers(w)
+    acts = [encode(v).astype(F16).astype(np.float64)]
+    for m in mats[:-1]:
+        acts.append(np.maximum(acts[-1] @ m.T, 0.0).astype(F16).astype(np.float64))
+    return acts, acts[-1] @ mats[-1].T
+
+
+def gradient(w, v, t):
+    mats = layers(w)
+    acts, y = forward(w, v)
+    den = y * y + 0.01
+    loss = float(np.sum((y - t) ** 2 / den))
+    d = np.clip(2 * (y - t) / den, -1000, 1000)
+    grads = [None] * 6
+    for k in range(5, -1, -1):
+        grads[k] = d.T @ acts[k]
+        if k > 0:
+            d = (d @ mats[k]) * (acts[k] > 0)
+    flat = np.zeros(PARAMS)
+    flat[:OFF_HIDDEN] = grads[0].ravel()
+    for k in range(4):
+        flat[OFF_HIDDEN + k * 4096:OFF_HIDDEN + (k + 1) * 4096] = grads[k + 1].ravel()
+    flat[OFF_OUT:OFF_OUT + 3 * 64] = grads[5].ravel()
+    return flat, loss
+
+
+def random_inputs(n, seed):
+    return np.random.default_rng(seed).uniform(0, 1, (n, 11)).astype(np.float32)
+
+
+@pytest.fixture(scope="module")
+def renderer():
+    r = V.Renderer(16, 16)
+    yield r
+    r.close()
+
+
+def test_parameter_layout(renderer):
+    renderer.nrc_reset(3)
+    w = renderer.nrc_weights()
+    assert w.size == PARAMS
+    assert np.all(w[OFF_OUT + 3 * 64:] == 0.0)  # padded output rows
+    assert np.all(w[:OFF_HIDDEN].reshape(64, 192)[:, 173:] == 0.0)  # zero-padded inputs
+    assert np.abs(w[:OFF_HIDDEN]).max() <= math.sqrt(6 / 173) + 1e-6
+
+
+def test_inference_matches_reference(renderer):
+    rng = np.random.default_rng(1)
+    w = rng.normal(0, 0.15, PARAMS).astype(np.float32)
+    renderer.nrc_set_weights(w)
+    w = renderer.nrc_weights()
+    for n in (1, 15, 16, 1000, 4099):  # ragged counts: partial blocks of 16 samples
+        v = random_inputs(n, n)
+        gpu = renderer.nrc_infer(v)
+        _, ref = forward(w, v)
+        scale = np.abs(ref).max() + 1e-3
+        err = np.abs(gpu - ref).max() / scale
+        print(f"n={n}: max |gpu - ref| / max |ref| = {err:.2e}")
+        assert err < 1e-2
+
+
+def test_gradient_matches_reference(renderer):
+    rng = np.random.default_rng(2)
+    renderer.nrc_set_weights(rng.normal(0, 0.15, PARAMS).astype(np.float32))
+    w = renderer.nrc_weights()
+    n = 777
+    v = random_inputs(n, 5)
+    t = rng.uniform(0, 2, (n, 3)).astype(np.float32)
+    g, loss = renderer.nrc_gradient(v, t)
+    ref, ref_loss = gradient(w, v, t)
+    assert loss == pytest.approx(ref_loss, rel=1e-2)
+    spans = [(0, OFF_HIDDEN)] + [(OFF_HIDDEN + k * 4096, OFF_HIDDEN + (k + 1) * 4096) for k in range(4)]
+    spans.append((OFF_OUT, OFF_OUT + 192))
+    for a, b in spans:
+        rel = np.linalg.norm(g[a:b] - ref[a:b]) / np.linalg.norm(ref[a:b])
+        print(f"params {a}:{b}: relative gradient error {rel:.2e}")
+        assert rel < 2e-2
+    assert np.all(g[OFF_OUT + 192:] == 0.0)
+
+
+def test_training_fits_a_smooth_function(renderer):
+    renderer.nrc_reset(7)
+    n = 1 << 15
+    v = random_inputs(n, 11)
+    t = np.stack([0.5 + 0.3 * np.sin(2 * math.pi * v[:, 0]) * v[:, 8],
+                  0.4 + 0.2 * v[:, 1] * v[:, 9],
+                  0.3 + 0.25 * np.cos(2 * math.pi * v[:, 5]) * v[:, 10]], 1).astype(np.float32)
+    first = renderer.nrc_train(v, t, 1, 4096, 1e-2) / (4096 * 3)
+    for _ in range(30):
+        last = renderer.nrc_train(v, t, 10, 4096, 1e-2) / (10 * 4096 * 3)
+    pred = renderer.nrc_infer(v[:2000])
+    mse = float(np.mean((pred - t[:2000]) ** 2))
+    var = float(np.mean(np.var(t[:2000], axis=0)))
+    print(f"relative L2 loss {first:.3e} -> {last:.3e}; mse after training {mse:.2e} (target variance {var:.2e})")
+    assert last < 0.05 * first
+    assert mse < 0.4 * var

@@ -285,6 +285,28 @@ class MaterialSnapshot(C.Structure):
     _fields_ = [("material", Material), ("useCount", C.c_uint32), ("name", C.c_char * 256), ("_pad", C.c_uint8 * 12)]
 
 
+class NeuralCacheSettings(C.Structure):
+    _fields_ = [
+        ("enabled", C.c_uint32),
+        ("prefixBounces", C.c_uint32),
+        ("trainingPaths", C.c_uint32),
+        ("trainingSteps", C.c_uint32),
+        ("batchSize", C.c_uint32),
+        ("learningRate", C.c_float),
+        ("train", C.c_uint32),
+    ]
+
+
+class NeuralCacheStats(C.Structure):
+    _fields_ = [
+        ("lastLoss", C.c_float),
+        ("queries", C.c_uint32),
+        ("trainingRecords", C.c_uint32),
+        ("inferMs", C.c_float),
+        ("trainMs", C.c_float),
+    ]
+
+
 class Gradients(C.Structure):
     _fields_ = [
         ("baseColor", C.POINTER(C.c_float)),
@@ -340,6 +362,8 @@ _EXPECTED_SIZES = {
     EmissiveTriangle: 48,
     SceneData: 240,
     MaterialSnapshot: 544,
+    NeuralCacheSettings: 28,
+    NeuralCacheStats: 20,
 }
 for _cls, _size in _EXPECTED_SIZES.items():
     assert C.sizeof(_cls) == _size, f"{_cls.__name__} is {C.sizeof(_cls)} bytes, expected {_size}"
@@ -400,6 +424,16 @@ def library() -> C.CDLL:
         "VKRT_getTextureCount": (C.c_int, [P, up]),
         "VKRT_addTextureFromPixels": (C.c_int, [P, C.POINTER(TextureUpload), up]),
         "VKRT_removeTexture": (C.c_int, [P, u32]),
+        "VKRT_defaultNeuralCacheSettings": (None, [C.POINTER(NeuralCacheSettings)]),
+        "VKRT_setNeuralCache": (C.c_int, [P, C.POINTER(NeuralCacheSettings)]),
+        "VKRT_getNeuralCacheStats": (C.c_int, [P, C.POINTER(NeuralCacheStats)]),
+        "VKRT_neuralCacheParameterCount": (u32, []),
+        "VKRT_neuralCacheReset": (C.c_int, [P, u32]),
+        "VKRT_neuralCacheGetWeights": (C.c_int, [P, fp, sz]),
+        "VKRT_neuralCacheSetWeights": (C.c_int, [P, fp, sz]),
+        "VKRT_neuralCacheInfer": (C.c_int, [P, fp, u32, fp]),
+        "VKRT_neuralCacheTrain": (C.c_int, [P, fp, fp, u32, u32, u32, f32, fp]),
+        "VKRT_neuralCacheGradient": (C.c_int, [P, fp, fp, u32, fp, fp]),
         "VKRT_addMaterial": (C.c_int, [P, C.POINTER(Material), C.c_char_p, up]),
         "VKRT_setMaterial": (C.c_int, [P, u32, C.POINTER(Material)]),
         "VKRT_setMeshMaterialIndex": (C.c_int, [P, u32, u32]),
@@ -668,6 +702,57 @@ class Renderer:
             g.environmentWidth, g.environmentHeight = w, h
         _ok(self.lib.VKRT_renderBackward(self.h, fptr(d), C.byref(g)), "VKRT_renderBackward")
         return out
+
+    # neural radiance cache --------------------------------------------
+    def set_neural_cache(self, enabled=True, **kw) -> NeuralCacheSettings:
+        st = NeuralCacheSettings()
+        self.lib.VKRT_defaultNeuralCacheSettings(C.byref(st))
+        st.enabled = 1 if enabled else 0
+        for k, v in kw.items():
+            setattr(st, k, v)
+        _ok(self.lib.VKRT_setNeuralCache(self.h, C.byref(st)), "VKRT_setNeuralCache")
+        return st
+
+    def neural_cache_stats(self) -> NeuralCacheStats:
+        out = NeuralCacheStats()
+        _ok(self.lib.VKRT_getNeuralCacheStats(self.h, C.byref(out)), "VKRT_getNeuralCacheStats")
+        return out
+
+    def nrc_reset(self, seed: int = 0):
+        _ok(self.lib.VKRT_neuralCacheReset(self.h, seed), "VKRT_neuralCacheReset")
+
+    def nrc_weights(self) -> np.ndarray:
+        n = self.lib.VKRT_neuralCacheParameterCount()
+        out = np.zeros(n, np.float32)
+        _ok(self.lib.VKRT_neuralCacheGetWeights(self.h, fptr(out), n), "VKRT_neuralCacheGetWeights")
+        return out
+
+    def nrc_set_weights(self, w: np.ndarray):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        _ok(self.lib.VKRT_neuralCacheSetWeights(self.h, fptr(w), w.size), "VKRT_neuralCacheSetWeights")
+
+    def nrc_infer(self, inputs: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(inputs, dtype=np.float32).reshape(-1, 11)
+        out = np.zeros((len(x), 3), np.float32)
+        _ok(self.lib.VKRT_neuralCacheInfer(self.h, fptr(x), len(x), fptr(out)), "VKRT_neuralCacheInfer")
+        return out
+
+    def nrc_train(self, inputs, targets, steps: int, batch: int, lr: float) -> float:
+        x = np.ascontiguousarray(inputs, dtype=np.float32).reshape(-1, 11)
+        t = np.ascontiguousarray(targets, dtype=np.float32).reshape(-1, 3)
+        loss = np.zeros(1, np.float32)
+        _ok(self.lib.VKRT_neuralCacheTrain(self.h, fptr(x), fptr(t), len(x), steps, batch, lr, fptr(loss)),
+            "VKRT_neuralCacheTrain")
+        return float(loss[0])
+
+    def nrc_gradient(self, inputs, targets):
+        x = np.ascontiguousarray(inputs, dtype=np.float32).reshape(-1, 11)
+        t = np.ascontiguousarray(targets, dtype=np.float32).reshape(-1, 3)
+        g = np.zeros(self.lib.VKRT_neuralCacheParameterCount(), np.float32)
+        loss = np.zeros(1, np.float32)
+        _ok(self.lib.VKRT_neuralCacheGradient(self.h, fptr(x), fptr(t), len(x), fptr(g), fptr(loss)),
+            "VKRT_neuralCacheGradient")
+        return g, float(loss[0])
 
     def synchronize(self):
         _ok(self.lib.VKRT_deviceSynchronize(self.h), "VKRT_deviceSynchronize")

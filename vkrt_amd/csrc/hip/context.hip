@@ -67,6 +67,7 @@ extern "C" void mrt_destroy(mrt_context* ctx) {
     if (!ctx) return;
     hipSetDevice(ctx->device);
     hipStreamSynchronize(ctx->stream);
+    mrt::nrcRelease(ctx);
     ctx->vertices.release();
     ctx->indices.release();
     ctx->worldTris.release();

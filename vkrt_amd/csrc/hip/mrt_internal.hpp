@@ -55,9 +55,18 @@ struct PathBuffers {
     size_t capacity = 0;
 };
 
+// neural radiance cache (nrc.hip)
+bool nrcEnsure(mrt_context* ctx, uint32_t seed);
+void nrcRelease(mrt_context* ctx);
+bool nrcInferQueries(mrt_context* ctx, const float4* queries, const uint32_t* count, const float4* beta, float4* rad);
+bool nrcTrainSteps(mrt_context* ctx, const float4* in, const float4* target, uint32_t n, uint32_t steps,
+                   uint32_t batch, float lr);
+float nrcReadLoss(mrt_context* ctx);
+
 }  // namespace mrt
 
 struct mrt_context {
+    void* nrc = nullptr;  // neural radiance cache state (nrc.hip), created on first use
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t evBegin = nullptr, evEnd = nullptr;

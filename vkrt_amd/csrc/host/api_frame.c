@@ -39,6 +39,7 @@ VKRT_Result VKRT_create(VKRT** outVkrt) {
     vkrtSyncCameraMatrices(vkrt);
     vkrt->accumulationNeedsReset = 1;
     vkrt->rngPolicy = VKRT_RNG_HASH;
+    VKRT_defaultNeuralCacheSettings(&vkrt->nrc);
     vkrt->renderStatus.renderPhase = VKRT_RENDER_PHASE_INACTIVE;
     *outVkrt = vkrt;
     return VKRT_SUCCESS;

@@ -113,6 +113,8 @@ struct VKRT {
     mrt_context* device;
     VKRT_SystemInfo systemInfo;
     VKRT_TraceStats stats;
+    VKRT_NeuralCacheSettings nrc;
+    VKRT_NeuralCacheStats nrcStats;
 };
 
 /* scene preparation (scene_prep.c) */

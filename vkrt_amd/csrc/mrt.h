@@ -114,6 +114,20 @@ int mrt_clear_accumulation(mrt_context* ctx);
 int mrt_read_accumulation(mrt_context* ctx, float* out, size_t floatCount);
 int mrt_read_output(mrt_context* ctx, uint16_t* out, size_t count);
 
+/* Neural radiance cache network (nrc.hip). Inputs are 11 floats per sample in
+ * [0, 1] (position 3, direction theta/pi phi/2pi, normal likewise, roughness,
+ * albedo 3); outputs 3 floats. Weights are the natural fp32 parameters:
+ * W0 [64][192], W1..W4 [64][64], W5 [16][64] (rows 3..15 zero). */
+int mrt_nrc_parameter_count(void);
+int mrt_nrc_reset(mrt_context* ctx, uint32_t seed);
+int mrt_nrc_get_weights(mrt_context* ctx, float* out, size_t count);
+int mrt_nrc_set_weights(mrt_context* ctx, const float* weights, size_t count);
+int mrt_nrc_infer(mrt_context* ctx, const float* inputs, uint32_t n, float* outputs);
+int mrt_nrc_train(mrt_context* ctx, const float* inputs, const float* targets, uint32_t n, uint32_t steps,
+                  uint32_t batch, float learningRate, float* lossOut);
+int mrt_nrc_gradient(mrt_context* ctx, const float* inputs, const float* targets, uint32_t n, float* grad,
+                     float* lossOut);
+
 /* Path-replay backpropagation of one frame (RGB mode): gradients of
  * sum_px dImage[px] . I[px], where I is the frame the same parameters and
  * frame number render, with respect to material base colour, roughness and
