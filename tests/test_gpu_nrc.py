@@ -136,3 +136,77 @@ def test_training_fits_a_smooth_function(renderer):
     print(f"relative L2 loss {first:.3e} -> {last:.3e}; mse after training {mse:.2e} (target variance {var:.2e})")
     assert last < 0.05 * first
     assert mse < 0.4 * var
+
+
+def cornell(size, spp, depth, nee=True, seed=1):
+    from vkrt_amd import scenes
+
+    r = V.Renderer(size, size)
+    scenes.load(r, scenes.cornell_box())
+    r.configure(spp=spp, max_depth=depth, min_depth=4, mode=V.RENDER_MODE_RGB, rng=V.RNG_PCG32, seed=seed, nee=nee)
+    r.start(size, size, 0)
+    return r
+
+
+def test_zero_cache_equals_the_truncated_tracer():
+    """A cache that predicts 0 and NEE off: an image path gathers exactly the
+    emission of its first prefix+1 vertices, which is the plain tracer with depth
+    prefix+1; the training paths use streams of their own and leave it unchanged."""
+    size, spp = 48, 8
+    a = cornell(size, spp, 8, nee=False)
+    a.nrc_set_weights(np.zeros(PARAMS, np.float32))
+    a.set_neural_cache(True, prefixBounces=2, trainingPaths=4096, train=0)
+    a.start(size, size, 0)
+    a.draw()
+    img = a.accumulation()
+    st = a.neural_cache_stats()
+    a.close()
+    b = cornell(size, spp, 3, nee=False)
+    b.draw()
+    ref = b.accumulation()
+    b.close()
+    print(f"queries {st.queries}, training records {st.trainingRecords}")
+    assert st.queries > 0.5 * size * size * spp and st.trainingRecords > 4096
+    assert np.array_equal(img, ref)
+
+
+def test_cache_output_is_added_with_the_path_throughput():
+    """A cache that predicts the constant c adds c times the throughput of every
+    query; with c = 1 and c = 2 the two images differ from the zero-cache image by
+    amounts in the ratio 1 : 2."""
+    size, spp = 32, 8
+    w = np.zeros(PARAMS, np.float32)
+    w[:OFF_HIDDEN].reshape(64, 192)[0, 172] = 1.0  # neuron 0 of the first layer = the constant input 1
+    for k in range(4):
+        w[OFF_HIDDEN + k * 4096:OFF_HIDDEN + (k + 1) * 4096].reshape(64, 64)[0, 0] = 1.0
+    imgs = []
+    for c in (0.0, 1.0, 2.0):
+        w[OFF_OUT:OFF_OUT + 192].reshape(3, 64)[:, 0] = c
+        r = cornell(size, spp, 8, nee=True)
+        r.nrc_set_weights(w)
+        r.set_neural_cache(True, trainingPaths=1024, train=0)
+        r.start(size, size, 0)
+        r.draw()
+        imgs.append(r.accumulation()[..., :3].astype(np.float64))
+        r.close()
+    d1, d2 = imgs[1] - imgs[0], imgs[2] - imgs[0]
+    assert d1.sum() > 0
+    assert np.allclose(d2, 2 * d1, rtol=1e-4, atol=1e-6)
+
+
+def test_online_training_lowers_the_loss():
+    size = 64
+    r = cornell(size, 1, 8)
+    r.nrc_reset(1)
+    r.set_neural_cache(True, trainingPaths=16384, trainingSteps=4, batchSize=4096, learningRate=1e-2)
+    losses = []
+    for _ in range(40):
+        r.draw()
+        losses.append(r.neural_cache_stats().lastLoss)
+    st = r.neural_cache_stats()
+    img = r.accumulation()
+    r.close()
+    print(f"loss {losses[0]:.3f} -> {np.mean(losses[-5:]):.3f}; queries {st.queries}, records {st.trainingRecords}, "
+          f"inference {st.inferMs:.3f} ms, training {st.trainMs:.3f} ms")
+    assert np.isfinite(img).all()
+    assert np.mean(losses[-5:]) < 0.5 * losses[0]

@@ -104,6 +104,9 @@ extern "C" void mrt_destroy(mrt_context* ctx) {
     b.tpp.release();
     b.ptpp.release();
     b.pbtp.release();
+    b.nrcQuery.release();
+    b.nrcRecord.release();
+    b.nrcTarget.release();
     b.lam.release();
     b.misc.release();
     b.sigma.release();

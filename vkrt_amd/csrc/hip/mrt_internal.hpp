@@ -52,15 +52,17 @@ struct PathBuffers {
     DeviceArray<float4> shadow[4];   // origin+path, direction+tmax, contribution if visible, otherwise
     DeviceArray<float4> shadowEnv[3];  // environment NEE: origin+path, direction+tmax, contribution if visible
     DeviceArray<float4> sampleRad;
+    DeviceArray<float4> nrcQuery, nrcRecord, nrcTarget;  // neural cache queries, training records, targets
     size_t capacity = 0;
 };
 
 // neural radiance cache (nrc.hip)
 bool nrcEnsure(mrt_context* ctx, uint32_t seed);
 void nrcRelease(mrt_context* ctx);
-bool nrcInferQueries(mrt_context* ctx, const float4* queries, const uint32_t* count, const float4* beta, float4* rad);
-bool nrcTrainSteps(mrt_context* ctx, const float4* in, const float4* target, uint32_t n, uint32_t steps,
-                   uint32_t batch, float lr);
+bool nrcInferQueries(mrt_context* ctx, const float4* queries, uint32_t stride, const uint32_t* count,
+                     uint32_t capacity, float4* rad);
+bool nrcTrainSteps(mrt_context* ctx, const float4* in, uint32_t stride, const float4* target, const uint32_t* count,
+                   uint32_t capacity, uint32_t steps, uint32_t batch, float lr);
 float nrcReadLoss(mrt_context* ctx);
 
 }  // namespace mrt

@@ -45,7 +45,6 @@ constexpr int kNrcParams = kNrcOffOut + kNrcOutRows * kNrcWidth;  // 29696
 constexpr int kNrcBwdOut = 4 * kNrcWidth * kNrcWidth;  // 16384
 constexpr int kNrcBwdParams = kNrcBwdOut + kNrcWidth * kNrcOutRows;  // 17408
 constexpr int kNrcBlock = 256;
-constexpr float kNrcPi = 3.14159265358979323846f;
 
 // hidden neuron held in K-slot s of a 64-wide hidden input (see the header)
 __host__ __device__ inline int nrcNeuronOfSlot(int s) {
@@ -65,40 +64,63 @@ __host__ __device__ inline int nrcSlotOfNeuron(int n) {
     return 32 * kb + 8 * g + t;
 }
 
-// Encoded feature f of one sample. v[0..6] periodic, v[7..10] raw.
-__device__ inline float nrcFeature(const float* v, int f) {
-    if (f < 7 * 2 * kNrcFreqs) {
-        int d = f / (2 * kNrcFreqs), r = f % (2 * kNrcFreqs);
-        int freq = r >> 1;
-        // sin(2^freq pi v) = sin(2 pi x) with x = frac(2^(freq-1) v); the scaling is exact
-        float x = v[d] * (freq == 0 ? 0.5f : (float)(1u << (freq - 1)));
-        x = x - floorf(x);
-        float a = 2.0f * kNrcPi * x;
-        return (r & 1) ? cosf(a) : sinf(a);
-    }
-    if (f < 7 * 2 * kNrcFreqs + 4) return v[7 + (f - 7 * 2 * kNrcFreqs)];
-    if (f == 7 * 2 * kNrcFreqs + 4) return 1.0f;
-    return 0.0f;
-}
-
-__device__ inline void nrcLoadRaw(const float4* in, uint32_t idx, bool valid, float* v) {
+__device__ inline void nrcLoadRaw(const float4* in, uint32_t stride, uint32_t idx, bool valid, float* v) {
     if (!valid) {
         for (int k = 0; k < 11; k++) v[k] = 0.0f;
         return;
     }
-    float4 a = in[3 * (size_t)idx], b = in[3 * (size_t)idx + 1], c = in[3 * (size_t)idx + 2];
+    const float4* r = in + (size_t)stride * idx;
+    float4 a = r[0], b = r[1], c = r[2];
     v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
     v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
     v[8] = c.x; v[9] = c.y; v[10] = c.z;
 }
 
-// B fragments of the encoded input of sample l%16: K-block kb holds features 32kb + 8g + t.
+// v[d] for a lane-dependent d < 7 without indexing a register array
+__device__ inline float nrcPick(const float* v, int d) {
+    float r = v[0];
+    r = d == 1 ? v[1] : r;
+    r = d == 2 ? v[2] : r;
+    r = d == 3 ? v[3] : r;
+    r = d == 4 ? v[4] : r;
+    r = d == 5 ? v[5] : r;
+    r = d == 6 ? v[6] : r;
+    return r;
+}
+
+// B fragments of the encoded input of sample l%16: K-block kb holds features
+// 32kb + 8g + t. Feature 24d + 2f (+1) is sin (cos) of 2^f pi v[d], computed as
+// the hardware sine / cosine of frac(2^(f-1) v[d]) revolutions (the scaling is
+// exact); 168..171 are roughness and albedo, 172 the constant 1, the rest 0.
 __device__ inline void nrcEncode(const float* v, int g, half8 x[6]) {
 #pragma unroll
     for (int kb = 0; kb < 6; kb++) {
+        const int f0 = 32 * kb + 8 * g;
+        half8 h = {0, 0, 0, 0, 0, 0, 0, 0};
+        if (f0 < 7 * 2 * kNrcFreqs) {
+            const float vd = nrcPick(v, f0 / (2 * kNrcFreqs));
+            const int freq0 = (f0 % (2 * kNrcFreqs)) >> 1;
 #pragma unroll
-        for (int t = 0; t < 8; t++) x[kb][t] = (_Float16)nrcFeature(v, 32 * kb + 8 * g + t);
+            for (int t = 0; t < 4; t++) {
+                float u = ldexpf(vd, freq0 + t - 1);
+                u -= floorf(u);
+                h[2 * t] = (_Float16)__builtin_amdgcn_sinf(u);
+                h[2 * t + 1] = (_Float16)__builtin_amdgcn_cosf(u);
+            }
+        } else if (f0 == 7 * 2 * kNrcFreqs) {
+            h = half8{(_Float16)v[7], (_Float16)v[8], (_Float16)v[9], (_Float16)v[10], (_Float16)1.0f, 0, 0, 0};
+        }
+        x[kb] = h;
     }
+}
+
+// A zero the compiler cannot see through. Added to the weight pointers inside a
+// sample loop it keeps the weight fragment loads in the loop instead of hoisted
+// out of it, where all 58 fragments would occupy 232 registers.
+__device__ inline int nrcOpaqueZero() {
+    int z;
+    asm volatile("s_mov_b32 %0, 0" : "=s"(z));
+    return z;
 }
 
 __device__ inline half8 ldHalf8(const _Float16* p) { return *reinterpret_cast<const half8*>(p); }
@@ -157,17 +179,19 @@ __device__ inline float4v nrcForward(const _Float16* w, const half8 x[6], half8 
     return out;  // lanes 0..15 hold outputs 0..3 of sample l (rows 0..2 used)
 }
 
-// Inference over n samples. in: 3 float4 per sample (raw inputs; in2.w = path
-// index bits when rad != null). Writes raw outputs to out (when non-null) and/or
-// adds beta * max(output, 0) to rad[path].
+// Inference over n samples (n = min(*count, nFixed) when count is given).
+// in: `stride` float4 per sample, the first three the raw inputs (in[2].w = path
+// index bits when rad != null), in[3] the throughput when rad != null. Writes
+// raw outputs to out (when non-null) and/or adds throughput * max(output, 0) to
+// rad[path].
 __global__ __launch_bounds__(kNrcBlock) void kNrcInfer(const _Float16* __restrict__ wfwd, const float4* __restrict__ in,
-                                                      const uint32_t* count, uint32_t nFixed, const float4* beta,
+                                                      uint32_t stride, const uint32_t* count, uint32_t nFixed,
                                                       float4* out, float4* rad) {
     __shared__ __align__(16) _Float16 w[kNrcParams];
     for (int k = threadIdx.x; k < kNrcParams / 8; k += kNrcBlock)
         reinterpret_cast<half8*>(w)[k] = reinterpret_cast<const half8*>(wfwd)[k];
     __syncthreads();
-    const uint32_t n = count ? *count : nFixed;
+    const uint32_t n = count ? min(*count, nFixed) : nFixed;
     const int l = threadIdx.x & 63, g = l >> 4;
     const uint32_t wave = blockIdx.x * (kNrcBlock / 64) + (threadIdx.x >> 6);
     const uint32_t waves = gridDim.x * (kNrcBlock / 64);
@@ -175,20 +199,21 @@ __global__ __launch_bounds__(kNrcBlock) void kNrcInfer(const _Float16* __restric
         const uint32_t idx = base + (uint32_t)(l & 15);
         const bool valid = idx < n;
         float v[11];
-        nrcLoadRaw(in, idx, valid, v);
+        nrcLoadRaw(in, stride, idx, valid, v);
         half8 x[6];
         nrcEncode(v, g, x);
-        float4v o = nrcForward<false>(w, x, nullptr);
+        float4v o = nrcForward<false>(w + nrcOpaqueZero(), x, nullptr);
         if (g == 0 && valid) {
             if (out) out[idx] = make_float4(o[0], o[1], o[2], 0.0f);
             if (rad) {
-                const uint32_t path = __float_as_uint(in[3 * (size_t)idx + 2].w);
-                float4 b = beta[idx];
-                float4 r = rad[path];
-                r.x += b.x * fmaxf(o[0], 0.0f);
-                r.y += b.y * fmaxf(o[1], 0.0f);
-                r.z += b.z * fmaxf(o[2], 0.0f);
-                rad[path] = r;
+                const float4* rec = in + (size_t)stride * idx;
+                const uint32_t path = __float_as_uint(rec[2].w);
+                const float4 b = rec[3];
+                float4 acc = rad[path];
+                acc.x += b.x * fmaxf(o[0], 0.0f);
+                acc.y += b.y * fmaxf(o[1], 0.0f);
+                acc.z += b.z * fmaxf(o[2], 0.0f);
+                rad[path] = acc;
             }
         }
     }
@@ -207,9 +232,11 @@ __device__ inline void nrcWaitLds() {
 // which is added to the global fp32 gradient once per block.
 __global__ __launch_bounds__(kNrcBlock) void kNrcTrain(const _Float16* __restrict__ wfwd,
                                                       const _Float16* __restrict__ wbwd, const float4* __restrict__ in,
-                                                      const float4* __restrict__ target, uint32_t n, uint32_t batch,
+                                                      uint32_t stride, const float4* __restrict__ target,
+                                                      const uint32_t* count, uint32_t nFixed, uint32_t batch,
                                                       uint32_t batchOffset, uint32_t shuffle, float* grad,
                                                       float* lossSum) {
+    const uint32_t n = count ? min(*count, nFixed) : nFixed;
     extern __shared__ __align__(16) float nrcShared[];
     float* gl = nrcShared;                                                    // kNrcParams floats
     _Float16* tabBase = reinterpret_cast<_Float16*>(nrcShared + kNrcParams);  // per wave: 64x16 + 192x16 halfs
@@ -229,16 +256,19 @@ __global__ __launch_bounds__(kNrcBlock) void kNrcTrain(const _Float16* __restric
                                     : (shuffle ? (uint32_t)(((uint64_t)(batchOffset + bi) * 2654435761ull) % n)
                                                : bi % n);
         float v[11];
-        nrcLoadRaw(in, idx, valid, v);
+        nrcLoadRaw(in, stride, idx, valid, v);
         half8 x[6];
         nrcEncode(v, g, x);
         half8 h[kNrcHidden][2];
-        float4v o = nrcForward<true>(wfwd, x, h);
+        const int z = nrcOpaqueZero();
+        const _Float16* wf = wfwd + z;
+        const _Float16* wb = wbwd + z;
+        float4v o = nrcForward<true>(wf, x, h);
 
         // loss gradient, D layout (rows 0..2 of lane group 0)
         float4v dOut = {0.0f, 0.0f, 0.0f, 0.0f};
-        if (g == 0 && valid) {
-            float4 tg = target[idx];
+        const float4 tg = valid ? target[idx] : make_float4(0.0f, 0.0f, 0.0f, -1.0f);
+        if (g == 0 && tg.w >= 0.0f) {
             float tv[3] = {tg.x, tg.y, tg.z};
 #pragma unroll
             for (int c = 0; c < 3; c++) {
@@ -272,7 +302,7 @@ __global__ __launch_bounds__(kNrcBlock) void kNrcTrain(const _Float16* __restric
         float4v dl[4];
 #pragma unroll
         for (int mb = 0; mb < 4; mb++) {
-            half4 a = ldHalf4(wbwd + kNrcBwdOut + (mb * 16 + row) * kNrcOutRows + 4 * g);
+            half4 a = ldHalf4(wb + kNrcBwdOut + (mb * 16 + row) * kNrcOutRows + 4 * g);
             float4v acc = __builtin_amdgcn_mfma_f32_16x16x16f16(a, dOutH, float4v{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
 #pragma unroll
             for (int t = 0; t < 4; t++) acc[t] = (float)h[4][mb >> 1][(mb & 1) * 4 + t] > 0.0f ? acc[t] : 0.0f;
@@ -305,10 +335,10 @@ __global__ __launch_bounds__(kNrcBlock) void kNrcTrain(const _Float16* __restric
                 }
             }
             const half8 bd[2] = {nrcPack(dl[0], dl[1]), nrcPack(dl[2], dl[3])};
-            const _Float16* wb = wbwd + (layer - 1) * kNrcWidth * kNrcWidth;
+            const _Float16* wbl = wb + (layer - 1) * kNrcWidth * kNrcWidth;
 #pragma unroll
             for (int mb = 0; mb < 4; mb++) {
-                const _Float16* wr = wb + (mb * 16 + row) * kNrcWidth + 8 * g;
+                const _Float16* wr = wbl + (mb * 16 + row) * kNrcWidth + 8 * g;
                 float4v acc = {0.0f, 0.0f, 0.0f, 0.0f};
                 acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ldHalf8(wr), bd[0], acc, 0, 0, 0);
                 acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ldHalf8(wr + 32), bd[1], acc, 0, 0, 0);
@@ -488,22 +518,26 @@ void nrcRelease(mrt_context* ctx) {
 }
 
 // rad[path] += beta * cache(query) for *count queries (device counter)
-bool nrcInferQueries(mrt_context* ctx, const float4* queries, const uint32_t* count, const float4* beta, float4* rad) {
+// rad[path] += throughput * cache(query) for min(*count, capacity) queries of `stride` float4
+bool nrcInferQueries(mrt_context* ctx, const float4* queries, uint32_t stride, const uint32_t* count,
+                     uint32_t capacity, float4* rad) {
     NrcState* st = nrcState(ctx);
-    hipLaunchKernelGGL(kNrcInfer, dim3(st->inferGrid), dim3(kNrcBlock), 0, ctx->stream, st->wfwd.ptr, queries, count,
-                       0u, beta, nullptr, rad);
+    hipLaunchKernelGGL(kNrcInfer, dim3(st->inferGrid), dim3(kNrcBlock), 0, ctx->stream, st->wfwd.ptr, queries, stride,
+                       count, capacity, nullptr, rad);
     return check(hipGetLastError(), "nrc inference");
 }
 
-// `steps` Adam steps of `batch` records drawn from the n (host-known) training records
-bool nrcTrainSteps(mrt_context* ctx, const float4* in, const float4* target, uint32_t n, uint32_t steps,
-                   uint32_t batch, float lr) {
+// `steps` Adam steps of `batch` records drawn from the min(*count, capacity)
+// training records (count may be null: capacity records)
+bool nrcTrainSteps(mrt_context* ctx, const float4* in, uint32_t stride, const float4* target, const uint32_t* count,
+                   uint32_t capacity, uint32_t steps, uint32_t batch, float lr) {
     NrcState* st = nrcState(ctx);
-    if (n == 0 || steps == 0 || batch == 0) return true;
+    if (capacity == 0 || steps == 0 || batch == 0) return true;
     hipMemsetAsync(st->loss.ptr, 0, sizeof(float), ctx->stream);
     for (uint32_t s = 0; s < steps; s++) {
         hipLaunchKernelGGL(kNrcTrain, dim3(st->trainGrid), dim3(kNrcBlock), nrcTrainLds(), ctx->stream, st->wfwd.ptr,
-                           st->wbwd.ptr, in, target, n, batch, st->step * batch, 1u, st->grad.ptr, st->loss.ptr);
+                           st->wbwd.ptr, in, stride, target, count, capacity, batch, st->step * batch, 1u,
+                           st->grad.ptr, st->loss.ptr);
         st->step++;
         const float b1 = 0.9f, b2 = 0.99f;
         const float c1 = 1.0f - powf(b1, (float)st->step), c2 = 1.0f - powf(b2, (float)st->step);
@@ -568,7 +602,7 @@ extern "C" int mrt_nrc_infer(mrt_context* ctx, const float* inputs, uint32_t n, 
     }
     if (!st->scratchIn.upload(in.data(), in.size(), ctx->stream) || !st->scratchOut.reserve(n)) return -3;
     hipLaunchKernelGGL(kNrcInfer, dim3(st->inferGrid), dim3(kNrcBlock), 0, ctx->stream, st->wfwd.ptr,
-                       st->scratchIn.ptr, nullptr, n, nullptr, st->scratchOut.ptr, nullptr);
+                       st->scratchIn.ptr, 3u, nullptr, n, st->scratchOut.ptr, nullptr);
     if (!check(hipGetLastError(), "nrc inference")) return -3;
     std::vector<float4> out(n);
     if (!check(hipMemcpyAsync(out.data(), st->scratchOut.ptr, n * sizeof(float4), hipMemcpyDeviceToHost, ctx->stream),
@@ -599,7 +633,8 @@ extern "C" int mrt_nrc_train(mrt_context* ctx, const float* inputs, const float*
     if (!st->scratchIn.upload(in.data(), in.size(), ctx->stream) ||
         !st->scratchTarget.upload(tg.data(), tg.size(), ctx->stream))
         return -3;
-    if (!nrcTrainSteps(ctx, st->scratchIn.ptr, st->scratchTarget.ptr, n, steps, batch, learningRate)) return -3;
+    if (!nrcTrainSteps(ctx, st->scratchIn.ptr, 3u, st->scratchTarget.ptr, nullptr, n, steps, batch, learningRate))
+        return -3;
     float loss = nrcReadLoss(ctx);
     if (lossOut) *lossOut = loss;
     return loss < 0.0f ? -3 : 0;
@@ -626,8 +661,8 @@ extern "C" int mrt_nrc_gradient(mrt_context* ctx, const float* inputs, const flo
     hipMemsetAsync(st->loss.ptr, 0, sizeof(float), ctx->stream);
     hipMemsetAsync(st->grad.ptr, 0, kNrcParams * sizeof(float), ctx->stream);
     hipLaunchKernelGGL(kNrcTrain, dim3(st->trainGrid), dim3(kNrcBlock), nrcTrainLds(), ctx->stream, st->wfwd.ptr,
-                       st->wbwd.ptr, st->scratchIn.ptr, st->scratchTarget.ptr, n, n, 0u, 0u, st->grad.ptr,
-                       st->loss.ptr);
+                       st->wbwd.ptr, st->scratchIn.ptr, 3u, st->scratchTarget.ptr, nullptr, n, n, 0u, 0u,
+                       st->grad.ptr, st->loss.ptr);
     if (!check(hipGetLastError(), "nrc gradient")) return -3;
     if (!check(hipMemcpyAsync(grad, st->grad.ptr, kNrcParams * sizeof(float), hipMemcpyDeviceToHost, ctx->stream),
                "nrc gradient D2H"))

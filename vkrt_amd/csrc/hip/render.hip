@@ -43,7 +43,8 @@ constexpr uint64_t kMaxChunk = 1ull << 27;
 constexpr int kBlock = 256;
 
 // counters[] layout
-constexpr int kCntQueue0 = 0, kCntQueue1 = 1, kCntShadow = 2, kCntShadowEnv = 3, kCntDropped = 5,
+constexpr int kCntQueue0 = 0, kCntQueue1 = 1, kCntShadow = 2, kCntShadowEnv = 3, kCntNrcQuery = 4, kCntDropped = 5,
+              kCntNrcRecord = 6, kCntNrcTrain = 7,
               kCntStats = 8;  // 64-bit stats at counters+8: [0] ext rays, [1] shadow rays, [2] nodes, [3] triangles
 // Work claims of the persistent traversal kernels. The batch space of a queue
 // is split into kFetchParts interleaved partitions, each with its own counter
@@ -71,6 +72,31 @@ struct FrameParams {
     uint32_t traceShadowAlways;
     uint32_t envNee;  // environment importance sampling for NEE
     Spec spec;
+    // neural radiance cache (RGB): paths [0, nrcRenderPaths) of the chunk render the
+    // image and query the cache after nrcPrefix bounces; the rest are training paths
+    uint32_t nrc, nrcPrefix, nrcRenderPaths;
+    float nrcBoxMin[3], nrcBoxInv[3];
+};
+
+// Neural radiance cache records. A query is 4 float4: the 11 network inputs in
+// three (the third's w holds the path index) and the throughput. A training
+// record is 6 float4: the inputs (w = path index), the throughput arriving at
+// the vertex, the path radiance before the vertex and the emission it adds.
+struct NrcView {
+    float4* query;
+    float4* record;
+    float4* trainIn;
+    float4* trainTarget;
+    uint32_t queryCapacity, recordCapacity;
+};
+constexpr int kNrcQueryWords = 4, kNrcRecordWords = 6;
+
+// What one chunk does with the neural cache after its depth loop.
+struct NrcRun {
+    NrcView view;
+    bool train;
+    uint32_t steps, batch;
+    float learningRate;
 };
 
 struct PathView {
@@ -101,8 +127,9 @@ DEV uint4 storeRng(const Rng& r) {
     return make_uint4((uint32_t)r.state, (uint32_t)(r.state >> 32), (uint32_t)r.inc, (uint32_t)(r.inc >> 32));
 }
 
-// Appends this lane's item if pred; one atomic per wavefront. Every lane of
-// the wavefront must call it. Returns the slot, or 0xFFFFFFFF.
+// Appends this lane's item if pred; one atomic per wavefront. Every active lane
+// that may append must call it (lanes that have left the calling branch do
+// not take part). Returns the slot, or 0xFFFFFFFF.
 DEV uint32_t waveAppend(bool pred, uint32_t* counter) {
     unsigned long long mask = __ballot(pred);
     if (mask == 0ull) return 0xFFFFFFFFu;
@@ -324,10 +351,21 @@ __global__ __launch_bounds__(kBlock) void kGenerate(FrameParams fp, PathView pv,
     uint64_t p = fp.pathBegin + i;
     uint32_t s = (uint32_t)(p / fp.numPixels);
     uint32_t k = (uint32_t)(p % fp.numPixels);
+    const bool training = fp.nrc && i >= fp.nrcRenderPaths;
+    if (training) {
+        // a cache training path through a pseudo-random pixel, on a stream of its own
+        uint32_t h = (i - fp.nrcRenderPaths) * 0x9E3779B9u ^ (fp.sd.frameNumber * 0x85EBCA6Bu + 0x165667B1u);
+        h ^= h >> 16;
+        h *= 0x7FEB352Du;
+        h ^= h >> 15;
+        h *= 0x846CA68Bu;
+        h ^= h >> 16;
+        k = h % fp.numPixels;
+    }
     int px = (int)(fp.sd.viewportRect[0] + k % fp.vpW);
     int py = (int)(fp.sd.viewportRect[1] + k / fp.vpW);
     float4 prev = accum[(size_t)py * fp.accumWidth + px];
-    uint32_t sampleAbs = (uint32_t)(prev.w + 0.5f) + s;
+    uint32_t sampleAbs = training ? 0x80000000u + (i - fp.nrcRenderPaths) : (uint32_t)(prev.w + 0.5f) + s;
     Rng rng = rngInit(fp.rngPolicy, fp.rngSeed, px, py, fp.sd.frameNumber, sampleAbs);
     float jx = rnd(rng);
     float jy = rnd(rng);
@@ -515,6 +553,45 @@ __global__ __launch_bounds__(kTraceBlock) void kShadow(SceneView sv, PathView pv
     if (st.dropped) atomicAdd(&counters[kCntDropped], st.dropped);
 }
 
+// The 11 inputs of the neural cache at a surface vertex, in [0, 1]: position in
+// the scene box, outgoing direction and shading normal as (theta/pi, phi/2pi),
+// roughness and albedo; f[2].w carries the path index.
+DEV void nrcInputs(const FrameParams& fp, const Surface& sf, const BMat& mat, F3 d, uint32_t q, float4* f) {
+    const float invPi = 0.31830988618379067f, inv2Pi = 0.15915494309189535f;
+    F3 pn = F3{sat((sf.p.x - fp.nrcBoxMin[0]) * fp.nrcBoxInv[0]), sat((sf.p.y - fp.nrcBoxMin[1]) * fp.nrcBoxInv[1]),
+               sat((sf.p.z - fp.nrcBoxMin[2]) * fp.nrcBoxInv[2])};
+    F3 wo = -d, n = sf.basis.n;
+    float dt = acosf(clampf(wo.z, -1.0f, 1.0f)) * invPi, dp = atan2f(wo.y, wo.x) * inv2Pi;
+    float nt = acosf(clampf(n.z, -1.0f, 1.0f)) * invPi, np = atan2f(n.y, n.x) * inv2Pi;
+    if (dp < 0.0f) dp += 1.0f;
+    if (np < 0.0f) np += 1.0f;
+    f[0] = make_float4(pn.x, pn.y, pn.z, dt);
+    f[1] = make_float4(dp, nt, np, mat.roughness);
+    f[2] = make_float4(mat.baseColor.x, mat.baseColor.y, mat.baseColor.z, __uint_as_float(q));
+}
+
+// Training targets of the cache: the radiance the rest of the path carried back
+// to each recorded vertex, divided by the throughput that arrived there, less
+// the vertex's own emission. Records whose throughput is too small to divide
+// by are marked with w = -1.
+__global__ __launch_bounds__(kBlock) void kNrcTargets(const float4* record, const uint32_t* count, uint32_t capacity,
+                                                     const float4* rad, float4* target) {
+    const uint32_t r = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t n = min(*count, capacity);
+    if (r >= n) return;
+    const float4* rec = record + (size_t)r * kNrcRecordWords;
+    const uint32_t path = __float_as_uint(rec[2].w);
+    const float4 L = rad[path], b = rec[3], R = rec[4], E = rec[5];
+    const float lc[3] = {L.x, L.y, L.z}, bc[3] = {b.x, b.y, b.z}, rc[3] = {R.x, R.y, R.z}, ec[3] = {E.x, E.y, E.z};
+    float t[3];
+    bool ok = true;
+    for (int c = 0; c < 3; c++) {
+        ok = ok && bc[c] > 1e-4f;
+        t[c] = ok ? fminf(fmaxf((lc[c] - rc[c]) / bc[c] - ec[c], 0.0f), 1000.0f) : 0.0f;
+    }
+    target[r] = make_float4(t[0], t[1], t[2], ok ? 0.0f : -1.0f);
+}
+
 // One path vertex for every ray of the current queue, at three waves per SIMD
 // (at most 168 registers). Every variant must fit that budget without spilling
 // registers to scratch: a hero build that spilled about 60 registers gave wrong
@@ -522,10 +599,10 @@ __global__ __launch_bounds__(kTraceBlock) void kShadow(SceneView sv, PathView pv
 // bit. What keeps the hero variant inside the budget: the rgb2spec resolution is
 // a compile-time constant, medium coefficients and the scalar-wavelength state
 // are read only where used, and shadow rays are queued before BSDF sampling.
-template <int MODE>
+template <int MODE, bool NRC = false>
 __global__ __launch_bounds__(kBlock, 3) void kShade(FrameParams fp, SceneView sv, PathView pv, QueueView in,
                                                  const float4* hits, QueueView out, ShadowView sh, ShadowView she,
-                                                 uint32_t* counters, int inSlot, int outSlot) {
+                                                 uint32_t* counters, int inSlot, int outSlot, NrcView nv) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t n = counters[inSlot];
     bool cont = false, shadowEmit = false, envEmit = false;
@@ -617,6 +694,9 @@ __global__ __launch_bounds__(kBlock, 3) void kShade(FrameParams fp, SceneView sv
         const float stateLambda = MODE == kModeRgb ? 0.0f : (MODE == kModeSingle ? lam.x : (heroActive ? lam.x : misc.z));
         BState st = makeState(mat, wo, sf.frontFace, stateLambda, MODE == kModeRgb ? 0u : 1u);
         const bool curNee = !refractive;
+        // neural cache: training paths record every vertex; image paths stop at the query vertex
+        const bool nrcTrain = NRC && q >= fp.nrcRenderPaths;
+        const bool nrcQuery = NRC && !nrcTrain && fp.depth == fp.nrcPrefix;
         Rng rng = loadRng(pv.rng[q], fp.rngPolicy);
         F4 tpp = heroActive ? ld4(pv.tpp[q]) : f4(0.0f);
 
@@ -624,7 +704,7 @@ __global__ __launch_bounds__(kBlock, 3) void kShade(FrameParams fp, SceneView sv
         // evaluated here assuming visibility; the shadow kernel picks it or drops it
         F4 nee = f4(0.0f);
         bool neeTried = false;
-        if (!fp.bsdfOnly && fp.neeEnabled && curNee && wo.z > 0.0f) {
+        if (!fp.bsdfOnly && fp.neeEnabled && curNee && wo.z > 0.0f && !nrcQuery) {
             LightSurfaceSample ls = sampleLightSurface(sv, sd, sf.p, rng);
             if (ls.valid) {
                 neeTried = true;
@@ -665,7 +745,7 @@ __global__ __launch_bounds__(kBlock, 3) void kShade(FrameParams fp, SceneView sv
 
         // environment NEE (an addition): drawn after the light NEE, traced by its own shadow pass
         bool envTried = false;
-        if (!fp.bsdfOnly && fp.envNee && curNee && !absorbing && wo.z > 0.0f) {
+        if (!fp.bsdfOnly && fp.envNee && curNee && !absorbing && wo.z > 0.0f && !nrcQuery) {
             envTried = true;
             EnvSample es = sampleEnvironmentDirection(sv, sd, rng);
             if (es.valid) {
@@ -734,6 +814,17 @@ __global__ __launch_bounds__(kBlock, 3) void kShade(FrameParams fp, SceneView sv
             }
         }
 
+        if (NRC && nrcTrain && wo.z > 0.0f) {
+            const uint32_t slot = waveAppend(true, &counters[kCntNrcRecord]);
+            if (slot < nv.recordCapacity) {
+                float4* r = nv.record + (size_t)slot * kNrcRecordWords;
+                nrcInputs(fp, sf, mat, d, q, r);
+                r[3] = make_float4(thr.x, thr.y, thr.z, 0.0f);
+                r[4] = pv.rad[q];
+                r[5] = hasEm ? make_float4(em.x, em.y, em.z, 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            }
+        }
+
         // radiance += throughput * contribution, now or once visibility is known
         F4 thrUse = (MODE == kModeHero && !heroActive) ? F4{misc.y, 0.0f, 0.0f, 0.0f} : thr;
         if (neeTried && (fp.traceShadowAlways || anyPos(nee))) {
@@ -782,6 +873,19 @@ __global__ __launch_bounds__(kBlock, 3) void kShade(FrameParams fp, SceneView sv
                 she.d[eslot] = make_float4(seD.x, seD.y, seD.z, kRayTMax);
                 she.a[eslot] = st4(seA);
             }
+        }
+
+        // the cache stands in for the rest of an image path from its query vertex on
+        if (NRC && nrcQuery) {
+            if (wo.z > 0.0f) {
+                const uint32_t slot = waveAppend(true, &counters[kCntNrcQuery]);
+                if (slot < nv.queryCapacity) {
+                    float4* r = nv.query + (size_t)slot * kNrcQueryWords;
+                    nrcInputs(fp, sf, mat, d, q, r);
+                    r[3] = make_float4(thr.x, thr.y, thr.z, 0.0f);
+                }
+            }
+            break;
         }
 
         // the last vertex of the loop never traces its continuation
@@ -1459,7 +1563,7 @@ static PathView pathView(mrt_context* ctx) {
 
 struct StageEvents {
     std::vector<hipEvent_t> pool;
-    std::vector<int> stage;  // per event pair: 0 extend, 1 shade, 2 shadow
+    std::vector<int> stage;  // per event pair: 0 extend, 1 shade, 2 shadow, 3 cache inference, 4 cache training
     size_t used = 0;
     bool begin(hipStream_t s, int which) {
         if (used + 2 > pool.size()) return false;
@@ -1483,7 +1587,8 @@ static void queueViews(mrt_context* ctx, QueueView qv[2]) {
 }
 
 template <int MODE>
-static bool launchChunk(mrt_context* ctx, FrameParams fp, StageEvents& ev, bool countTraversal) {
+static bool launchChunk(mrt_context* ctx, FrameParams fp, StageEvents& ev, bool countTraversal,
+                        const NrcRun* nrc = nullptr) {
     hipStream_t s = ctx->stream;
     PathBuffers& b = ctx->paths;
     PathView pv = pathView(ctx);
@@ -1514,8 +1619,12 @@ static bool launchChunk(mrt_context* ctx, FrameParams fp, StageEvents& ev, bool 
                                b.hits.ptr, cnt, cur, ctx->traversalStack.ptr);
         if (timed) ev.end(s);
         timed = ev.begin(s, 1);
-        hipLaunchKernelGGL(kShade<MODE>, dim3(blocksFor(fp.count)), dim3(kBlock), 0, s, fp, sv, pv, qv[cur], b.hits.ptr,
-                           qv[nxt], shv, shev, cnt, cur, nxt);
+        if (MODE == kModeRgb && nrc)
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(kShade<kModeRgb, true>), dim3(blocksFor(fp.count)), dim3(kBlock), 0, s,
+                               fp, sv, pv, qv[cur], b.hits.ptr, qv[nxt], shv, shev, cnt, cur, nxt, nrc->view);
+        else
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(kShade<MODE, false>), dim3(blocksFor(fp.count)), dim3(kBlock), 0, s, fp,
+                               sv, pv, qv[cur], b.hits.ptr, qv[nxt], shv, shev, cnt, cur, nxt, NrcView{});
         if (timed) ev.end(s);
         if (fp.neeEnabled || fp.envNee) {  // otherwise no shadow record can exist
             hipLaunchKernelGGL(kPrepShadow, dim3(1), dim3(1), 0, s, cnt);
@@ -1530,8 +1639,27 @@ static bool launchChunk(mrt_context* ctx, FrameParams fp, StageEvents& ev, bool 
         }
         cur = nxt;
     }
-    hipLaunchKernelGGL(kFinalize<MODE>, dim3(blocksFor(fp.count)), dim3(kBlock), 0, s, fp.count, pv, b.sampleRad.ptr);
-    hipLaunchKernelGGL(kReduce, dim3(blocksFor(fp.numPixels)), dim3(kBlock), 0, s, fp.numPixels, fp.pathBegin, fp.count,
+    uint32_t imagePaths = fp.count;
+    if (MODE == kModeRgb && nrc) {
+        // the cache's estimate of what image paths would have gathered after their query vertex
+        imagePaths = fp.nrcRenderPaths;
+        bool timed = ev.begin(s, 3);
+        if (!nrcInferQueries(ctx, nrc->view.query, kNrcQueryWords, cnt + kCntNrcQuery, nrc->view.queryCapacity,
+                             pv.rad))
+            return false;
+        if (timed) ev.end(s);
+        timed = ev.begin(s, 4);
+        hipLaunchKernelGGL(kNrcTargets, dim3(blocksFor(std::max(1u, nrc->view.recordCapacity))), dim3(kBlock), 0, s,
+                           nrc->view.record, cnt + kCntNrcRecord, nrc->view.recordCapacity, pv.rad,
+                           nrc->view.trainTarget);
+        if (nrc->train &&
+            !nrcTrainSteps(ctx, nrc->view.record, kNrcRecordWords, nrc->view.trainTarget, cnt + kCntNrcRecord,
+                           nrc->view.recordCapacity, nrc->steps, nrc->batch, nrc->learningRate))
+            return false;
+        if (timed) ev.end(s);
+    }
+    hipLaunchKernelGGL(kFinalize<MODE>, dim3(blocksFor(imagePaths)), dim3(kBlock), 0, s, imagePaths, pv, b.sampleRad.ptr);
+    hipLaunchKernelGGL(kReduce, dim3(blocksFor(fp.numPixels)), dim3(kBlock), 0, s, fp.numPixels, fp.pathBegin, imagePaths,
                        b.sampleRad.ptr, ctx->frameSum.ptr);
     return check(hipGetLastError(), "kernel launch");
 }
@@ -1571,6 +1699,7 @@ struct FrameSetup {
     uint64_t totalPaths = 0, chunk = 0;
     int mode = kModeRgb;
     uint32_t renderMode = 0, vw = 0, vh = 0;
+    uint32_t nrcQueryCapacity = 0, nrcRecordCapacity = 0;
 };
 
 static int setupFrame(mrt_context* ctx, const SceneData* scene, const mrt_render_params* prm, FrameSetup& fs) {
@@ -1610,8 +1739,36 @@ static int setupFrame(mrt_context* ctx, const SceneData* scene, const mrt_render
     uint64_t totalPaths = numPixels * spp;
     uint64_t chunk = std::min<uint64_t>(totalPaths, kMaxChunk);
     if (numPixels > 0 && numPixels <= kMaxChunk) chunk = std::min<uint64_t>(totalPaths, (kMaxChunk / numPixels) * numPixels);
+    const bool nrc = prm->nrcEnabled != 0u && mode == kModeRgb && numPixels > 0;
+    if (nrc) {
+        // one chunk: the image paths followed by the cache's training paths
+        if (totalPaths + prm->nrcTrainingPaths > kMaxChunk) {
+            setError("neural cache frames must fit one chunk of paths");
+            return -2;
+        }
+        chunk = totalPaths + prm->nrcTrainingPaths;
+        if (!nrcEnsure(ctx, 0)) return -3;
+    }
     FrameParams& fp = fs.fp;
     memset(&fp, 0, sizeof(fp));
+    if (nrc) {
+        fp.nrc = 1u;
+        fp.nrcPrefix = prm->nrcPrefix;
+        fp.nrcRenderPaths = (uint32_t)totalPaths;
+        for (int a = 0; a < 3; a++) {
+            float ext = prm->sceneMax[a] - prm->sceneMin[a];
+            fp.nrcBoxMin[a] = prm->sceneMin[a];
+            fp.nrcBoxInv[a] = ext > 0.0f ? 1.0f / ext : 0.0f;
+        }
+        const size_t records = (size_t)prm->nrcTrainingPaths * std::max(1u, scene->rrMaxDepth);
+        PathBuffers& b = ctx->paths;
+        if (!b.nrcQuery.reserve(totalPaths * kNrcQueryWords) || !b.nrcRecord.reserve(std::max<size_t>(records, 1) *
+                                                                                      kNrcRecordWords) ||
+            !b.nrcTarget.reserve(std::max<size_t>(records, 1)))
+            return -3;
+        fs.nrcQueryCapacity = (uint32_t)totalPaths;
+        fs.nrcRecordCapacity = (uint32_t)records;
+    }
     fp.sd = *scene;
     fp.numPixels = (uint32_t)numPixels;
     fp.vpW = vw;
@@ -1664,10 +1821,21 @@ extern "C" int mrt_render_frame(mrt_context* ctx, const SceneData* scene, const 
     const bool countTraversal = prm->traversalStatistics != 0u;
     hipEventRecord(ctx->evBegin, s);
     uint32_t chunks = 0;
+    NrcRun nrcRun{};
+    if (fp.nrc) {
+        PathBuffers& b = ctx->paths;
+        nrcRun.view = NrcView{b.nrcQuery.ptr, b.nrcRecord.ptr, nullptr, b.nrcTarget.ptr, fs.nrcQueryCapacity,
+                              fs.nrcRecordCapacity};
+        nrcRun.train = prm->nrcTrain != 0u;
+        nrcRun.steps = prm->nrcTrainingSteps;
+        nrcRun.batch = prm->nrcBatch;
+        nrcRun.learningRate = prm->nrcLearningRate;
+    }
     for (uint64_t begin = 0; begin < totalPaths; begin += chunk) {
         fp.pathBegin = begin;
         fp.count = (uint32_t)std::min<uint64_t>(chunk, totalPaths - begin);
-        bool ok = mode == kModeRgb    ? launchChunk<kModeRgb>(ctx, fp, ev, countTraversal)
+        if (fp.nrc) fp.count = (uint32_t)chunk;  // image paths and training paths in one chunk
+        bool ok = mode == kModeRgb    ? launchChunk<kModeRgb>(ctx, fp, ev, countTraversal, fp.nrc ? &nrcRun : nullptr)
                   : mode == kModeHero ? launchChunk<kModeHero>(ctx, fp, ev, countTraversal)
                                       : launchChunk<kModeSingle>(ctx, fp, ev, countTraversal);
         if (!ok) return -4;
@@ -1696,8 +1864,8 @@ extern "C" int mrt_render_frame(mrt_context* ctx, const SceneData* scene, const 
         float ms = 0.0f;
         hipEventElapsedTime(&ms, ctx->evBegin, ctx->evEnd);
         stats->traceMs = ms;
-        float stageMs[3] = {0.0f, 0.0f, 0.0f};
-        uint32_t stageN[3] = {0u, 0u, 0u};
+        float stageMs[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        uint32_t stageN[5] = {0u, 0u, 0u, 0u, 0u};
         for (size_t e = 0; e + 1 < ev.used; e += 2) {
             float t = 0.0f;
             hipEventElapsedTime(&t, ev.pool[e], ev.pool[e + 1]);
@@ -1711,6 +1879,14 @@ extern "C" int mrt_render_frame(mrt_context* ctx, const SceneData* scene, const 
         stats->extendLaunches = stageN[0];
         stats->shadeLaunches = stageN[1];
         stats->shadowLaunches = stageN[2];
+        stats->nrcInferMs = stageMs[3];
+        stats->nrcTrainMs = stageMs[4];
+        if (fp.nrc) {
+            stats->nrcQueries = std::min(hostCounters[kCntNrcQuery], fs.nrcQueryCapacity);
+            stats->nrcRecords = std::min(hostCounters[kCntNrcRecord], fs.nrcRecordCapacity);
+            const float denom = (float)prm->nrcTrainingSteps * (float)prm->nrcBatch * 3.0f;
+            stats->nrcLoss = prm->nrcTrain && denom > 0.0f ? nrcReadLoss(ctx) / denom : 0.0f;
+        }
         (void)wall0;
     }
     return 0;

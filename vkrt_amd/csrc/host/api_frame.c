@@ -154,6 +154,21 @@ static VKRT_Result syncScene(VKRT* vkrt, int requireDevice) {
         vkrt->stats.bvhLeafTriangles = bs.leafTriangles;
         vkrt->stats.bvhMaxDepth = bs.maxDepth;
         vkrt->stats.triangleCount = p->triangleCount;
+        /* scene box (the neural cache's position input is normalised by it) */
+        for (int a = 0; a < 3; a++) {
+            vkrt->sceneMin[a] = 0.0f;
+            vkrt->sceneMax[a] = 1.0f;
+        }
+        for (uint32_t t = 0; t < p->triangleCount; t++) {
+            for (int v = 0; v < 3; v++) {
+                const float* q = p->worldTriangles + 9u * t + 3u * v;
+                for (int a = 0; a < 3; a++) {
+                    if (t == 0 && v == 0) vkrt->sceneMin[a] = vkrt->sceneMax[a] = q[a];
+                    if (q[a] < vkrt->sceneMin[a]) vkrt->sceneMin[a] = q[a];
+                    if (q[a] > vkrt->sceneMax[a]) vkrt->sceneMax[a] = q[a];
+                }
+            }
+        }
     }
     if (mrt_upload_materials(vkrt->device, p->materials, p->materialCount) != 0) return deviceError();
     if (mrt_upload_meshes(vkrt->device, p->meshInfos, vkrt->meshCount) != 0) return deviceError();
@@ -205,6 +220,17 @@ static void fillRenderParams(const VKRT* vkrt, mrt_render_params* rp) {
     rp->envImportance = vkrt->envImportance;
     rp->hasTransmissive = vkrt->packed.hasTransmissive;
     rp->traversalStatistics = vkrt->traversalStatistics;
+    if (vkrt->nrc.enabled && vkrt->settings.renderMode == VKRT_RENDER_MODE_RGB) {
+        rp->nrcEnabled = 1u;
+        rp->nrcPrefix = vkrt->nrc.prefixBounces;
+        rp->nrcTrainingPaths = vkrt->nrc.trainingPaths;
+        rp->nrcTrainingSteps = vkrt->nrc.trainingSteps;
+        rp->nrcBatch = vkrt->nrc.batchSize;
+        rp->nrcTrain = vkrt->nrc.train;
+        rp->nrcLearningRate = vkrt->nrc.learningRate;
+        memcpy(rp->sceneMin, vkrt->sceneMin, sizeof(rp->sceneMin));
+        memcpy(rp->sceneMax, vkrt->sceneMax, sizeof(rp->sceneMax));
+    }
 }
 
 VKRT_Result VKRT_trace(VKRT* vkrt) {
@@ -235,6 +261,11 @@ VKRT_Result VKRT_trace(VKRT* vkrt) {
     vkrt->stats.shadowLaunches = rs.shadowLaunches;
     vkrt->stats.nodeVisits = rs.nodeVisits;
     vkrt->stats.triangleTests = rs.triangleTests;
+    vkrt->nrcStats.lastLoss = rs.nrcLoss;
+    vkrt->nrcStats.queries = rs.nrcQueries;
+    vkrt->nrcStats.trainingRecords = rs.nrcRecords;
+    vkrt->nrcStats.inferMs = rs.nrcInferMs;
+    vkrt->nrcStats.trainMs = rs.nrcTrainMs;
     vkrt->renderStatus.renderTimeMs = rs.traceMs;
     vkrt->frameTraced = 1;
     return VKRT_SUCCESS;

@@ -115,6 +115,7 @@ struct VKRT {
     VKRT_TraceStats stats;
     VKRT_NeuralCacheSettings nrc;
     VKRT_NeuralCacheStats nrcStats;
+    float sceneMin[3], sceneMax[3]; /* box of the uploaded geometry */
 };
 
 /* scene preparation (scene_prep.c) */

@@ -88,6 +88,16 @@ typedef struct mrt_render_params {
     uint32_t hasTransmissive; /* any material with transmission > 0 */
     uint32_t resetAccumulation;
     uint32_t traversalStatistics;
+    /* neural radiance cache (RGB mode) */
+    uint32_t nrcEnabled;
+    uint32_t nrcPrefix;
+    uint32_t nrcTrainingPaths;
+    uint32_t nrcTrainingSteps;
+    uint32_t nrcBatch;
+    uint32_t nrcTrain;
+    float nrcLearningRate;
+    float sceneMin[3]; /* scene box, for the cache's position input */
+    float sceneMax[3];
 } mrt_render_params;
 
 typedef struct mrt_render_stats {
@@ -103,6 +113,11 @@ typedef struct mrt_render_stats {
     uint32_t shadowLaunches;
     uint64_t nodeVisits;
     uint64_t triangleTests;
+    uint32_t nrcQueries;
+    uint32_t nrcRecords;
+    float nrcLoss;
+    float nrcInferMs;
+    float nrcTrainMs;
 } mrt_render_stats;
 
 /* One frame: samplesPerPixel paths for every viewport pixel, accumulated into
