@@ -82,6 +82,8 @@ def parse_args():
     p.add_argument("--prb-iters", type=int, default=200,
                    help="Adam iterations of the inverse-rendering variant (0: forward/backward timings only, "
                         "-1: no PRB variant)")
+    p.add_argument("--nrc-frames", type=int, default=240,
+                   help="orbit frames of the neural radiance cache variant (0 skips it)")
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                    help="only run device-to-device copies of MIB mebibytes (reference kernel for HBM counters)")
     return p.parse_args()
@@ -239,6 +241,108 @@ def prb_variant(iters: int):
     return out
 
 
+def nrc_variant(frames: int):
+    """Neural radiance cache configuration: the Cornell box at 1920x1080, 1 spp per
+    frame, depth 8, RGB. The cache trains online along `frames` frames of a slow
+    orbit (12 degrees about the box's vertical axis) that ends at the default view;
+    then, at that view, the plain tracer and the cached tracer (still training)
+    each accumulate 1-spp frames, and relMSE = mean((x - ref)^2 / (ref^2 + 1e-2))
+    against a 2048-spp plain-tracer image of the view is recorded with wall time
+    at 1, 2, 4, ..., 64 frames (readbacks excluded). time_ratio is the wall time
+    the cached tracer needs to reach the plain tracer's 64-frame relMSE, divided
+    by the plain tracer's time for it (null if it never does)."""
+    import math
+
+    import vkrt_amd as V
+    from vkrt_amd import scenes
+
+    w, h = 1920, 1080
+    radius = 1.85
+
+    def pose(deg):
+        th = math.radians(deg)
+        return ((0.5 + radius * math.sin(th), 0.5 - radius * math.cos(th), 0.5), (0.5, 0.5, 0.5), (0.0, 0.0, 1.0), 40.0)
+
+    def make(spp, cache):
+        r = V.Renderer(w, h)
+        scenes.load(r, scenes.cornell_box())
+        r.configure(spp=spp, max_depth=8, min_depth=MIN_DEPTH, mode=V.RENDER_MODE_RGB, rng=V.RNG_PCG32, seed=3)
+        if cache:
+            r.nrc_reset(1)
+            r.set_neural_cache(True)
+        return r
+
+    def relmse(x, ref):
+        return float(np.mean((x - ref) ** 2 / (ref * ref + 1e-2)))
+
+    ref_spp, ref_draws = 256, 8
+    rr = make(ref_spp, False)
+    rr.set_camera(*pose(0.0))
+    rr.start(w, h, 0)
+    for _ in range(ref_draws):
+        rr.draw()
+    ref = rr.accumulation()[..., :3].astype(np.float64)
+    rr.close()
+
+    r = make(1, True)
+    r.synchronize()
+    t0 = time.perf_counter()
+    for deg in np.linspace(-12.0, 0.0, frames):
+        r.set_camera(*pose(float(deg)))
+        r.start(w, h, 0)
+        r.draw()
+    r.synchronize()
+    orbit_ms = (time.perf_counter() - t0) * 1e3 / frames
+    st = r.neural_cache_stats()
+    log(f"NRC: {frames} orbit frames at {orbit_ms:.2f} ms/frame (inference {st.inferMs:.3f} ms, training "
+        f"{st.trainMs:.3f} ms, {st.queries} queries, {st.trainingRecords} records, loss {st.lastLoss:.3f})")
+
+    def converge(rend, n=64):
+        rend.set_camera(*pose(0.0))
+        rend.start(w, h, 0)
+        marks, elapsed = [], 0.0
+        for i in range(1, n + 1):
+            rend.synchronize()
+            t = time.perf_counter()
+            rend.draw()
+            rend.synchronize()
+            elapsed += time.perf_counter() - t
+            if i & (i - 1) == 0:
+                marks.append((i, elapsed * 1e3, relmse(rend.accumulation()[..., :3].astype(np.float64), ref)))
+        return marks
+
+    nrc_marks = converge(r)
+    r.close()
+    plain = make(1, False)
+    plain_marks = converge(plain)
+    plain.close()
+    target_err, target_ms = plain_marks[-1][2], plain_marks[-1][1]
+    ratio = None
+    prev = None
+    for frames_i, ms, err in nrc_marks:
+        if err <= target_err:
+            if prev is None:
+                ratio = ms / target_ms
+            else:  # interpolate the time in log error between the two checkpoints
+                (pms, perr) = prev
+                f = (math.log(perr) - math.log(target_err)) / max(math.log(perr) - math.log(err), 1e-12)
+                ratio = (pms + f * (ms - pms)) / target_ms
+            break
+        prev = (ms, err)
+    for (f, pms, perr), (_, nms, nerr) in zip(plain_marks, nrc_marks):
+        log(f"NRC equal time: {f:3d} frames plain {pms:8.1f} ms relMSE {perr:.4e} | cached {nms:8.1f} ms relMSE {nerr:.4e}")
+    log(f"NRC: time to reach the plain tracer's 64-frame relMSE, cached / plain = {ratio}")
+    return {"width": w, "height": h, "spp_per_frame": 1, "orbit_frames": frames, "orbit_ms_per_frame": round(orbit_ms, 3),
+            "reference_spp": ref_spp * ref_draws, "infer_ms": round(st.inferMs, 3), "train_ms": round(st.trainMs, 3),
+            "queries_per_frame": st.queries, "training_records_per_frame": st.trainingRecords,
+            "checkpoints": [{"frames": f, "plain_ms": round(pms, 2), "plain_relmse": perr, "cached_ms": round(nms, 2),
+                             "cached_relmse": nerr}
+                            for (f, pms, perr), (_, nms, nerr) in zip(plain_marks, nrc_marks)],
+            "time_ratio": None if ratio is None else round(ratio, 3),
+            "scene": "Cornell box (36 triangles), depth 8, RGB, PCG32 seed 3; cache after 2 bounces, 65536 training "
+                     "paths and 4 Adam steps of 16384 records per frame"}
+
+
 def main():
     args = parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -363,6 +467,8 @@ def main():
 
     if args.workload == "spectral" and args.prb_iters >= 0 and args.steps > 0:
         variants["prb"] = prb_variant(args.prb_iters)
+    if args.workload == "spectral" and args.nrc_frames > 0 and args.steps > 0:
+        variants["nrc"] = nrc_variant(args.nrc_frames)
 
     cpu = None
     if not args.no_cpu_baseline:
