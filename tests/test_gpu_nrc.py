@@ -194,19 +194,37 @@ def test_cache_output_is_added_with_the_path_throughput():
     assert np.allclose(d2, 2 * d1, rtol=1e-4, atol=1e-6)
 
 
-def test_online_training_lowers_the_loss():
+def test_online_training_improves_the_image():
+    """The cache trained online for 60 frames, then frozen, against the same cache
+    untrained: 64 accumulated frames of each against a 4096-spp plain image."""
     size = 64
-    r = cornell(size, 1, 8)
-    r.nrc_reset(1)
-    r.set_neural_cache(True, trainingPaths=16384, trainingSteps=4, batchSize=4096, learningRate=1e-2)
-    losses = []
-    for _ in range(40):
-        r.draw()
-        losses.append(r.neural_cache_stats().lastLoss)
-    st = r.neural_cache_stats()
-    img = r.accumulation()
-    r.close()
-    print(f"loss {losses[0]:.3f} -> {np.mean(losses[-5:]):.3f}; queries {st.queries}, records {st.trainingRecords}, "
-          f"inference {st.inferMs:.3f} ms, training {st.trainMs:.3f} ms")
-    assert np.isfinite(img).all()
-    assert np.mean(losses[-5:]) < 0.5 * losses[0]
+
+    def relmse(x, ref):
+        return float(np.mean((x - ref) ** 2 / (ref * ref + 1e-2)))
+
+    ref_r = cornell(size, 4096, 8)
+    ref_r.draw()
+    ref = ref_r.accumulation()[..., :3].astype(np.float64)
+    ref_r.close()
+    errs = {}
+    for trained in (False, True):
+        r = cornell(size, 1, 8)
+        r.nrc_reset(1)
+        if trained:
+            r.set_neural_cache(True, trainingPaths=16384, trainingSteps=4, batchSize=4096, learningRate=1e-2)
+            for _ in range(60):
+                r.start(size, size, 0)
+                r.draw()
+            st = r.neural_cache_stats()
+            print(f"last loss {st.lastLoss:.3f}, queries {st.queries}, records {st.trainingRecords}, "
+                  f"inference {st.inferMs:.3f} ms, training {st.trainMs:.3f} ms")
+        r.set_neural_cache(True, trainingPaths=16384, train=0)
+        r.start(size, size, 0)
+        for _ in range(64):
+            r.draw()
+        img = r.accumulation()[..., :3].astype(np.float64)
+        assert np.isfinite(img).all()
+        errs[trained] = relmse(img, ref)
+        r.close()
+    print(f"relMSE of 64 frames: untrained cache {errs[False]:.4e}, trained cache {errs[True]:.4e}")
+    assert errs[True] < 0.5 * errs[False]

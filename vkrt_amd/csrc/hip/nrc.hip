@@ -229,7 +229,7 @@ __device__ inline void nrcWaitLds() {
 // batch, runs the forward pass keeping the activations, back-propagates the
 // relative L2 loss sum_c (y_c - t_c)^2 / (y_c^2 + 0.01) (y detached in the
 // denominator) and adds the weight gradients into an LDS copy of the gradient,
-// which is added to the global fp32 gradient once per block.
+// which each block writes out as its partial gradient (grad + block * params).
 __global__ __launch_bounds__(kNrcBlock) void kNrcTrain(const _Float16* __restrict__ wfwd,
                                                       const _Float16* __restrict__ wbwd, const float4* __restrict__ in,
                                                       uint32_t stride, const float4* __restrict__ target,
@@ -372,20 +372,28 @@ __global__ __launch_bounds__(kNrcBlock) void kNrcTrain(const _Float16* __restric
         nrcWaitLds();
     }
     __syncthreads();
-    for (int k = threadIdx.x; k < kNrcParams; k += kNrcBlock) {
-        float v = gl[k];
-        if (v != 0.0f) atomicAdd(&grad[k], v);
-    }
+    // this block's partial gradient, summed over blocks by the optimiser step
+    float* part = grad + (size_t)blockIdx.x * kNrcParams;
+    for (int k = threadIdx.x; k < kNrcParams; k += kNrcBlock) part[k] = gl[k];
     // one loss sum per wave
     for (int off = 32; off > 0; off >>= 1) lossAcc += __shfl_down(lossAcc, off);
     if (l == 0 && lossAcc != 0.0f) atomicAdd(lossSum, lossAcc);
 }
 
-// Adam on the fp32 master weights, then the fp16 forward and backward copies.
-// The gradient is cleared for the next step.
-__global__ __launch_bounds__(kNrcBlock) void kNrcAdam(float* master, float* m1, float* m2, float* grad, _Float16* wfwd,
-                                                     _Float16* wbwd, float scale, float lr, float b1, float b2,
-                                                     float c1, float c2, float eps) {
+// Sum of the training blocks' partial gradients.
+__global__ __launch_bounds__(kNrcBlock) void kNrcSumParts(const float* grad, uint32_t parts, float* out) {
+    const int p = blockIdx.x * kNrcBlock + threadIdx.x;
+    if (p >= kNrcParams) return;
+    float sum = 0.0f;
+    for (uint32_t b = 0; b < parts; b++) sum += grad[(size_t)b * kNrcParams + p];
+    out[p] = sum;
+}
+
+// Adam on the fp32 master weights (gradient = sum of `parts` partials times
+// scale), then the fp16 forward and backward copies.
+__global__ __launch_bounds__(kNrcBlock) void kNrcAdam(float* master, float* m1, float* m2, const float* grad,
+                                                     uint32_t parts, _Float16* wfwd, _Float16* wbwd, float scale,
+                                                     float lr, float b1, float b2, float c1, float c2, float eps) {
     const int p = blockIdx.x * kNrcBlock + threadIdx.x;
     if (p >= kNrcParams) return;
     int layer, i, k;
@@ -405,7 +413,9 @@ __global__ __launch_bounds__(kNrcBlock) void kNrcAdam(float* master, float* m1, 
     }
     float w = master[p];
     if (layer != kNrcHidden || i < 3) {
-        float gr = grad[p] * scale;
+        float gsum = 0.0f;
+        for (uint32_t b = 0; b < parts; b++) gsum += grad[(size_t)b * kNrcParams + p];
+        float gr = gsum * scale;
         float a = b1 * m1[p] + (1.0f - b1) * gr;
         float b = b2 * m2[p] + (1.0f - b2) * gr * gr;
         m1[p] = a;
@@ -413,7 +423,6 @@ __global__ __launch_bounds__(kNrcBlock) void kNrcAdam(float* master, float* m1, 
         w -= lr * (a / c1) / (sqrtf(b / c2) + eps);
         master[p] = w;
     }
-    grad[p] = 0.0f;
     const _Float16 hw = (_Float16)w;
     if (layer == 0) {
         wfwd[p] = hw;
@@ -433,7 +442,7 @@ using namespace mrt;
 
 // ----------------------------------------------------------------- host ----
 struct NrcState {
-    DeviceArray<float> master, m1, m2, grad, loss;
+    DeviceArray<float> master, m1, m2, grad, gradSum, loss;  // grad: one partial per training block
     DeviceArray<_Float16> wfwd, wbwd;
     DeviceArray<float4> scratchIn, scratchTarget, scratchOut;
     uint32_t step = 0;
@@ -471,7 +480,7 @@ static bool nrcUploadMaster(mrt_context* ctx, const float* w) {
     if (!st->master.upload(w, kNrcParams, ctx->stream)) return false;
     // rebuild the fp16 copies through one zero-step Adam pass (lr 0 keeps the weights)
     hipLaunchKernelGGL(kNrcAdam, dim3((kNrcParams + kNrcBlock - 1) / kNrcBlock), dim3(kNrcBlock), 0, ctx->stream,
-                       st->master.ptr, st->m1.ptr, st->m2.ptr, st->grad.ptr, st->wfwd.ptr, st->wbwd.ptr, 0.0f, 0.0f,
+                       st->master.ptr, st->m1.ptr, st->m2.ptr, st->grad.ptr, 0u, st->wfwd.ptr, st->wbwd.ptr, 0.0f, 0.0f,
                        0.9f, 0.99f, 1.0f, 1.0f, 1e-8f);
     return check(hipGetLastError(), "nrc weights");
 }
@@ -480,13 +489,15 @@ bool nrcEnsure(mrt_context* ctx, uint32_t seed) {
     if (ctx->nrc) return true;
     NrcState* st = new NrcState();
     ctx->nrc = st;
+    // one training block per CU (the LDS gradient copy fills most of a CU's LDS)
+    st->inferGrid = (uint32_t)std::max(1, ctx->cuCount) * 2u;
+    st->trainGrid = (uint32_t)std::max(1, ctx->cuCount);
     bool ok = st->master.reserve(kNrcParams) && st->m1.reserve(kNrcParams) && st->m2.reserve(kNrcParams) &&
-              st->grad.reserve(kNrcParams) && st->loss.reserve(1) && st->wfwd.reserve(kNrcParams) &&
-              st->wbwd.reserve(kNrcBwdParams);
+              st->grad.reserve((size_t)st->trainGrid * kNrcParams) && st->loss.reserve(1) &&
+              st->wfwd.reserve(kNrcParams) && st->wbwd.reserve(kNrcBwdParams) && st->gradSum.reserve(kNrcParams);
     if (!ok) return false;
     hipMemsetAsync(st->m1.ptr, 0, kNrcParams * sizeof(float), ctx->stream);
     hipMemsetAsync(st->m2.ptr, 0, kNrcParams * sizeof(float), ctx->stream);
-    hipMemsetAsync(st->grad.ptr, 0, kNrcParams * sizeof(float), ctx->stream);
     hipMemsetAsync(st->wbwd.ptr, 0, kNrcBwdParams * sizeof(_Float16), ctx->stream);
     std::vector<float> w;
     nrcInitWeights(w, seed);
@@ -495,8 +506,6 @@ bool nrcEnsure(mrt_context* ctx, uint32_t seed) {
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)nrcTrainLds()),
                "nrc train LDS"))
         return false;
-    st->inferGrid = (uint32_t)std::max(1, ctx->cuCount) * 2u;
-    st->trainGrid = (uint32_t)std::max(1, ctx->cuCount);
     return check(hipStreamSynchronize(ctx->stream), "nrc init");
 }
 
@@ -507,6 +516,7 @@ void nrcRelease(mrt_context* ctx) {
     st->m1.release();
     st->m2.release();
     st->grad.release();
+    st->gradSum.release();
     st->loss.release();
     st->wfwd.release();
     st->wbwd.release();
@@ -542,7 +552,8 @@ bool nrcTrainSteps(mrt_context* ctx, const float4* in, uint32_t stride, const fl
         const float b1 = 0.9f, b2 = 0.99f;
         const float c1 = 1.0f - powf(b1, (float)st->step), c2 = 1.0f - powf(b2, (float)st->step);
         hipLaunchKernelGGL(kNrcAdam, dim3((kNrcParams + kNrcBlock - 1) / kNrcBlock), dim3(kNrcBlock), 0, ctx->stream,
-                           st->master.ptr, st->m1.ptr, st->m2.ptr, st->grad.ptr, st->wfwd.ptr, st->wbwd.ptr,
+                           st->master.ptr, st->m1.ptr, st->m2.ptr, st->grad.ptr, st->trainGrid, st->wfwd.ptr,
+                           st->wbwd.ptr,
                            1.0f / (float)batch, lr, b1, b2, c1, c2, 1e-8f);
     }
     return check(hipGetLastError(), "nrc training");
@@ -659,15 +670,15 @@ extern "C" int mrt_nrc_gradient(mrt_context* ctx, const float* inputs, const flo
         !st->scratchTarget.upload(tg.data(), tg.size(), ctx->stream))
         return -3;
     hipMemsetAsync(st->loss.ptr, 0, sizeof(float), ctx->stream);
-    hipMemsetAsync(st->grad.ptr, 0, kNrcParams * sizeof(float), ctx->stream);
     hipLaunchKernelGGL(kNrcTrain, dim3(st->trainGrid), dim3(kNrcBlock), nrcTrainLds(), ctx->stream, st->wfwd.ptr,
                        st->wbwd.ptr, st->scratchIn.ptr, 3u, st->scratchTarget.ptr, nullptr, n, n, 0u, 0u,
                        st->grad.ptr, st->loss.ptr);
+    hipLaunchKernelGGL(kNrcSumParts, dim3((kNrcParams + kNrcBlock - 1) / kNrcBlock), dim3(kNrcBlock), 0, ctx->stream,
+                       st->grad.ptr, st->trainGrid, st->gradSum.ptr);
     if (!check(hipGetLastError(), "nrc gradient")) return -3;
-    if (!check(hipMemcpyAsync(grad, st->grad.ptr, kNrcParams * sizeof(float), hipMemcpyDeviceToHost, ctx->stream),
+    if (!check(hipMemcpyAsync(grad, st->gradSum.ptr, kNrcParams * sizeof(float), hipMemcpyDeviceToHost, ctx->stream),
                "nrc gradient D2H"))
         return -3;
-    hipMemsetAsync(st->grad.ptr, 0, kNrcParams * sizeof(float), ctx->stream);
     float loss = nrcReadLoss(ctx);
     if (lossOut) *lossOut = loss;
     return loss < 0.0f ? -3 : 0;
