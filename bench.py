@@ -241,7 +241,7 @@ def prb_variant(iters: int):
     return out
 
 
-def nrc_variant(frames: int):
+def nrc_variant(frames: int, steps: int = 1, batch: int = 16384):
     """Neural radiance cache configuration: the Cornell box at 1920x1080, 1 spp per
     frame, depth 8, RGB. The cache trains online along `frames` frames of a slow
     orbit (12 degrees about the box's vertical axis) that ends at the default view;
@@ -250,7 +250,9 @@ def nrc_variant(frames: int):
     against a 2048-spp plain-tracer image of the view is recorded with wall time
     at 1, 2, 4, ..., 64 frames (readbacks excluded). time_ratio is the wall time
     the cached tracer needs to reach the plain tracer's 64-frame relMSE, divided
-    by the plain tracer's time for it (null if it never does)."""
+    by the plain tracer's time for it (null if it never does). The cache takes
+    `steps` Adam steps of `batch` records per frame (one step costs about 0.2 ms;
+    1 step gave the best time ratio of 1, 2 and 4)."""
     import math
 
     import vkrt_amd as V
@@ -269,7 +271,7 @@ def nrc_variant(frames: int):
         r.configure(spp=spp, max_depth=8, min_depth=MIN_DEPTH, mode=V.RENDER_MODE_RGB, rng=V.RNG_PCG32, seed=3)
         if cache:
             r.nrc_reset(1)
-            r.set_neural_cache(True)
+            r.set_neural_cache(True, trainingSteps=steps, batchSize=batch)
         return r
 
     def relmse(x, ref):
@@ -339,8 +341,8 @@ def nrc_variant(frames: int):
                              "cached_relmse": nerr}
                             for (f, pms, perr), (_, nms, nerr) in zip(plain_marks, nrc_marks)],
             "time_ratio": None if ratio is None else round(ratio, 3),
-            "scene": "Cornell box (36 triangles), depth 8, RGB, PCG32 seed 3; cache after 2 bounces, 65536 training "
-                     "paths and 4 Adam steps of 16384 records per frame"}
+            "scene": f"Cornell box (36 triangles), depth 8, RGB, PCG32 seed 3; cache after 2 bounces, 65536 training "
+                     f"paths and {steps} Adam steps of {batch} records per frame"}
 
 
 def main():

@@ -45,6 +45,12 @@ constexpr int kNrcParams = kNrcOffOut + kNrcOutRows * kNrcWidth;  // 29696
 constexpr int kNrcBwdOut = 4 * kNrcWidth * kNrcWidth;  // 16384
 constexpr int kNrcBwdParams = kNrcBwdOut + kNrcWidth * kNrcOutRows;  // 17408
 constexpr int kNrcBlock = 256;
+// LDS copy of the gradient during training, rows padded by 4 floats so that the
+// four lane groups of a fragment (rows 4 apart) fall in different banks
+constexpr int kNrcPadIn = kNrcIn + 4, kNrcPadW = kNrcWidth + 4;
+constexpr int kNrcLdsHidden = kNrcWidth * kNrcPadIn;                          // 12544
+constexpr int kNrcLdsOut = kNrcLdsHidden + 4 * kNrcWidth * kNrcPadW;          // 29952
+constexpr int kNrcLdsGrad = kNrcLdsOut + kNrcOutRows * kNrcWidth;             // 30976
 
 // hidden neuron held in K-slot s of a 64-wide hidden input (see the header)
 __host__ __device__ inline int nrcNeuronOfSlot(int s) {
@@ -238,9 +244,9 @@ __global__ __launch_bounds__(kNrcBlock) void kNrcTrain(const _Float16* __restric
                                                       float* lossSum) {
     const uint32_t n = count ? min(*count, nFixed) : nFixed;
     extern __shared__ __align__(16) float nrcShared[];
-    float* gl = nrcShared;                                                    // kNrcParams floats
-    _Float16* tabBase = reinterpret_cast<_Float16*>(nrcShared + kNrcParams);  // per wave: 64x16 + 192x16 halfs
-    for (int k = threadIdx.x; k < kNrcParams; k += kNrcBlock) gl[k] = 0.0f;
+    float* gl = nrcShared;                                                    // kNrcLdsGrad floats
+    _Float16* tabBase = reinterpret_cast<_Float16*>(nrcShared + kNrcLdsGrad);  // per wave: 64x16 + 192x16 halfs
+    for (int k = threadIdx.x; k < kNrcLdsGrad; k += kNrcBlock) gl[k] = 0.0f;
     __syncthreads();
     const int l = threadIdx.x & 63, row = l & 15, g = l >> 4;
     _Float16* td = tabBase + (threadIdx.x >> 6) * (kNrcWidth * 16 + kNrcIn * 16);  // deltas [64][16]
@@ -295,7 +301,7 @@ __global__ __launch_bounds__(kNrcBlock) void kNrcTrain(const _Float16* __restric
                 float4v acc = __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, float4v{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
                 if (g == 0)
 #pragma unroll
-                    for (int t = 0; t < 3; t++) atomicAdd(&gl[kNrcOffOut + t * kNrcWidth + kc * 16 + row], acc[t]);
+                    for (int t = 0; t < 3; t++) atomicAdd(&gl[kNrcLdsOut + t * kNrcWidth + kc * 16 + row], acc[t]);
             }
         }
         // delta of h5: (W5^T dOut) masked by h5 > 0, D layout in natural neuron blocks
@@ -322,7 +328,7 @@ __global__ __launch_bounds__(kNrcBlock) void kNrcTrain(const _Float16* __restric
                 for (int t = 0; t < 8; t++)
                     tx[nrcNeuronOfSlot(32 * kb + 8 * g + t) * 16 + row] = h[layer - 1][kb][t];
             nrcWaitLds();
-            float* gw = gl + kNrcOffHidden + (layer - 1) * kNrcWidth * kNrcWidth;
+            float* gw = gl + kNrcLdsHidden + (layer - 1) * kNrcWidth * kNrcPadW;
 #pragma unroll
             for (int ib = 0; ib < 4; ib++) {
                 half4 a = ldHalf4(td + (ib * 16 + row) * 16 + 4 * g);
@@ -331,7 +337,7 @@ __global__ __launch_bounds__(kNrcBlock) void kNrcTrain(const _Float16* __restric
                     half4 b = ldHalf4(tx + (kc * 16 + row) * 16 + 4 * g);
                     float4v acc = __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, float4v{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
 #pragma unroll
-                    for (int t = 0; t < 4; t++) atomicAdd(&gw[(ib * 16 + 4 * g + t) * kNrcWidth + kc * 16 + row], acc[t]);
+                    for (int t = 0; t < 4; t++) atomicAdd(&gw[(ib * 16 + 4 * g + t) * kNrcPadW + kc * 16 + row], acc[t]);
                 }
             }
             const half8 bd[2] = {nrcPack(dl[0], dl[1]), nrcPack(dl[2], dl[3])};
@@ -366,7 +372,7 @@ __global__ __launch_bounds__(kNrcBlock) void kNrcTrain(const _Float16* __restric
                 half4 b = ldHalf4(tx + (kc * 16 + row) * 16 + 4 * g);
                 float4v acc = __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, float4v{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
 #pragma unroll
-                for (int t = 0; t < 4; t++) atomicAdd(&gl[kNrcOffW0 + (ib * 16 + 4 * g + t) * kNrcIn + kc * 16 + row], acc[t]);
+                for (int t = 0; t < 4; t++) atomicAdd(&gl[(ib * 16 + 4 * g + t) * kNrcPadIn + kc * 16 + row], acc[t]);
             }
         }
         nrcWaitLds();
@@ -374,19 +380,42 @@ __global__ __launch_bounds__(kNrcBlock) void kNrcTrain(const _Float16* __restric
     __syncthreads();
     // this block's partial gradient, summed over blocks by the optimiser step
     float* part = grad + (size_t)blockIdx.x * kNrcParams;
-    for (int k = threadIdx.x; k < kNrcParams; k += kNrcBlock) part[k] = gl[k];
+    for (int k = threadIdx.x; k < kNrcParams; k += kNrcBlock) {
+        int src;
+        if (k < kNrcOffHidden) {
+            src = (k / kNrcIn) * kNrcPadIn + k % kNrcIn;
+        } else if (k < kNrcOffOut) {
+            const int r = k - kNrcOffHidden, layer = r / (kNrcWidth * kNrcWidth), e = r % (kNrcWidth * kNrcWidth);
+            src = kNrcLdsHidden + layer * kNrcWidth * kNrcPadW + (e / kNrcWidth) * kNrcPadW + e % kNrcWidth;
+        } else {
+            src = kNrcLdsOut + (k - kNrcOffOut);
+        }
+        part[k] = gl[src];
+    }
     // one loss sum per wave
     for (int off = 32; off > 0; off >>= 1) lossAcc += __shfl_down(lossAcc, off);
     if (l == 0 && lossAcc != 0.0f) atomicAdd(lossSum, lossAcc);
+}
+
+// Sum over the training blocks' partial gradients of parameter p, with 8
+// independent loads in flight per step (a chain of single loads would wait on
+// each one in turn).
+__device__ inline float nrcSumParts(const float* grad, uint32_t parts, int p) {
+    float acc[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    uint32_t b = 0;
+    for (; b + 8 <= parts; b += 8) {
+#pragma unroll
+        for (int u = 0; u < 8; u++) acc[u] += grad[(size_t)(b + u) * kNrcParams + p];
+    }
+    for (; b < parts; b++) acc[0] += grad[(size_t)b * kNrcParams + p];
+    return ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
 }
 
 // Sum of the training blocks' partial gradients.
 __global__ __launch_bounds__(kNrcBlock) void kNrcSumParts(const float* grad, uint32_t parts, float* out) {
     const int p = blockIdx.x * kNrcBlock + threadIdx.x;
     if (p >= kNrcParams) return;
-    float sum = 0.0f;
-    for (uint32_t b = 0; b < parts; b++) sum += grad[(size_t)b * kNrcParams + p];
-    out[p] = sum;
+    out[p] = nrcSumParts(grad, parts, p);
 }
 
 // Adam on the fp32 master weights (gradient = sum of `parts` partials times
@@ -413,9 +442,7 @@ __global__ __launch_bounds__(kNrcBlock) void kNrcAdam(float* master, float* m1, 
     }
     float w = master[p];
     if (layer != kNrcHidden || i < 3) {
-        float gsum = 0.0f;
-        for (uint32_t b = 0; b < parts; b++) gsum += grad[(size_t)b * kNrcParams + p];
-        float gr = gsum * scale;
+        float gr = nrcSumParts(grad, parts, p) * scale;
         float a = b1 * m1[p] + (1.0f - b1) * gr;
         float b = b2 * m2[p] + (1.0f - b2) * gr * gr;
         m1[p] = a;
@@ -450,7 +477,7 @@ struct NrcState {
 };
 
 namespace mrt {
-size_t nrcTrainLds() { return (size_t)kNrcParams * 4 + (size_t)(kNrcBlock / 64) * (kNrcWidth * 16 + kNrcIn * 16) * 2; }
+size_t nrcTrainLds() { return (size_t)kNrcLdsGrad * 4 + (size_t)(kNrcBlock / 64) * (kNrcWidth * 16 + kNrcIn * 16) * 2; }
 
 static NrcState* nrcState(mrt_context* ctx) { return reinterpret_cast<NrcState*>(ctx->nrc); }
 
