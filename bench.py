@@ -155,7 +155,10 @@ def cpu_baseline(renderer, rows: int, spp: int, depth: int):
     }
 
 
-def prb_variant(iters: int):
+PRB_LR = {"albedo": 0.01, "roughness": 0.01, "emission": 0.01, "environment": 0.002}
+
+
+def prb_variant(iters: int, lr=None):
     """Differentiable configuration: the Cornell geometry with 64 materials (one
     per face patch) and a 512x256 environment, 512x512, 32 spp forward and 32 spp
     backward from 3 seeded views. Times VKRT_draw and VKRT_renderBackward per view
@@ -208,13 +211,25 @@ def prb_variant(iters: int):
         target = A.SceneParameters(*[t * torch.tensor(rng.uniform(0.8, 1.2, t.shape), dtype=torch.float32)
                                      for t in start.tensors()])
         target.clamp_()
+        held_out = scenes.prb_views(seed=1, count=1)[0]  # a fourth view, not optimised on
         with torch.no_grad():
             ref = [A.render(r, target, v) for v in views]
+            ref_held = A.render(r, target, held_out)
         p = A.SceneParameters(*[t.clone().requires_grad_(True) for t in start.tensors()])
+
+        def held_loss():
+            with torch.no_grad():
+                return float(((A.render(r, p, held_out) - ref_held) ** 2).mean())
+
+        held_first = held_loss()
         # step sizes per parameter kind: albedo and roughness live in [0, 1]; emission
-        # of non-emitting patches starts at 0 and is only weakly observed
-        opt = torch.optim.Adam([{"params": [p.base_color, p.roughness], "lr": 0.01},
-                                {"params": [p.emission], "lr": 0.002}, {"params": [p.environment], "lr": 0.002}])
+        # of non-emitting patches starts at 0 and, like the sky texels, is only weakly
+        # observed (Adam moves every parameter by about its step size per iteration)
+        lr = dict(PRB_LR, **(lr or {}))
+        opt = torch.optim.Adam([{"params": [p.base_color], "lr": lr["albedo"]},
+                                {"params": [p.roughness], "lr": lr["roughness"]},
+                                {"params": [p.emission], "lr": lr["emission"]},
+                                {"params": [p.environment], "lr": lr["environment"]}])
         losses = []
         seen = None
         t0 = time.perf_counter()
@@ -231,10 +246,14 @@ def prb_variant(iters: int):
             losses.append(float(loss.detach()))
         el = time.perf_counter() - t0
         err1 = float((p.base_color - target.base_color).detach().abs()[seen].mean())
-        log(f"PRB: {iters} Adam iterations in {el:.1f} s, loss {losses[0]:.3e} -> {losses[-1]:.3e}, "
-            f"albedo error of {int(seen.sum())} seen materials {err0:.4f} -> {err1:.4f}")
-        out["optimisation"] = {"iterations": iters, "seconds": round(el, 2), "loss_first": losses[0],
-                               "loss_last": losses[-1], "seen_materials": int(seen.sum()),
+        held_last = held_loss()
+        log(f"PRB: {iters} Adam iterations in {el:.1f} s, loss {losses[0]:.3e} -> {losses[-1]:.3e}, held-out view "
+            f"{held_first:.3e} -> {held_last:.3e}, albedo error of {int(seen.sum())} seen materials "
+            f"{err0:.4f} -> {err1:.4f}")
+        out["optimisation"] = {"iterations": iters, "seconds": round(el, 2), "learning_rates": lr,
+                               "loss_first": losses[0],
+                               "loss_last": losses[-1], "held_out_view_loss_first": held_first,
+                               "held_out_view_loss_last": held_last, "seen_materials": int(seen.sum()),
                                "albedo_error_first": round(err0, 5), "albedo_error_last": round(err1, 5),
                                "parameters_optimised": "albedo, roughness, emission of 64 materials + 512x256 sky"}
     r.close()
