@@ -76,6 +76,9 @@ struct FrameParams {
     // image and query the cache after nrcPrefix bounces; the rest are training paths
     uint32_t nrc, nrcPrefix, nrcRenderPaths;
     float nrcBoxMin[3], nrcBoxInv[3];
+    // kGenerate wrote only the primary rays; kShade builds each path's state at depth 0
+    uint32_t leanStart;
+    const float4* accum;  // accumulation image (sample counts for the random streams)
 };
 
 // Neural radiance cache records. A query is 4 float4: the 11 network inputs in
@@ -344,16 +347,23 @@ DEV void updateMedium4(const Spec& sp, const BMat& m, uint32_t ff, uint32_t isT,
 }
 
 // ------------------------------------------------------------ kernels --
+// Pixel, random stream and wavelengths of chunk path i, as the primary ray was
+// generated: image paths are sample s of pixel k with p = s * pixels + k; cache
+// training paths go through a pseudo-random pixel on a stream of their own.
+struct PathStart {
+    Rng rng;    // after the two jitter draws and the wavelength draw
+    F4 lam;
+    float jx, jy;
+    int px, py;
+};
 template <int MODE>
-__global__ __launch_bounds__(kBlock) void kGenerate(FrameParams fp, PathView pv, QueueView q, const float4* accum) {
-    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= fp.count) return;
+DEV PathStart pathStart(const FrameParams& fp, uint32_t i) {
+    PathStart ps;
     uint64_t p = fp.pathBegin + i;
     uint32_t s = (uint32_t)(p / fp.numPixels);
     uint32_t k = (uint32_t)(p % fp.numPixels);
     const bool training = fp.nrc && i >= fp.nrcRenderPaths;
     if (training) {
-        // a cache training path through a pseudo-random pixel, on a stream of its own
         uint32_t h = (i - fp.nrcRenderPaths) * 0x9E3779B9u ^ (fp.sd.frameNumber * 0x85EBCA6Bu + 0x165667B1u);
         h ^= h >> 16;
         h *= 0x7FEB352Du;
@@ -362,36 +372,53 @@ __global__ __launch_bounds__(kBlock) void kGenerate(FrameParams fp, PathView pv,
         h ^= h >> 16;
         k = h % fp.numPixels;
     }
-    int px = (int)(fp.sd.viewportRect[0] + k % fp.vpW);
-    int py = (int)(fp.sd.viewportRect[1] + k / fp.vpW);
-    float4 prev = accum[(size_t)py * fp.accumWidth + px];
+    ps.px = (int)(fp.sd.viewportRect[0] + k % fp.vpW);
+    ps.py = (int)(fp.sd.viewportRect[1] + k / fp.vpW);
+    float4 prev = fp.accum[(size_t)ps.py * fp.accumWidth + ps.px];
     uint32_t sampleAbs = training ? 0x80000000u + (i - fp.nrcRenderPaths) : (uint32_t)(prev.w + 0.5f) + s;
-    Rng rng = rngInit(fp.rngPolicy, fp.rngSeed, px, py, fp.sd.frameNumber, sampleAbs);
-    float jx = rnd(rng);
-    float jy = rnd(rng);
-    F3 o, d;
-    primaryRay(fp.sd, px, py, jx - 0.5f, jy - 0.5f, o, d);
-    uint32_t flags = 0u;
+    ps.rng = rngInit(fp.rngPolicy, fp.rngSeed, ps.px, ps.py, fp.sd.frameNumber, sampleAbs);
+    ps.jx = rnd(ps.rng);
+    ps.jy = rnd(ps.rng);
+    ps.lam = f4(0.0f);
     if (MODE == kModeSingle) {
-        float u = wavelengthUnit(rng, sampleAbs);
-        pv.lam[i] = make_float4(360.0f + sat(u) * 470.0f, 470.0f, 0.0f, 0.0f);
+        float u = wavelengthUnit(ps.rng, sampleAbs);
+        ps.lam = F4{360.0f + sat(u) * 470.0f, 470.0f, 0.0f, 0.0f};
     } else if (MODE == kModeHero) {
-        float u = wavelengthUnit(rng, sampleAbs);
-        pv.lam[i] = make_float4(360.0f + fracf(u + 0.0f) * 470.0f, 360.0f + fracf(u + 0.25f) * 470.0f,
-                                360.0f + fracf(u + 0.5f) * 470.0f, 360.0f + fracf(u + 0.75f) * 470.0f);
-        pv.radXYZ[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        pv.tpp[i] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
-        // ptpp / pbtp are read only after a BSDF sample has written them
-        flags |= kFHeroActive;
+        float u = wavelengthUnit(ps.rng, sampleAbs);
+        ps.lam = F4{360.0f + fracf(u + 0.0f) * 470.0f, 360.0f + fracf(u + 0.25f) * 470.0f,
+                    360.0f + fracf(u + 0.5f) * 470.0f, 360.0f + fracf(u + 0.75f) * 470.0f};
     }
-    pv.rng[i] = storeRng(rng);
-    pv.flags[i] = flags;
-    pv.thr[i] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
-    pv.rad[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    // hero paths write every field of misc when they collapse to one wavelength,
-    // before anything reads it
-    if (MODE != kModeHero) pv.misc[i] = make_float4(0.0f, 1.0f, 0.0f, 0.0f);
-    // sigma is read only inside an absorbing medium (kFAbsorption), after a transmission wrote it
+    return ps;
+}
+
+// Primary rays of the chunk. With FULL the path state is initialised too (the
+// adjoint's replay reads it); otherwise kShade rebuilds it at depth 0 from the
+// same streams, which saves writing and re-reading about 130 bytes per path.
+template <int MODE, bool FULL>
+__global__ __launch_bounds__(kBlock) void kGenerate(FrameParams fp, PathView pv, QueueView q) {
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= fp.count) return;
+    PathStart ps = pathStart<MODE>(fp, i);
+    F3 o, d;
+    primaryRay(fp.sd, ps.px, ps.py, ps.jx - 0.5f, ps.jy - 0.5f, o, d);
+    if (FULL) {
+        uint32_t flags = 0u;
+        if (MODE != kModeRgb) pv.lam[i] = st4(ps.lam);
+        if (MODE == kModeHero) {
+            pv.radXYZ[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            pv.tpp[i] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+            // ptpp / pbtp are read only after a BSDF sample has written them
+            flags |= kFHeroActive;
+        }
+        pv.rng[i] = storeRng(ps.rng);
+        pv.flags[i] = flags;
+        pv.thr[i] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+        pv.rad[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        // hero paths write every field of misc when they collapse to one wavelength,
+        // before anything reads it
+        if (MODE != kModeHero) pv.misc[i] = make_float4(0.0f, 1.0f, 0.0f, 0.0f);
+        // sigma is read only inside an absorbing medium (kFAbsorption), after a transmission wrote it
+    }
     q.o[i] = make_float4(o.x, o.y, o.z, __uint_as_float(i));
     q.d[i] = make_float4(d.x, d.y, d.z, kRayTMax);
 }
@@ -617,8 +644,21 @@ __global__ __launch_bounds__(kBlock, 3) void kShade(FrameParams fp, SceneView sv
         F3 o = xyz(ro), d = xyz(rdir);
         float4 hh = hits[i];
         Hit h{hh.x, __float_as_uint(hh.y), hh.z, hh.w};
-        uint32_t flags = pv.flags[q];
-        F4 lam = (MODE != kModeRgb) ? ld4(pv.lam[q]) : f4(0.0f);
+        // depth 0 of a lean frame: the path state is the one kGenerate would have written
+        const bool first = fp.leanStart != 0u && fp.depth == 0u;
+        PathStart ps{};
+        if (first) ps = pathStart<MODE>(fp, q);
+        // first read of the accumulated radiance in this vertex (0 at depth 0 of a lean frame)
+        auto radIn = [&]() { return first ? f4(0.0f) : ld4(pv.rad[q]); };
+        auto radXYZIn = [&]() { return first ? f4(0.0f) : ld4(pv.radXYZ[q]); };
+        uint32_t flags = first ? (MODE == kModeHero ? kFHeroActive : 0u) : pv.flags[q];
+        F4 lam = (MODE != kModeRgb) ? (first ? ps.lam : ld4(pv.lam[q])) : f4(0.0f);
+        if (first) {  // what the finalisation and the shadow passes read of every path
+            pv.rad[q] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (MODE == kModeHero) pv.radXYZ[q] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (MODE != kModeRgb) pv.lam[q] = st4(lam);
+            pv.flags[q] = flags;
+        }
         if (MODE == kModeHero && (flags & kFPendingCollapse)) {
             // collapse to the hero wavelength (src/shaders/integrator/path/spectral_hero/transport.slang:77-87),
             // deferred from the previous vertex until its shadow rays had added their contributions
@@ -631,10 +671,10 @@ __global__ __launch_bounds__(kBlock, 3) void kShade(FrameParams fp, SceneView sv
         const bool absorbing = (flags & kFAbsorption) != 0u;
         const bool refractive = (flags & kFRefractive) != 0u;
         const bool heroActive = MODE == kModeHero && (flags & kFHeroActive) != 0u;
-        F4 thr = ld4(pv.thr[q]);
+        F4 thr = first ? f4(1.0f) : ld4(pv.thr[q]);
         // prevBsdfPdf, scalar throughput, scalar wavelength, scalar 1/pdf; unused while
         // all four hero wavelengths are alive
-        F4 misc = heroActive ? f4(0.0f) : ld4(pv.misc[q]);
+        F4 misc = heroActive ? f4(0.0f) : (first ? F4{0.0f, 1.0f, 0.0f, 0.0f} : ld4(pv.misc[q]));
 
         if (h.tri == kNoHit) {
             if (!fp.neeOnly && !(refractive || absorbing)) {
@@ -645,22 +685,22 @@ __global__ __launch_bounds__(kBlock, 3) void kShade(FrameParams fp, SceneView sv
                 float w = 1.0f;
                 if (prevEnv && !heroActive && envPdf > 0.0f && misc.x > 0.0f) w = powerHeuristic(misc.x, envPdf);
                 if (MODE == kModeRgb) {
-                    F4 rad = ld4(pv.rad[q]);
+                    F4 rad = radIn();
                     pv.rad[q] = st4(F3{rad.x, rad.y, rad.z} + F3{thr.x, thr.y, thr.z} * (env * w), 0.0f);
                 } else if (MODE == kModeSingle) {
-                    F4 rad = ld4(pv.rad[q]);
+                    F4 rad = radIn();
                     rad.x += thr.x * (spec1(fp.spec, env, lam.x) * w);
                     pv.rad[q] = st4(rad);
                 } else if (heroActive) {
-                    F4 tpp = ld4(pv.tpp[q]);
+                    F4 tpp = first ? f4(1.0f) : ld4(pv.tpp[q]);
                     float hw = heroBalance(tpp);
                     if (prevEnv && envPdf > 0.0f) {
                         F4 ptpp = ld4(pv.ptpp[q]), pbtp = ld4(pv.pbtp[q]);
                         hw = spectralMis(ptpp * pbtp, ptpp * envPdf);
                     }
-                    pv.rad[q] = st4(ld4(pv.rad[q]) + thr * hw * spec4(fp.spec, env, lam));
+                    pv.rad[q] = st4(radIn() + thr * hw * spec4(fp.spec, env, lam));
                 } else {
-                    F4 rx = ld4(pv.radXYZ[q]);
+                    F4 rx = radXYZIn();
                     rx.w += misc.y * (spec1(fp.spec, env, misc.z) * w);
                     pv.radXYZ[q] = st4(rx);
                 }
@@ -697,8 +737,8 @@ __global__ __launch_bounds__(kBlock, 3) void kShade(FrameParams fp, SceneView sv
         // neural cache: training paths record every vertex; image paths stop at the query vertex
         const bool nrcTrain = NRC && q >= fp.nrcRenderPaths;
         const bool nrcQuery = NRC && !nrcTrain && fp.depth == fp.nrcPrefix;
-        Rng rng = loadRng(pv.rng[q], fp.rngPolicy);
-        F4 tpp = heroActive ? ld4(pv.tpp[q]) : f4(0.0f);
+        Rng rng = first ? ps.rng : loadRng(pv.rng[q], fp.rngPolicy);
+        F4 tpp = heroActive ? (first ? f4(1.0f) : ld4(pv.tpp[q])) : f4(0.0f);
 
         // next-event estimation (src/shaders/light/direct/*_lighting.slang): the contribution is
         // evaluated here assuming visibility; the shadow kernel picks it or drops it
@@ -820,7 +860,7 @@ __global__ __launch_bounds__(kBlock, 3) void kShade(FrameParams fp, SceneView sv
                 float4* r = nv.record + (size_t)slot * kNrcRecordWords;
                 nrcInputs(fp, sf, mat, d, q, r);
                 r[3] = make_float4(thr.x, thr.y, thr.z, 0.0f);
-                r[4] = pv.rad[q];
+                r[4] = st4(radIn());
                 r[5] = hasEm ? make_float4(em.x, em.y, em.z, 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             }
         }
@@ -841,16 +881,16 @@ __global__ __launch_bounds__(kBlock, 3) void kShade(FrameParams fp, SceneView sv
             if (hasEm) cv = cv + em;
             F4 add = thrUse * cv;
             if (MODE == kModeRgb) {
-                F4 rad = ld4(pv.rad[q]);
+                F4 rad = radIn();
                 pv.rad[q] = st4(F3{rad.x, rad.y, rad.z} + F3{add.x, add.y, add.z}, 0.0f);
             } else if (MODE == kModeSingle) {
-                F4 rad = ld4(pv.rad[q]);
+                F4 rad = radIn();
                 rad.x += add.x;
                 pv.rad[q] = st4(rad);
             } else if (heroActive) {
-                pv.rad[q] = st4(ld4(pv.rad[q]) + add);
+                pv.rad[q] = st4(radIn() + add);
             } else {
-                F4 rx = ld4(pv.radXYZ[q]);
+                F4 rx = radXYZIn();
                 rx.w += add.x;
                 pv.radXYZ[q] = st4(rx);
             }
@@ -921,7 +961,7 @@ __global__ __launch_bounds__(kBlock, 3) void kShade(FrameParams fp, SceneView sv
                 if (shadowEmit || envEmit) {
                     newFlags |= kFPendingCollapse;
                 } else {
-                    F4 rx = ld4(pv.radXYZ[q]);
+                    F4 rx = ld4(pv.radXYZ[q]);  // this vertex may have added to both already
                     F3 xyzv = F3{rx.x, rx.y, rx.z} + toXYZ4(ld4(pv.rad[q]), lam, f4(470.0f));
                     pv.radXYZ[q] = make_float4(xyzv.x, xyzv.y, xyzv.z, rx.w);
                     pv.rad[q] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -997,10 +1037,9 @@ __global__ __launch_bounds__(kBlock, 3) void kShade(FrameParams fp, SceneView sv
     }
 }
 
+// The radiance path i delivers to its pixel (RGB, or XYZ in the spectral modes).
 template <int MODE>
-__global__ __launch_bounds__(kBlock) void kFinalize(uint32_t count, PathView pv, float4* sampleRad) {
-    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= count) return;
+DEV F3 pathRadiance(const PathView& pv, uint32_t i) {
     F4 rad = ld4(pv.rad[i]);
     F3 out;
     if (MODE == kModeRgb) {
@@ -1018,7 +1057,37 @@ __global__ __launch_bounds__(kBlock) void kFinalize(uint32_t count, PathView pv,
             out = out + toXYZ1(rx.w, m.z, m.w);
         }
     }
+    return out;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void kFinalize(uint32_t count, PathView pv, float4* sampleRad) {
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= count) return;
+    F3 out = pathRadiance<MODE>(pv, i);
     sampleRad[i] = make_float4(out.x, out.y, out.z, 0.0f);
+}
+
+// kFinalize and kReduce in one pass: frameSum[k] += the radiance of every sample
+// of pixel k inside [pathBegin, pathBegin+count), in sample order. Neighbouring
+// threads are neighbouring pixels, so every sample's loads are coalesced.
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void kFinalizeReduce(uint32_t numPixels, uint64_t pathBegin, uint32_t count,
+                                                          PathView pv, float4* frameSum) {
+    uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= numPixels) return;
+    uint64_t end = pathBegin + count;
+    uint64_t s0 = pathBegin / numPixels;
+    F4 acc = ld4(frameSum[k]);
+#pragma unroll 4
+    for (uint64_t s = s0;; s++) {
+        uint64_t p = s * numPixels + k;
+        if (p >= end) break;
+        if (p < pathBegin) continue;
+        F3 v = pathRadiance<MODE>(pv, (uint32_t)(p - pathBegin));
+        acc = F4{acc.x + v.x, acc.y + v.y, acc.z + v.z, 0.0f};
+    }
+    frameSum[k] = st4(acc);
 }
 
 // frameSum[k] += radiance of every sample of pixel k inside [pathBegin, pathBegin+count), in sample order
@@ -1588,7 +1657,7 @@ static void queueViews(mrt_context* ctx, QueueView qv[2]) {
 
 template <int MODE>
 static bool launchChunk(mrt_context* ctx, FrameParams fp, StageEvents& ev, bool countTraversal,
-                        const NrcRun* nrc = nullptr) {
+                        const NrcRun* nrc = nullptr, bool keepSampleRad = false) {
     hipStream_t s = ctx->stream;
     PathBuffers& b = ctx->paths;
     PathView pv = pathView(ctx);
@@ -1602,7 +1671,12 @@ static bool launchChunk(mrt_context* ctx, FrameParams fp, StageEvents& ev, bool 
     }
     ShadowView shv{b.shadow[0].ptr, b.shadow[1].ptr, b.shadow[2].ptr, b.shadow[3].ptr, cnt + kCntShadow};
     ShadowView shev{b.shadowEnv[0].ptr, b.shadowEnv[1].ptr, b.shadowEnv[2].ptr, nullptr, cnt + kCntShadowEnv};
-    hipLaunchKernelGGL(kGenerate<MODE>, dim3(blocksFor(fp.count)), dim3(kBlock), 0, s, fp, pv, qv[0], ctx->accum.ptr);
+    if (fp.leanStart)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(kGenerate<MODE, false>), dim3(blocksFor(fp.count)), dim3(kBlock), 0, s, fp, pv,
+                           qv[0]);
+    else
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(kGenerate<MODE, true>), dim3(blocksFor(fp.count)), dim3(kBlock), 0, s, fp, pv,
+                           qv[0]);
     hipLaunchKernelGGL(kSetCount, dim3(1), dim3(1), 0, s, cnt, kCntQueue0, fp.count);
     int cur = 0;
     uint32_t maxDepth = fp.sd.rrMaxDepth;
@@ -1658,9 +1732,15 @@ static bool launchChunk(mrt_context* ctx, FrameParams fp, StageEvents& ev, bool 
             return false;
         if (timed) ev.end(s);
     }
-    hipLaunchKernelGGL(kFinalize<MODE>, dim3(blocksFor(imagePaths)), dim3(kBlock), 0, s, imagePaths, pv, b.sampleRad.ptr);
-    hipLaunchKernelGGL(kReduce, dim3(blocksFor(fp.numPixels)), dim3(kBlock), 0, s, fp.numPixels, fp.pathBegin, imagePaths,
-                       b.sampleRad.ptr, ctx->frameSum.ptr);
+    if (keepSampleRad) {
+        hipLaunchKernelGGL(kFinalize<MODE>, dim3(blocksFor(imagePaths)), dim3(kBlock), 0, s, imagePaths, pv,
+                           b.sampleRad.ptr);
+        hipLaunchKernelGGL(kReduce, dim3(blocksFor(fp.numPixels)), dim3(kBlock), 0, s, fp.numPixels, fp.pathBegin,
+                           imagePaths, b.sampleRad.ptr, ctx->frameSum.ptr);
+    } else {
+        hipLaunchKernelGGL(kFinalizeReduce<MODE>, dim3(blocksFor(fp.numPixels)), dim3(kBlock), 0, s, fp.numPixels,
+                           fp.pathBegin, imagePaths, pv, ctx->frameSum.ptr);
+    }
     return check(hipGetLastError(), "kernel launch");
 }
 
@@ -1782,6 +1862,8 @@ static int setupFrame(mrt_context* ctx, const SceneData* scene, const mrt_render
     fp.envNee = prm->envImportance != 0u && ctx->envWidth > 0u && ctx->envAliasQ.ptr &&
                 scene->environmentTextureIndex != VKRT_INVALID_INDEX;
     if (chunk > 0 && !ensurePaths(ctx, chunk, mode, fp.neeEnabled != 0u, fp.envNee != 0u)) return -3;
+    fp.leanStart = 1u;
+    fp.accum = ctx->accum.ptr;
     fp.spec.table = ctx->rgb2spec.ptr;
     fp.spec.res = scene->rgb2specSRGB.res;
     fp.spec.scaleOffset = scene->rgb2specSRGB.scaleOffset;
@@ -1946,10 +2028,10 @@ extern "C" int mrt_render_backward(mrt_context* ctx, const SceneData* scene, con
         fp.pathBegin = begin;
         fp.count = (uint32_t)std::min<uint64_t>(fs.chunk, fs.totalPaths - begin);
         // primal: the radiance of every path of the chunk (sampleRad)
-        if (!launchChunk<kModeRgb>(ctx, fp, untimed, false)) return -4;
+        if (!launchChunk<kModeRgb>(ctx, fp, untimed, false, nullptr, true)) return -4;
         // replay with the same streams
-        hipLaunchKernelGGL(kGenerate<kModeRgb>, dim3(blocksFor(fp.count)), dim3(kBlock), 0, s, fp, pv, qv[0],
-                           ctx->accum.ptr);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(kGenerate<kModeRgb, true>), dim3(blocksFor(fp.count)), dim3(kBlock), 0, s, fp,
+                           pv, qv[0]);
         hipLaunchKernelGGL(kSetCount, dim3(1), dim3(1), 0, s, cnt, kCntQueue0, fp.count);
         hipLaunchKernelGGL(kAdjointInit, dim3(blocksFor(fp.count)), dim3(kBlock), 0, s, fp.count, pv,
                            b.sampleRad.ptr);
