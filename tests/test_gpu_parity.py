@@ -187,3 +187,18 @@ def test_environment_nee_converges_to_the_reference_estimator():
     diff = means[True].mean() - means[False].mean()
     print(f"image mean without NEE {means[False].mean():.6f}, with {means[True].mean():.6f}, diff {diff:.2e}, se {se:.2e}")
     assert abs(diff) < 4 * se + 1e-4 * means[False].mean()
+
+
+def test_material_sorted_shading_gives_identical_images():
+    """Shading the hits of every depth grouped by material class changes only the
+    order of the work: every path keeps its stream, so the image is bit-identical."""
+    s = small_sphere_scene()
+    imgs = []
+    for sort in (False, True):
+        r = make(s, 64, 48, 16, mode=V.RENDER_MODE_SPECTRAL, seed=4)
+        r.set_material_sorting(sort)
+        r.start(64, 48, 0)
+        r.draw()
+        imgs.append(r.accumulation())
+        r.close()
+    assert np.array_equal(imgs[0], imgs[1])

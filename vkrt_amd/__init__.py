@@ -424,6 +424,7 @@ def library() -> C.CDLL:
         "VKRT_getTextureCount": (C.c_int, [P, up]),
         "VKRT_addTextureFromPixels": (C.c_int, [P, C.POINTER(TextureUpload), up]),
         "VKRT_removeTexture": (C.c_int, [P, u32]),
+        "VKRT_setMaterialSorting": (C.c_int, [P, C.c_uint8]),
         "VKRT_defaultNeuralCacheSettings": (None, [C.POINTER(NeuralCacheSettings)]),
         "VKRT_setNeuralCache": (C.c_int, [P, C.POINTER(NeuralCacheSettings)]),
         "VKRT_getNeuralCacheStats": (C.c_int, [P, C.POINTER(NeuralCacheStats)]),
@@ -756,6 +757,9 @@ class Renderer:
 
     def synchronize(self):
         _ok(self.lib.VKRT_deviceSynchronize(self.h), "VKRT_deviceSynchronize")
+
+    def set_material_sorting(self, enabled: bool):
+        _ok(self.lib.VKRT_setMaterialSorting(self.h, 1 if enabled else 0), "VKRT_setMaterialSorting")
 
     def set_traversal_statistics(self, enabled: bool):
         _ok(self.lib.VKRT_setTraversalStatistics(self.h, 1 if enabled else 0), "VKRT_setTraversalStatistics")

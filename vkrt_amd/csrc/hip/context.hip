@@ -76,6 +76,7 @@ extern "C" void mrt_destroy(mrt_context* ctx) {
     ctx->tris.release();
     ctx->nodes.release();
     ctx->shadeTris.release();
+    ctx->triClass.release();
     ctx->meshes.release();
     ctx->materials.release();
     ctx->emissiveMeshes.release();
@@ -107,6 +108,11 @@ extern "C" void mrt_destroy(mrt_context* ctx) {
     b.nrcQuery.release();
     b.nrcRecord.release();
     b.nrcTarget.release();
+    b.sortKey.release();
+    b.sortPerm.release();
+    b.sortCount.release();
+    b.sortOffset.release();
+    b.sortScratch.release();
     b.lam.release();
     b.misc.release();
     b.sigma.release();
@@ -159,6 +165,7 @@ extern "C" int mrt_upload_geometry(mrt_context* ctx, const mrt_geometry_desc* g)
     if (!ok) return -3;
     ctx->triCount = g->triangleCount;
     ctx->shadeTrisDirty = true;
+    ctx->triClassDirty = true;
     return sync(ctx, "upload geometry");
 }
 
@@ -167,6 +174,7 @@ extern "C" int mrt_upload_meshes(mrt_context* ctx, const MeshInfo* meshInfos, ui
     if (!check(hipSetDevice(ctx->device), "hipSetDevice")) return -2;
     if (!ctx->meshes.upload(meshInfos, meshCount, ctx->stream)) return -3;
     ctx->shadeTrisDirty = true;
+    ctx->triClassDirty = true;
     return sync(ctx, "upload meshes");
 }
 
@@ -174,6 +182,7 @@ extern "C" int mrt_upload_materials(mrt_context* ctx, const Material* materials,
     if (!ctx) return -1;
     if (!check(hipSetDevice(ctx->device), "hipSetDevice")) return -2;
     if (!ctx->materials.upload(materials, materialCount, ctx->stream)) return -3;
+    ctx->triClassDirty = true;
     return sync(ctx, "upload materials");
 }
 

@@ -53,6 +53,9 @@ struct PathBuffers {
     DeviceArray<float4> shadowEnv[3];  // environment NEE: origin+path, direction+tmax, contribution if visible
     DeviceArray<float4> sampleRad;
     DeviceArray<float4> nrcQuery, nrcRecord, nrcTarget;  // neural cache queries, training records, targets
+    DeviceArray<uint8_t> sortKey;                         // material class of every queued ray
+    DeviceArray<uint32_t> sortPerm, sortCount, sortOffset;  // shading order, per-block counts, their scan
+    DeviceArray<uint8_t> sortScratch;                     // rocPRIM scan temporary storage
     size_t capacity = 0;
 };
 
@@ -87,6 +90,8 @@ struct mrt_context {
     mrt::DeviceArray<mrt::BvhNode> nodes;
     mrt::DeviceArray<mrt::ShadeTri> shadeTris;
     bool shadeTrisDirty = true;  // geometry or mesh transforms changed since the records were built
+    mrt::DeviceArray<uint8_t> triClass;  // material class per triangle (shading sort)
+    bool triClassDirty = true;
     mrt::DeviceArray<MeshInfo> meshes;
     mrt::DeviceArray<Material> materials;
     mrt::DeviceArray<EmissiveMesh> emissiveMeshes;

@@ -18,6 +18,8 @@
 #include <chrono>
 #include <cstring>
 
+#include <rocprim/device/device_scan.hpp>
+
 #include "dev_shading.h"
 #include "dev_traverse.h"
 #include "mrt_internal.hpp"
@@ -629,7 +631,8 @@ __global__ __launch_bounds__(kBlock) void kNrcTargets(const float4* record, cons
 template <int MODE, bool NRC = false>
 __global__ __launch_bounds__(kBlock, 3) void kShade(FrameParams fp, SceneView sv, PathView pv, QueueView in,
                                                  const float4* hits, QueueView out, ShadowView sh, ShadowView she,
-                                                 uint32_t* counters, int inSlot, int outSlot, NrcView nv) {
+                                                 uint32_t* counters, int inSlot, int outSlot, NrcView nv,
+                                                 const uint32_t* __restrict__ order) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t n = counters[inSlot];
     bool cont = false, shadowEmit = false, envEmit = false;
@@ -639,10 +642,11 @@ __global__ __launch_bounds__(kBlock, 3) void kShade(FrameParams fp, SceneView sv
     uint32_t q = 0;
     if (i < n) do {
         const SceneData& sd = fp.sd;
-        float4 ro = in.o[i], rdir = in.d[i];
+        const uint32_t j = order ? order[i] : i;  // shading order (sorted by material class) or queue order
+        float4 ro = in.o[j], rdir = in.d[j];
         q = __float_as_uint(ro.w);
         F3 o = xyz(ro), d = xyz(rdir);
-        float4 hh = hits[i];
+        float4 hh = hits[j];
         Hit h{hh.x, __float_as_uint(hh.y), hh.z, hh.w};
         // depth 0 of a lean frame: the path state is the one kGenerate would have written
         const bool first = fp.leanStart != 0u && fp.depth == 0u;
@@ -1567,6 +1571,63 @@ __global__ __launch_bounds__(kBlock) void kBuildShadeTris(SceneView sv, uint32_t
     if (i < n) out[i] = buildShadeTri(sv, i);
 }
 
+// Material class of every triangle for the shading sort: 1 if its material
+// transmits (the dielectric lobes), else 0.
+__global__ __launch_bounds__(kBlock) void kBuildTriClass(SceneView sv, uint32_t n, uint8_t* cls) {
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const Material& m = sv.materials[sv.meshes[sv.triMesh[i]].materialIndex];
+    cls[i] = m.transmission > 0.0f ? 1u : 0u;
+}
+
+// Sorting the hits of a depth by material class before shading (waves then
+// run one family of BSDF lobes instead of both): class of every queued ray
+// (misses count as class 0) and the number of class-0 rays of every block.
+__global__ __launch_bounds__(kBlock) void kClassCount(const uint32_t* count, const float4* hits, const uint8_t* triClass,
+                                                     uint8_t* key, uint32_t* blockCount0) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t n = *count;
+    uint32_t c = 1u;
+    if (i < n) {
+        const uint32_t tri = __float_as_uint(hits[i].y);
+        c = tri == kNoHit ? 0u : triClass[tri];
+        key[i] = (uint8_t)c;
+    }
+    const int zeros = __syncthreads_count(i < n && c == 0u);
+    if (threadIdx.x == 0) blockCount0[blockIdx.x] = (uint32_t)zeros;
+}
+
+// Stable partition: class-0 rays first, then class 1, each in queue order.
+// offset0 = exclusive scan of blockCount0; every block but the last is full.
+__global__ __launch_bounds__(kBlock) void kClassScatter(const uint32_t* count, const uint8_t* key,
+                                                       const uint32_t* blockCount0, const uint32_t* offset0,
+                                                       uint32_t blocks, uint32_t* perm) {
+    __shared__ uint32_t waveZeros[kBlock / 64];
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t n = *count;
+    const uint32_t blockStart = blockIdx.x * kBlock;
+    if (blockStart >= n) return;  // uniform per block
+    const bool valid = i < n;
+    const uint32_t c = valid ? key[i] : 1u;
+    const unsigned long long zmask = __ballot(valid && c == 0u);
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    if (lane == 0) waveZeros[wave] = (uint32_t)__popcll(zmask);
+    __syncthreads();
+    uint32_t before = 0;
+    for (uint32_t w = 0; w < wave; w++) before += waveZeros[w];
+    const uint32_t rank0 = before + (uint32_t)__popcll(zmask & ((1ull << lane) - 1ull));
+    if (!valid) return;
+    const uint32_t total0 = offset0[blocks - 1] + blockCount0[blocks - 1];
+    uint32_t pos;
+    if (c == 0u) {
+        pos = offset0[blockIdx.x] + rank0;
+    } else {
+        const uint32_t rank1 = (i - blockStart) - rank0;  // class-1 rays before i in this block
+        pos = total0 + (blockStart - offset0[blockIdx.x]) + rank1;
+    }
+    perm[pos] = i;
+}
+
 // ------------------------------------------------------------- host side --
 static uint32_t blocksFor(uint64_t n) { return (uint32_t)((n + kBlock - 1) / kBlock); }
 
@@ -1582,6 +1643,20 @@ bool ensureShadeTris(mrt_context* ctx) {
         if (!check(hipGetLastError(), "shading records")) return false;
     }
     ctx->shadeTrisDirty = false;
+    return true;
+}
+
+// Per-triangle material classes, rebuilt after a geometry, mesh or material upload.
+bool ensureTriClass(mrt_context* ctx) {
+    if (!ctx->triClassDirty) return true;
+    uint32_t n = ctx->triCount;
+    if (n > 0) {
+        if (!ctx->triClass.reserve(n)) return false;
+        hipLaunchKernelGGL(kBuildTriClass, dim3(blocksFor(n)), dim3(kBlock), 0, ctx->stream, ctx->view(), n,
+                           ctx->triClass.ptr);
+        if (!check(hipGetLastError(), "material classes")) return false;
+    }
+    ctx->triClassDirty = false;
     return true;
 }
 
@@ -1657,7 +1732,7 @@ static void queueViews(mrt_context* ctx, QueueView qv[2]) {
 
 template <int MODE>
 static bool launchChunk(mrt_context* ctx, FrameParams fp, StageEvents& ev, bool countTraversal,
-                        const NrcRun* nrc = nullptr, bool keepSampleRad = false) {
+                        const NrcRun* nrc = nullptr, bool keepSampleRad = false, bool sortShading = false) {
     hipStream_t s = ctx->stream;
     PathBuffers& b = ctx->paths;
     PathView pv = pathView(ctx);
@@ -1693,12 +1768,26 @@ static bool launchChunk(mrt_context* ctx, FrameParams fp, StageEvents& ev, bool 
                                b.hits.ptr, cnt, cur, ctx->traversalStack.ptr);
         if (timed) ev.end(s);
         timed = ev.begin(s, 1);
+        const uint32_t* order = nullptr;
+        if (sortShading) {
+            const uint32_t nb = blocksFor(fp.count);
+            hipLaunchKernelGGL(kClassCount, dim3(nb), dim3(kBlock), 0, s, cnt + cur, b.hits.ptr, ctx->triClass.ptr,
+                               b.sortKey.ptr, b.sortCount.ptr);
+            size_t bytes = b.sortScratch.capacity;
+            if (!check(rocprim::exclusive_scan(b.sortScratch.ptr, bytes, b.sortCount.ptr, b.sortOffset.ptr, 0u, nb,
+                                               rocprim::plus<uint32_t>(), s),
+                       "shading sort scan"))
+                return false;
+            hipLaunchKernelGGL(kClassScatter, dim3(nb), dim3(kBlock), 0, s, cnt + cur, b.sortKey.ptr, b.sortCount.ptr,
+                               b.sortOffset.ptr, nb, b.sortPerm.ptr);
+            order = b.sortPerm.ptr;
+        }
         if (MODE == kModeRgb && nrc)
             hipLaunchKernelGGL(HIP_KERNEL_NAME(kShade<kModeRgb, true>), dim3(blocksFor(fp.count)), dim3(kBlock), 0, s,
-                               fp, sv, pv, qv[cur], b.hits.ptr, qv[nxt], shv, shev, cnt, cur, nxt, nrc->view);
+                               fp, sv, pv, qv[cur], b.hits.ptr, qv[nxt], shv, shev, cnt, cur, nxt, nrc->view, order);
         else
             hipLaunchKernelGGL(HIP_KERNEL_NAME(kShade<MODE, false>), dim3(blocksFor(fp.count)), dim3(kBlock), 0, s, fp,
-                               sv, pv, qv[cur], b.hits.ptr, qv[nxt], shv, shev, cnt, cur, nxt, NrcView{});
+                               sv, pv, qv[cur], b.hits.ptr, qv[nxt], shv, shev, cnt, cur, nxt, NrcView{}, order);
         if (timed) ev.end(s);
         if (fp.neeEnabled || fp.envNee) {  // otherwise no shadow record can exist
             hipLaunchKernelGGL(kPrepShadow, dim3(1), dim3(1), 0, s, cnt);
@@ -1780,6 +1869,7 @@ struct FrameSetup {
     int mode = kModeRgb;
     uint32_t renderMode = 0, vw = 0, vh = 0;
     uint32_t nrcQueryCapacity = 0, nrcRecordCapacity = 0;
+    bool sortShading = false;
 };
 
 static int setupFrame(mrt_context* ctx, const SceneData* scene, const mrt_render_params* prm, FrameSetup& fs) {
@@ -1828,6 +1918,19 @@ static int setupFrame(mrt_context* ctx, const SceneData* scene, const mrt_render
         }
         chunk = totalPaths + prm->nrcTrainingPaths;
         if (!nrcEnsure(ctx, 0)) return -3;
+    }
+    fs.sortShading = prm->materialSort != 0u && ctx->triCount > 0;
+    if (fs.sortShading && chunk > 0) {
+        PathBuffers& b = ctx->paths;
+        const uint32_t nb = blocksFor(chunk);
+        size_t scratch = 0;
+        if (!ensureTriClass(ctx) || !b.sortKey.reserve(chunk) || !b.sortPerm.reserve(chunk) ||
+            !b.sortCount.reserve(nb) || !b.sortOffset.reserve(nb) ||
+            !check(rocprim::exclusive_scan(nullptr, scratch, b.sortCount.ptr, b.sortOffset.ptr, 0u, nb,
+                                           rocprim::plus<uint32_t>(), ctx->stream),
+                   "shading sort scan size") ||
+            !b.sortScratch.reserve(scratch))
+            return -3;
     }
     FrameParams& fp = fs.fp;
     memset(&fp, 0, sizeof(fp));
@@ -1917,9 +2020,11 @@ extern "C" int mrt_render_frame(mrt_context* ctx, const SceneData* scene, const 
         fp.pathBegin = begin;
         fp.count = (uint32_t)std::min<uint64_t>(chunk, totalPaths - begin);
         if (fp.nrc) fp.count = (uint32_t)chunk;  // image paths and training paths in one chunk
-        bool ok = mode == kModeRgb    ? launchChunk<kModeRgb>(ctx, fp, ev, countTraversal, fp.nrc ? &nrcRun : nullptr)
-                  : mode == kModeHero ? launchChunk<kModeHero>(ctx, fp, ev, countTraversal)
-                                      : launchChunk<kModeSingle>(ctx, fp, ev, countTraversal);
+        const bool sorted = fs.sortShading;
+        bool ok = mode == kModeRgb
+                      ? launchChunk<kModeRgb>(ctx, fp, ev, countTraversal, fp.nrc ? &nrcRun : nullptr, false, sorted)
+                  : mode == kModeHero ? launchChunk<kModeHero>(ctx, fp, ev, countTraversal, nullptr, false, sorted)
+                                      : launchChunk<kModeSingle>(ctx, fp, ev, countTraversal, nullptr, false, sorted);
         if (!ok) return -4;
         chunks++;
     }

@@ -40,6 +40,7 @@ VKRT_Result VKRT_create(VKRT** outVkrt) {
     vkrt->accumulationNeedsReset = 1;
     vkrt->rngPolicy = VKRT_RNG_HASH;
     VKRT_defaultNeuralCacheSettings(&vkrt->nrc);
+    vkrt->materialSort = 0;
     vkrt->renderStatus.renderPhase = VKRT_RENDER_PHASE_INACTIVE;
     *outVkrt = vkrt;
     return VKRT_SUCCESS;
@@ -220,6 +221,7 @@ static void fillRenderParams(const VKRT* vkrt, mrt_render_params* rp) {
     rp->envImportance = vkrt->envImportance;
     rp->hasTransmissive = vkrt->packed.hasTransmissive;
     rp->traversalStatistics = vkrt->traversalStatistics;
+    rp->materialSort = vkrt->materialSort;
     if (vkrt->nrc.enabled && vkrt->settings.renderMode == VKRT_RENDER_MODE_RGB) {
         rp->nrcEnabled = 1u;
         rp->nrcPrefix = vkrt->nrc.prefixBounces;
@@ -385,6 +387,12 @@ VKRT_Result VKRT_deviceSynchronize(VKRT* vkrt) {
     if (!vkrt) return VKRT_ERROR_INVALID_ARGUMENT;
     if (!vkrt->device) return VKRT_ERROR_INITIALIZATION_FAILED;
     return mrt_device_synchronize(vkrt->device) == 0 ? VKRT_SUCCESS : deviceError();
+}
+
+VKRT_Result VKRT_setMaterialSorting(VKRT* vkrt, uint8_t enabled) {
+    if (!vkrt) return VKRT_ERROR_INVALID_ARGUMENT;
+    vkrt->materialSort = enabled ? 1u : 0u;
+    return VKRT_SUCCESS;
 }
 
 VKRT_Result VKRT_setTraversalStatistics(VKRT* vkrt, uint8_t enabled) {

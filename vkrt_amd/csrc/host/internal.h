@@ -116,6 +116,7 @@ struct VKRT {
     VKRT_NeuralCacheSettings nrc;
     VKRT_NeuralCacheStats nrcStats;
     float sceneMin[3], sceneMax[3]; /* box of the uploaded geometry */
+    uint8_t materialSort;
 };
 
 /* scene preparation (scene_prep.c) */

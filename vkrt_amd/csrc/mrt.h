@@ -98,6 +98,7 @@ typedef struct mrt_render_params {
     float nrcLearningRate;
     float sceneMin[3]; /* scene box, for the cache's position input */
     float sceneMax[3];
+    uint32_t materialSort; /* shade each depth's hits grouped by material class */
 } mrt_render_params;
 
 typedef struct mrt_render_stats {
