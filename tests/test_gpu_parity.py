@@ -202,3 +202,44 @@ def test_material_sorted_shading_gives_identical_images():
         imgs.append(r.accumulation())
         r.close()
     assert np.array_equal(imgs[0], imgs[1])
+
+
+@pytest.fixture(scope="module")
+def spectral_workload_scene():
+    """The 1080p spectral workload's scene: 1,048,820 displaced triangles, 64
+    materials, 2048x1024 sky with the 1e4 sun."""
+    s = scenes.displaced_sphere(229)
+    s.environment = scenes.sky_environment(2048, 1024)
+    return s
+
+
+def test_primary_hit_ids_spectral_workload_scene(spectral_workload_scene):
+    w, h = 480, 270  # the workload's 16:9 view at a quarter of its resolution
+    r = make(spectral_workload_scene, w, h, 1, mode=V.RENDER_MODE_SPECTRAL, depth=12)
+    gm, gp = r.primary_hits(0)
+    om, op = O.OracleScene(r.export(), use_bvh=True).primary_hits(w, h, 0)
+    hit = om != V.VKRT_INVALID_INDEX
+    agree = np.mean((gm == om) & (gp == op))
+    print(f"primary-hit agreement {agree:.6f} over {hit.sum()} hits of {r.stats().triangleCount} triangles")
+    assert r.stats().triangleCount == 1048820
+    assert hit.sum() > 0.2 * w * h
+    assert agree >= 0.9999
+    r.close()
+
+
+def test_spectral_workload_same_seed_parity(spectral_workload_scene):
+    """Hero-wavelength spectral mode, depth 12, on the full 1M-triangle scene:
+    GPU and FP32 oracle (its own binned-SAH BVH) on the same streams."""
+    w, h = 192, 108
+    r = make(spectral_workload_scene, w, h, 4, mode=V.RENDER_MODE_SPECTRAL, depth=12)
+    orc = O.OracleScene(r.export(), use_bvh=True)
+    r.draw()
+    gpu = r.accumulation()
+    ref, bounces = orc.render(w, h, precision=32)
+    err = O.relmse(gpu[..., :3], ref[..., :3])
+    close = np.mean(np.abs(gpu[..., :3] - ref[..., :3]) <= 1e-3 * (1.0 + np.abs(ref[..., :3])))
+    print(f"relMSE {err:.3e}, pixels within 1e-3: {close:.5f}, bounces gpu {r.stats().extensionRays} oracle {bounces}")
+    assert np.isfinite(gpu).all()
+    assert close > 0.97
+    assert err < 1e-3
+    r.close()
