@@ -37,3 +37,26 @@ def test_converged_cornell_matches_fp64_oracle(name, mode):
     print(f"{name}: relMSE(GPU 65536 spp, FP64 oracle 65536 spp) = {err:.3e}")
     assert err < 1e-4
     r.close()
+
+
+def test_converged_spectral_workload_scene_matches_fp64_oracle():
+    """The 1,048,820-triangle spectral scene (sky without the sun disc), 32x18
+    pixels of its view, hero spectral, depth 12, 65,536 spp on the GPU with an
+    independent seed against the FP64 oracle's image (tests/golden/
+    make_converged_fixtures.py)."""
+    ref = np.load(os.path.join(GOLDEN, "sphere_sky_hero_fp64.npy"))
+    h, w = ref.shape[:2]
+    s = scenes.displaced_sphere(229)
+    s.environment = scenes.sky_environment(2048, 1024, sun_radiance=0.0)
+    r = V.Renderer(w, h)
+    scenes.load(r, s)
+    r.configure(spp=65536, max_depth=12, min_depth=4, mode=V.RENDER_MODE_SPECTRAL, spectral=V.SPECTRAL_HERO,
+                rng=V.RNG_PCG32, seed=7)
+    r.start(w, h, 0)
+    r.draw()
+    gpu = r.accumulation()
+    assert np.all(gpu[..., 3] == 65536)
+    err = O.relmse(gpu[..., :3], ref)
+    print(f"sphere_sky_hero_fp64.npy: relMSE(GPU 65536 spp, FP64 oracle 65536 spp) = {err:.3e}")
+    assert err < 1e-4
+    r.close()
