@@ -82,25 +82,37 @@ VKRT_Result VKRT_getTextureCount(const VKRT* vkrt, uint32_t* outTextureCount);  
 VKRT_Result VKRT_addTextureFromPixels(VKRT* vkrt, const VKRT_TextureUpload* upload,
                                       uint32_t* outTextureIndex);                                     /* vkrt.h:79 */
 VKRT_Result VKRT_removeTexture(VKRT* vkrt, uint32_t textureIndex);                                  /* vkrt.h:87 */
+VKRT_Result VKRT_addTexturesBatch(VKRT* vkrt, const VKRT_TextureUpload* uploads, size_t uploadCount,
+                                  uint32_t* outTextureIndices);                                      /* vkrt.h:88 */
 VKRT_Result VKRT_addMaterial(VKRT* vkrt, const Material* material, const char* name,
                              uint32_t* outMaterialIndex);                                             /* vkrt.h:95 */
+VKRT_Result VKRT_removeMaterial(VKRT* vkrt, uint32_t materialIndex);                                 /* vkrt.h:96 */
+VKRT_Result VKRT_setMaterialName(VKRT* vkrt, uint32_t materialIndex, const char* name);              /* vkrt.h:97 */
 VKRT_Result VKRT_setMaterial(VKRT* vkrt, uint32_t materialIndex, const Material* material);           /* vkrt.h:98 */
 VKRT_Result VKRT_setMeshMaterialIndex(VKRT* vkrt, uint32_t meshIndex, uint32_t materialIndex);       /* vkrt.h:99 */
+VKRT_Result VKRT_clearMeshMaterialAssignment(VKRT* vkrt, uint32_t meshIndex);                        /* vkrt.h:100 */
+VKRT_Result VKRT_setMeshOpacity(VKRT* vkrt, uint32_t meshIndex, float opacity);                      /* vkrt.h:101 */
 VKRT_Result VKRT_setMeshName(VKRT* vkrt, uint32_t meshIndex, const char* name);                      /* vkrt.h:102 */
 VKRT_Result VKRT_setMeshTransform(VKRT* vkrt, uint32_t meshIndex, const float position[3], const float rotation[3],
                                   const float scale[3]);                                             /* vkrt.h:103 */
 VKRT_Result VKRT_setMeshTransformMatrix(VKRT* vkrt, uint32_t meshIndex, float worldTransform[4][4]); /* vkrt.h:104 */
+/* Stored and reported as in the reference; the reference traces with no culling
+ * ray flag, so the setting does not change which triangles rays hit. */
+VKRT_Result VKRT_setMeshRenderBackfaces(VKRT* vkrt, uint32_t meshIndex, uint32_t enabled);           /* vkrt.h:105 */
 VKRT_Result VKRT_setRenderViewport(VKRT* vkrt, uint32_t x, uint32_t y, uint32_t width, uint32_t height); /* vkrt.h:110 */
 VKRT_Result VKRT_cameraSetPose(VKRT* vkrt, const float position[3], const float target[3], const float upVector[3],
                                float vfov);                                                          /* vkrt.h:111 */
 VKRT_Result VKRT_cameraGetPose(const VKRT* vkrt, float position[3], float target[3], float upVector[3],
                                float* vfov);                                                         /* vkrt.h:112 */
 
-/* transform helpers: vkrt.h:114-116 */
+/* transform helpers: vkrt.h:114-117 */
 void VKRT_buildMeshTransformMatrix(const float position[3], const float rotationDegrees[3], const float scale[3],
                                    float outMatrix[4][4]);                                           /* vkrt.h:114 */
+void VKRT_buildImportedNodeTransform(float worldTransform[4][4], float outEngineTransform[4][4]);   /* vkrt.h:115 */
 void VKRT_decomposeMeshTransform(float worldTransform[4][4], float outPosition[3], float outRotation[3],
                                  float outScale[3]);                                                 /* vkrt.h:116 */
+void VKRT_decomposeMeshNodeTransform(float worldTransform[4][4], float outPosition[3], float outRotation[3],
+                                     float outScale[3]);                                             /* vkrt.h:117 */
 
 #ifdef __cplusplus
 }

@@ -24,7 +24,8 @@ def declared_functions():
         text = open(os.path.join(INCLUDE, fn)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
         text = re.sub(r"//[^\n]*", "", text)
-        for m in re.finditer(r"^([A-Za-z_][\w\s\*]*?)\b(VKRT_\w+)\s*\(", text, flags=re.M):
+        pattern = r"^([A-Za-z_][\w\s\*]*?)\b(VKRT_\w+|meshLoadFromFile|meshReleaseImportData)\s*\("
+        for m in re.finditer(pattern, text, flags=re.M):
             if "static" not in m.group(1):  # header-only inline helpers are not exported
                 names.add(m.group(2))
     return sorted(names)
@@ -32,7 +33,8 @@ def declared_functions():
 
 def test_headers_declare_the_reference_trace_surface():
     names = declared_functions()
-    for required in ("VKRT_create", "VKRT_initWithCreateInfo", "VKRT_destroy", "VKRT_uploadMeshData",
+    for required in ("meshLoadFromFile", "meshReleaseImportData", "VKRT_importMesh", "VKRT_loadSceneFile",
+                     "VKRT_create", "VKRT_initWithCreateInfo", "VKRT_destroy", "VKRT_uploadMeshData",
                      "VKRT_uploadMeshDataBatch", "VKRT_addMaterial", "VKRT_setMaterial", "VKRT_setMeshMaterialIndex",
                      "VKRT_setMeshTransformMatrix", "VKRT_cameraSetPose", "VKRT_setRenderMode",
                      "VKRT_setSpectralSamplingMode", "VKRT_setPathDepth", "VKRT_setSamplesPerPixel",
@@ -56,7 +58,8 @@ def test_library_exports_every_declared_function():
 @pytest.mark.parametrize("compiler,flag", [("gcc", "-std=c11"), ("g++", "-std=c++17")])
 def test_public_headers_compile(tmp_path, compiler, flag):
     src = tmp_path / ("t.c" if compiler == "gcc" else "t.cpp")
-    src.write_text('#include "vkrt.h"\n#include "vkrt_mi355x.h"\nint main(void){ VKRT* v = 0; (void)v;'
+    src.write_text('#include "vkrt.h"\n#include "vkrt_mi355x.h"\n#include "vkrt_import.h"\n'
+                   'int main(void){ VKRT* v = 0; (void)v; MeshImportData d; (void)d;'
                    ' return (int)sizeof(SceneData) - 240; }\n')
     subprocess.run([compiler, flag, "-Wall", "-Werror", "-fsyntax-only", "-I", INCLUDE, str(src)], check=True)
 

@@ -353,6 +353,80 @@ class SceneBuffers(C.Structure):
     ]
 
 
+class MeshSnapshot(C.Structure):
+    _fields_ = [
+        ("info", MeshInfo),
+        ("material", Material),
+        ("materialIndex", C.c_uint32),
+        ("geometrySource", C.c_uint32),
+        ("hasMaterialAssignment", C.c_uint8),
+        ("ownsGeometry", C.c_uint8),
+        ("name", C.c_char * 256),
+        ("_tail", C.c_uint8 * 6),  # the C struct is padded to Material's 16-byte alignment
+    ]
+
+
+# include/vkrt_import.h (src/app/mesh/loader.h:5-55)
+class MaterialImportEntry(C.Structure):
+    # Material holds 16-byte aligned vec4 members, so it starts at offset 16
+    _fields_ = [("name", C.c_char_p), ("_pad", C.c_uint8 * 8), ("material", Material)]
+
+
+class TextureImportEntry(C.Structure):
+    _fields_ = [
+        ("name", C.c_char_p),
+        ("pixels", C.c_void_p),
+        ("width", C.c_uint32),
+        ("height", C.c_uint32),
+        ("format", C.c_uint32),
+        ("colorSpace", C.c_uint32),
+    ]
+
+
+class MeshImportEntry(C.Structure):
+    _fields_ = [
+        ("name", C.c_char_p),
+        ("vertices", C.POINTER(Vertex)),
+        ("indices", C.POINTER(C.c_uint32)),
+        ("vertexCount", C.c_size_t),
+        ("indexCount", C.c_size_t),
+        ("nodeIndex", C.c_uint32),
+        ("materialIndex", C.c_uint32),
+        ("renderBackfaces", C.c_uint8),
+        ("position", C.c_float * 3),
+        ("rotation", C.c_float * 3),
+        ("scale", C.c_float * 3),
+    ]
+
+
+class NodeImportEntry(C.Structure):
+    _fields_ = [
+        ("localTransform", C.c_float * 16),
+        ("name", C.c_char_p),
+        ("parentIndex", C.c_uint32),
+        ("meshEntryCount", C.c_uint32),
+        ("position", C.c_float * 3),
+        ("rotation", C.c_float * 3),
+        ("scale", C.c_float * 3),
+    ]
+
+
+class MeshImportData(C.Structure):
+    _fields_ = [
+        ("entries", C.POINTER(MeshImportEntry)),
+        ("count", C.c_uint32),
+        ("entryCapacity", C.c_uint32),
+        ("nodes", C.POINTER(NodeImportEntry)),
+        ("nodeCount", C.c_uint32),
+        ("nodeCapacity", C.c_uint32),
+        ("materials", C.POINTER(MaterialImportEntry)),
+        ("materialCount", C.c_uint32),
+        ("textures", C.POINTER(TextureImportEntry)),
+        ("textureCount", C.c_uint32),
+        ("textureCapacity", C.c_uint32),
+    ]
+
+
 _EXPECTED_SIZES = {
     Vertex: 80,
     ShaderVertex: 48,
@@ -364,6 +438,12 @@ _EXPECTED_SIZES = {
     MaterialSnapshot: 544,
     NeuralCacheSettings: 28,
     NeuralCacheStats: 20,
+    MeshSnapshot: 624,
+    MaterialImportEntry: 288,
+    TextureImportEntry: 32,
+    MeshImportEntry: 88,
+    NodeImportEntry: 120,
+    MeshImportData: 64,
 }
 for _cls, _size in _EXPECTED_SIZES.items():
     assert C.sizeof(_cls) == _size, f"{_cls.__name__} is {C.sizeof(_cls)} bytes, expected {_size}"
@@ -459,6 +539,20 @@ def library() -> C.CDLL:
         "VKRT_deviceSynchronize": (C.c_int, [P]),
         "VKRT_renderBackward": (C.c_int, [P, fp, C.POINTER(Gradients)]),
         "VKRT_getMaterialSnapshot": (C.c_int, [P, u32, C.POINTER(MaterialSnapshot)]),
+        "VKRT_getMeshSnapshot": (C.c_int, [P, u32, C.POINTER(MeshSnapshot)]),
+        "VKRT_setMeshName": (C.c_int, [P, u32, C.c_char_p]),
+        "VKRT_removeMaterial": (C.c_int, [P, u32]),
+        "VKRT_setMaterialName": (C.c_int, [P, u32, C.c_char_p]),
+        "VKRT_clearMeshMaterialAssignment": (C.c_int, [P, u32]),
+        "VKRT_setMeshOpacity": (C.c_int, [P, u32, f32]),
+        "VKRT_setMeshRenderBackfaces": (C.c_int, [P, u32, u32]),
+        "VKRT_addTexturesBatch": (C.c_int, [P, C.POINTER(TextureUpload), sz, up]),
+        "VKRT_buildImportedNodeTransform": (None, [fp, fp]),
+        "VKRT_decomposeMeshNodeTransform": (None, [fp, fp, fp, fp]),
+        "meshLoadFromFile": (C.c_int, [C.c_char_p, C.POINTER(MeshImportData)]),
+        "meshReleaseImportData": (None, [C.POINTER(MeshImportData)]),
+        "VKRT_importMesh": (C.c_int, [P, C.c_char_p, C.c_char_p, up, up]),
+        "VKRT_loadSceneFile": (C.c_int, [P, C.c_char_p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -466,6 +560,40 @@ def library() -> C.CDLL:
         fn.argtypes = args
     _lib = lib
     return lib
+
+
+def load_gltf(path: str) -> dict:
+    """meshLoadFromFile through the C ABI, copied into numpy arrays.
+
+    Returns {"entries": [{"name", "vertices" (N x 20 float32, Vertex layout),
+    "indices", "node", "material", "renderBackfaces"}], "nodes": [{"name",
+    "parent", "local" (4 x 4, column-major rows = columns)}], "materials":
+    [{"name", "material"}]}. Raises VkrtError if the file cannot be imported.
+    """
+    lib = library()
+    data = MeshImportData()
+    if lib.meshLoadFromFile(os.fsencode(path), C.byref(data)) != 0:
+        raise VkrtError(f"meshLoadFromFile failed for {path}")
+    try:
+        entries = []
+        for i in range(data.count):
+            e = data.entries[i]
+            verts = np.ctypeslib.as_array(C.cast(e.vertices, C.POINTER(C.c_float)), (e.vertexCount * 20,))
+            idx = np.ctypeslib.as_array(e.indices, (e.indexCount,))
+            entries.append({"name": e.name.decode() if e.name else "", "vertices": verts.reshape(-1, 20).copy(),
+                            "indices": idx.copy(), "node": e.nodeIndex, "material": e.materialIndex,
+                            "renderBackfaces": bool(e.renderBackfaces)})
+        nodes = []
+        for i in range(data.nodeCount):
+            n = data.nodes[i]
+            nodes.append({"name": n.name.decode() if n.name else "", "parent": n.parentIndex,
+                          "local": np.array(n.localTransform[:], np.float32).reshape(4, 4),
+                          "meshEntryCount": n.meshEntryCount})
+        materials = [{"name": data.materials[i].name.decode(), "material": Material.from_buffer_copy(
+            data.materials[i].material)} for i in range(data.materialCount)]
+        return {"entries": entries, "nodes": nodes, "materials": materials, "textureCount": data.textureCount}
+    finally:
+        lib.meshReleaseImportData(C.byref(data))
 
 
 def fptr(a: np.ndarray):
@@ -603,6 +731,48 @@ class Renderer:
         c = C.c_uint32()
         _ok(self.lib.VKRT_getTextureCount(self.h, C.byref(c)), "VKRT_getTextureCount")
         return c.value
+
+    # scene input (include/vkrt_import.h)
+    def import_mesh(self, path: str, name: str | None = None) -> tuple[int, int]:
+        """VKRT_importMesh: a .gltf/.glb file into the scene; returns (first mesh, mesh count)."""
+        first, count = C.c_uint32(), C.c_uint32()
+        _ok(self.lib.VKRT_importMesh(self.h, os.fsencode(path), name.encode() if name else None, C.byref(first),
+                                     C.byref(count)), "VKRT_importMesh")
+        return first.value, count.value
+
+    def load_scene_file(self, path: str):
+        """VKRT_loadSceneFile: replaces the scene with a vkrt.scene document."""
+        _ok(self.lib.VKRT_loadSceneFile(self.h, os.fsencode(path)), "VKRT_loadSceneFile")
+
+    def mesh_count(self) -> int:
+        c = C.c_uint32()
+        _ok(self.lib.VKRT_getMeshCount(self.h, C.byref(c)), "VKRT_getMeshCount")
+        return c.value
+
+    def mesh(self, index: int) -> MeshSnapshot:
+        snap = MeshSnapshot()
+        _ok(self.lib.VKRT_getMeshSnapshot(self.h, index, C.byref(snap)), "VKRT_getMeshSnapshot")
+        return snap
+
+    def material_name(self, index: int) -> str:
+        snap = MaterialSnapshot()
+        _ok(self.lib.VKRT_getMaterialSnapshot(self.h, index, C.byref(snap)), "VKRT_getMaterialSnapshot")
+        return snap.name.decode()
+
+    def remove_material(self, index: int):
+        _ok(self.lib.VKRT_removeMaterial(self.h, index), "VKRT_removeMaterial")
+
+    def set_material_name(self, index: int, name: str):
+        _ok(self.lib.VKRT_setMaterialName(self.h, index, name.encode()), "VKRT_setMaterialName")
+
+    def clear_mesh_material(self, mesh: int):
+        _ok(self.lib.VKRT_clearMeshMaterialAssignment(self.h, mesh), "VKRT_clearMeshMaterialAssignment")
+
+    def set_mesh_opacity(self, mesh: int, opacity: float):
+        _ok(self.lib.VKRT_setMeshOpacity(self.h, mesh, opacity), "VKRT_setMeshOpacity")
+
+    def set_mesh_render_backfaces(self, mesh: int, enabled: bool):
+        _ok(self.lib.VKRT_setMeshRenderBackfaces(self.h, mesh, 1 if enabled else 0), "VKRT_setMeshRenderBackfaces")
 
     def scene_settings(self) -> SceneSettingsSnapshot:
         out = SceneSettingsSnapshot()

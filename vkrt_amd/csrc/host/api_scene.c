@@ -401,3 +401,114 @@ VKRT_Result VKRT_setEnvironmentTexture(VKRT* vkrt, uint32_t textureIndex) {
 }
 
 VKRT_Result VKRT_clearEnvironmentTexture(VKRT* vkrt) { return VKRT_setEnvironmentTexture(vkrt, VKRT_INVALID_INDEX); }
+
+/* src/core/api/texture.c:35-46: an empty batch is invalid; a batch is added
+ * whole or not at all. */
+VKRT_Result VKRT_addTexturesBatch(VKRT* vkrt, const VKRT_TextureUpload* uploads, size_t uploadCount,
+                                  uint32_t* outTextureIndices) {
+    if (!vkrt || !uploads || uploadCount == 0u) return VKRT_ERROR_INVALID_ARGUMENT;
+    uint32_t before = vkrt->textureCount;
+    for (size_t i = 0; i < uploadCount; i++) {
+        uint32_t index = VKRT_INVALID_INDEX;
+        VKRT_Result r = VKRT_addTextureFromPixels(vkrt, &uploads[i], &index);
+        if (r != VKRT_SUCCESS) {
+            while (vkrt->textureCount > before) VKRT_removeTexture(vkrt, vkrt->textureCount - 1u);
+            return r;
+        }
+        if (outTextureIndices) outTextureIndices[i] = index;
+    }
+    return VKRT_SUCCESS;
+}
+
+/* src/core/api/mesh.c:296-340,447-457: the default material (index 0) cannot
+ * be removed; meshes that used the removed material fall back to it, later
+ * materials move down one index, and textures only the removed material
+ * referenced are released. */
+VKRT_Result VKRT_removeMaterial(VKRT* vkrt, uint32_t materialIndex) {
+    if (!vkrt || materialIndex >= vkrt->materialCount) return VKRT_ERROR_INVALID_ARGUMENT;
+    if (materialIndex == 0u) return VKRT_ERROR_OPERATION_FAILED;
+    Material removed = vkrt->materials[materialIndex].material;
+    for (uint32_t i = 0; i < vkrt->meshCount; i++) {
+        HostMesh* mesh = &vkrt->meshes[i];
+        if (mesh->info.materialIndex == materialIndex) {
+            mesh->info.materialIndex = 0u;
+            mesh->hasMaterialAssignment = 0u;
+        } else if (mesh->info.materialIndex > materialIndex) {
+            mesh->info.materialIndex--;
+        }
+    }
+    memmove(&vkrt->materials[materialIndex], &vkrt->materials[materialIndex + 1u],
+            sizeof(HostMaterial) * (vkrt->materialCount - materialIndex - 1u));
+    vkrt->materialCount--;
+    vkrt->materialRevision++;
+    vkrtResetSceneData(vkrt);
+    uint32_t slots[4] = {removed.baseColorTextureIndex, removed.metallicRoughnessTextureIndex,
+                         removed.normalTextureIndex, removed.emissiveTextureIndex};
+    /* release in descending index order so earlier removals do not shift later ones */
+    for (int pass = 0; pass < 4; pass++) {
+        uint32_t best = VKRT_INVALID_INDEX;
+        int bestSlot = -1;
+        for (int k = 0; k < 4; k++) {
+            if (slots[k] == VKRT_INVALID_INDEX) continue;
+            if (best == VKRT_INVALID_INDEX || slots[k] > best) {
+                best = slots[k];
+                bestSlot = k;
+            }
+        }
+        if (bestSlot < 0) break;
+        for (int k = 0; k < 4; k++)
+            if (slots[k] == best) slots[k] = VKRT_INVALID_INDEX;
+        if (best >= vkrt->textureCount || best == vkrt->settings.environmentTextureIndex) continue;
+        /* VKRT_removeTexture refuses a texture another material still binds */
+        VKRT_removeTexture(vkrt, best);
+    }
+    return VKRT_SUCCESS;
+}
+
+/* src/core/api/mesh.c:287-294,459-473 */
+VKRT_Result VKRT_setMaterialName(VKRT* vkrt, uint32_t materialIndex, const char* name) {
+    if (!vkrt || !name || materialIndex >= vkrt->materialCount) return VKRT_ERROR_INVALID_ARGUMENT;
+    if (name[0])
+        snprintf(vkrt->materials[materialIndex].name, VKRT_NAME_LEN, "%s", name);
+    else
+        snprintf(vkrt->materials[materialIndex].name, VKRT_NAME_LEN, "Material %u", materialIndex);
+    return VKRT_SUCCESS;
+}
+
+/* src/core/api/mesh.c:592-621: the mesh goes back to the default material. */
+VKRT_Result VKRT_clearMeshMaterialAssignment(VKRT* vkrt, uint32_t meshIndex) {
+    if (!vkrt || meshIndex >= vkrt->meshCount) return VKRT_ERROR_INVALID_ARGUMENT;
+    VKRT_Result r = ensureDefaultMaterial(vkrt);
+    if (r != VKRT_SUCCESS) return r;
+    HostMesh* mesh = &vkrt->meshes[meshIndex];
+    if (!mesh->hasMaterialAssignment && mesh->info.materialIndex == 0u) return VKRT_SUCCESS;
+    mesh->info.materialIndex = 0u;
+    mesh->hasMaterialAssignment = 0u;
+    vkrt->materialRevision++;
+    vkrtResetSceneData(vkrt);
+    return VKRT_SUCCESS;
+}
+
+/* src/core/api/mesh.c:270-272,623-638: opacity must be finite and in [0, 1]. */
+VKRT_Result VKRT_setMeshOpacity(VKRT* vkrt, uint32_t meshIndex, float opacity) {
+    if (!vkrt || meshIndex >= vkrt->meshCount || !isfinite(opacity) || opacity < 0.0f || opacity > 1.0f)
+        return VKRT_ERROR_INVALID_ARGUMENT;
+    MeshInfo* info = &vkrt->meshes[meshIndex].info;
+    if (info->opacity == opacity) return VKRT_SUCCESS;
+    info->opacity = opacity;
+    vkrt->materialRevision++; /* light-sampling eligibility depends on it */
+    vkrtResetSceneData(vkrt);
+    return VKRT_SUCCESS;
+}
+
+/* src/core/api/mesh.c:646-663 */
+VKRT_Result VKRT_setMeshRenderBackfaces(VKRT* vkrt, uint32_t meshIndex, uint32_t enabled) {
+    if (!vkrt || meshIndex >= vkrt->meshCount) return VKRT_ERROR_INVALID_ARGUMENT;
+    MeshInfo* info = &vkrt->meshes[meshIndex].info;
+    uint32_t v = enabled ? 1u : 0u;
+    if (info->renderBackfaces == v) return VKRT_SUCCESS;
+    info->renderBackfaces = v;
+    vkrt->materialRevision++;
+    vkrtResetSceneData(vkrt);
+    return VKRT_SUCCESS;
+}

@@ -243,3 +243,30 @@ void VKRT_decomposeMeshTransform(float worldTransform[4][4], float outPosition[3
         }
     }
 }
+
+/* src/core/scene/transform.c:84-111: glTF nodes live in a +Y-up frame; the
+ * engine is +Z-up. The import basis B is a +90 degree rotation about X (built
+ * like any mesh transform: position 0, rotation (90, 0, 0), scale 1), and a
+ * node transform M becomes B * M * B^-1, so that composing converted node
+ * transforms equals converting the composed transform. */
+void VKRT_buildImportedNodeTransform(float worldTransform[4][4], float outEngineTransform[4][4]) {
+    if (!worldTransform || !outEngineTransform) return;
+    static const float zero[3] = {0.0f, 0.0f, 0.0f};
+    static const float basisRotation[3] = {90.0f, 0.0f, 0.0f};
+    static const float one[3] = {1.0f, 1.0f, 1.0f};
+    float basis[4][4], inverse[4][4], tmp[4][4];
+    VKRT_buildMeshTransformMatrix(zero, basisRotation, one, basis);
+    if (!mat4Inverse(basis, inverse)) mat4Identity(inverse);
+    mat4Mul(basis, worldTransform, tmp);
+    mat4Mul(tmp, inverse, outEngineTransform);
+}
+
+/* src/core/scene/transform.c:113-119 */
+void VKRT_decomposeMeshNodeTransform(float worldTransform[4][4], float outPosition[3], float outRotation[3],
+                                     float outScale[3]) {
+    if (!outPosition || !outRotation || !outScale) return;
+    float engine[4][4];
+    mat4Identity(engine);
+    VKRT_buildImportedNodeTransform(worldTransform, engine);
+    VKRT_decomposeMeshTransform(engine, outPosition, outRotation, outScale);
+}
