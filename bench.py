@@ -284,13 +284,15 @@ def nrc_variant(frames: int, steps: int = 1, batch: int = 16384):
         th = math.radians(deg)
         return ((0.5 + radius * math.sin(th), 0.5 - radius * math.cos(th), 0.5), (0.5, 0.5, 0.5), (0.0, 0.0, 1.0), 40.0)
 
+    settings = []
+
     def make(spp, cache):
         r = V.Renderer(w, h)
         scenes.load(r, scenes.cornell_box())
         r.configure(spp=spp, max_depth=8, min_depth=MIN_DEPTH, mode=V.RENDER_MODE_RGB, rng=V.RNG_PCG32, seed=3)
         if cache:
             r.nrc_reset(1)
-            r.set_neural_cache(True, trainingSteps=steps, batchSize=batch)
+            settings.append(r.set_neural_cache(True, trainingSteps=steps, batchSize=batch))
         return r
 
     def relmse(x, ref):
@@ -353,6 +355,7 @@ def nrc_variant(frames: int, steps: int = 1, batch: int = 16384):
     for (f, pms, perr), (_, nms, nerr) in zip(plain_marks, nrc_marks):
         log(f"NRC equal time: {f:3d} frames plain {pms:8.1f} ms relMSE {perr:.4e} | cached {nms:8.1f} ms relMSE {nerr:.4e}")
     log(f"NRC: time to reach the plain tracer's 64-frame relMSE, cached / plain = {ratio}")
+    cs = settings[0]
     return {"width": w, "height": h, "spp_per_frame": 1, "orbit_frames": frames, "orbit_ms_per_frame": round(orbit_ms, 3),
             "reference_spp": ref_spp * ref_draws, "infer_ms": round(st.inferMs, 3), "train_ms": round(st.trainMs, 3),
             "queries_per_frame": st.queries, "training_records_per_frame": st.trainingRecords,
@@ -360,8 +363,9 @@ def nrc_variant(frames: int, steps: int = 1, batch: int = 16384):
                              "cached_relmse": nerr}
                             for (f, pms, perr), (_, nms, nerr) in zip(plain_marks, nrc_marks)],
             "time_ratio": None if ratio is None else round(ratio, 3),
-            "scene": f"Cornell box (36 triangles), depth 8, RGB, PCG32 seed 3; cache after 2 bounces, 65536 training "
-                     f"paths and {steps} Adam steps of {batch} records per frame"}
+            "scene": f"Cornell box (36 triangles), depth 8, RGB, PCG32 seed 3; cache after {cs.prefixBounces} "
+                     f"bounce(s), {cs.trainingPaths} training paths and {steps} Adam step(s) of {batch} records at "
+                     f"learning rate {cs.learningRate:g} per frame"}
 
 
 def main():

@@ -117,11 +117,11 @@ VKRT_Result VKRT_setMaterialSorting(VKRT* vkrt, uint8_t enabled);
  * outputs; fp16 weights on MFMA with fp32 accumulation. */
 typedef struct VKRT_NeuralCacheSettings {
     uint32_t enabled;
-    uint32_t prefixBounces;   /* bounces traced before the cache is queried (2) */
+    uint32_t prefixBounces;   /* bounces traced before the cache is queried (1) */
     uint32_t trainingPaths;   /* full paths per frame that produce training records (65536) */
-    uint32_t trainingSteps;   /* optimiser steps per frame (4) */
+    uint32_t trainingSteps;   /* optimiser steps per frame (1) */
     uint32_t batchSize;       /* records per step (16384) */
-    float learningRate;       /* Adam step size (1e-3) */
+    float learningRate;       /* Adam step size (1e-2) */
     uint32_t train;           /* 0 keeps the weights fixed */
 } VKRT_NeuralCacheSettings;
 

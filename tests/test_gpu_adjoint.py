@@ -257,10 +257,17 @@ def test_torch_binding_gradients_equal_render_backward():
     r.set_camera(*view)
     r.start(size, size, 0)
     g = r.render_backward(w.numpy(), environment_shape=s.environment.shape[:2])
-    assert np.allclose(p.base_color.grad.numpy(), g["baseColor"], rtol=1e-5, atol=1e-7)
-    assert np.allclose(p.roughness.grad.numpy(), g["roughness"], rtol=1e-5, atol=1e-7)
-    assert np.allclose(p.emission.grad.numpy(), g["emission"], rtol=1e-5, atol=1e-7)
-    assert np.allclose(p.environment.grad.numpy(), g["environment"], rtol=1e-5, atol=1e-7)
+
+    def same(a, b):
+        # the two passes replay the same paths, but gradients are summed with
+        # float atomics whose order varies between launches: compare to the
+        # rounding of such sums, relative to the array's scale
+        return np.abs(a - b).max() <= 1e-4 * np.abs(b).max() + 1e-7
+
+    assert same(p.base_color.grad.numpy(), g["baseColor"])
+    assert same(p.roughness.grad.numpy(), g["roughness"])
+    assert same(p.emission.grad.numpy(), g["emission"])
+    assert same(p.environment.grad.numpy(), g["environment"])
     assert np.abs(g["environment"]).sum() > 0 and np.abs(g["roughness"]).sum() > 0
     r.close()
 

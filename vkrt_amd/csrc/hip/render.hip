@@ -470,6 +470,11 @@ __global__ __launch_bounds__(kTraceBlock) void kExtend(SceneView sv, const float
             hits[i] = make_float4(rd[i].w, __uint_as_float(kNoHit), 0.0f, 0.0f);
         return;
     }
+    // Wavefronts beyond one per 64 queued rays leave at once: every ray still
+    // has a wavefront to claim it (each wavefront drains all partitions), and a
+    // short queue does not pay kFetchParts contended claims in every resident
+    // wavefront (about 100 us per launch with the full grid).
+    if ((blockIdx.x * (kTraceBlock / 64u) + (threadIdx.x >> 6)) * 64ull >= n) return;
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t lanesBelow = (1ull << lane) - 1ull;
     const uint32_t partSize = (n + kFetchParts - 1u) / kFetchParts;
@@ -548,6 +553,12 @@ __global__ __launch_bounds__(kTraceBlock) void kShadow(SceneView sv, PathView pv
     st.dropped = 0;
     const uint32_t n = counters[countSlot];
     if (n == 0u) return;
+    // A wavefront claims batches of its block's partition only, and partition p
+    // holds batches p, p + kFetchParts, ...: keep the wavefronts whose rank within
+    // their partition is below that partition's batch count (rank 0 always), so
+    // short queues do not pay a contended claim in every resident wavefront.
+    const uint32_t rank = (blockIdx.x / kFetchParts) * (kTraceBlock / 64u) + (threadIdx.x >> 6);
+    if ((uint64_t)rank * kFetchBatch * kFetchParts >= n) return;
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t part = blockIdx.x % kFetchParts;
     while (true) {
