@@ -185,6 +185,8 @@ typedef struct VKRT_SceneBuffers {
     const float* environmentAliasQ;      /* width*height entries, NULL without an environment texture */
     const uint32_t* environmentAliasIdx;
     const float* environmentPdf;         /* texel selection probabilities */
+    const uint32_t* triangleAlpha;       /* per triangle: bit 31 = mesh runs the alpha test, bits 0-23 = the
+                                            three vertex colour alphas (8 bits each); NULL when all opaque */
 } VKRT_SceneBuffers;
 VKRT_Result VKRT_exportSceneBuffers(VKRT* vkrt, VKRT_SceneBuffers* out);
 

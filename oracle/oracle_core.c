@@ -246,7 +246,52 @@ static int slab(const ONode* n, v3 o, v3 inv, real tmax) {
     return t0 <= t1;
 }
 
-static void traceClosest(const Ctx* c, Ray ray, Hit* h) {
+/* Alpha test of a candidate hit (src/shaders/rt/alpha_test.slang:8-66) with the
+ * build's draw: one uniform number per (ray key, triangle) from the lowbias32
+ * mix, instead of the next number of the path's stream in the driver's
+ * any-hit order. Ray key = mix(mix(path index) ^ salt); salt from the frame
+ * number, the RNG seed, the depth and the ray kind (0 extension, 1 light
+ * shadow, 2 environment shadow, 3 query). Vertex-colour alpha is interpolated
+ * in float with weights (1-u-v, u, v); texture alpha is 1 (not sampled). */
+static uint32_t alphaSalt(uint32_t frame, uint64_t seed, uint32_t depth, uint32_t kind) {
+    return hash32(frame * 0x9E3779B9u ^ (uint32_t)seed ^ (uint32_t)(seed >> 32) * 0x85EBCA6Bu ^
+                  ((depth << 2) | kind) * 0xC2B2AE35u);
+}
+static uint32_t alphaRayKey(uint32_t salt, uint32_t pathIndex) { return hash32(hash32(pathIndex) ^ salt); }
+static uint32_t rayKey(const Ctx* c, uint32_t pathIndex, uint32_t depth, uint32_t kind) {
+    return alphaRayKey(alphaSalt(c->sd->frameNumber, c->b->rngSeed, depth, kind), pathIndex);
+}
+
+static int alphaAccept(const Ctx* c, uint32_t tri, real u, real v, uint32_t key) {
+    const uint32_t word = c->b->triangleAlpha[tri];
+    if (!(word >> 31)) return 1;
+    const MeshInfo* mesh = &c->b->meshInfos[c->b->triangleMesh[tri]];
+    const Material* m = &c->b->materials[mesh->materialIndex];
+    const float k = 1.0f / 255.0f;
+    const float fu = (float)u, fv = (float)v;
+    const float w = 1.0f - fu - fv;
+    const float ca = (float)(word & 0xffu) * k * w + (float)((word >> 8) & 0xffu) * k * fu +
+                     (float)((word >> 16) & 0xffu) * k * fv;
+    const float x = (float)(hash32(key ^ tri * 0x27D4EB2Fu) >> 8) * (1.0f / 16777216.0f);
+    if (m->alphaMode == VKRT_MATERIAL_ALPHA_MODE_MASK) {
+        if (ca < m->alphaCutoff) return 0;
+        float mo = mesh->opacity * m->opacity * ca;
+        mo = mo < 0.0f ? 0.0f : (mo > 1.0f ? 1.0f : mo);
+        if (mo >= 1.0f) return 1;
+        if (mo <= 0.0f) return 0;
+        return x <= mo;
+    }
+    float op = mesh->opacity * m->opacity * ca;
+    op = op < 0.0f ? 0.0f : (op > 1.0f ? 1.0f : op);
+    const int blend = m->alphaMode == VKRT_MATERIAL_ALPHA_MODE_BLEND || m->opacity < 0.999f || mesh->opacity < 0.999f;
+    if (!blend) return 1;
+    if (op <= 0.0f) return 0;
+    if (op >= 1.0f) return 1;
+    return x <= op;
+}
+#define ALPHA_OK(c, i, u, v, key) (!(c)->b->triangleAlpha || alphaAccept((c), (i), (u), (v), (key)))
+
+static void traceClosest(const Ctx* c, Ray ray, Hit* h, uint32_t key) {
     h->hit = 0;
     h->mesh = VKRT_INVALID_INDEX;
     h->prim = VKRT_INVALID_INDEX;
@@ -260,7 +305,8 @@ static void traceClosest(const Ctx* c, Ray ray, Hit* h) {
     if (!c->bvh) {
         for (uint32_t i = 0; i < c->b->triangleCount; i++) {
             real t, u, v;
-            if (intersectTri(&tris[(size_t)i * 9u], ray.o, &p, ray.tmin, best, &t, &u, &v)) {
+            if (intersectTri(&tris[(size_t)i * 9u], ray.o, &p, ray.tmin, best, &t, &u, &v) &&
+                ALPHA_OK(c, i, u, v, key)) {
                 best = t;
                 bestTri = i;
                 bu = u;
@@ -279,7 +325,8 @@ static void traceClosest(const Ctx* c, Ray ray, Hit* h) {
                 for (uint32_t k = 0; k < n->count; k++) {
                     uint32_t i = c->bvh->order[n->first + k];
                     real t, u, v;
-                    if (intersectTri(&tris[(size_t)i * 9u], ray.o, &p, ray.tmin, best, &t, &u, &v)) {
+                    if (intersectTri(&tris[(size_t)i * 9u], ray.o, &p, ray.tmin, best, &t, &u, &v) &&
+                        ALPHA_OK(c, i, u, v, key)) {
                         best = t;
                         bestTri = i;
                         bu = u;
@@ -309,13 +356,15 @@ static void traceClosest(const Ctx* c, Ray ray, Hit* h) {
 /* Any hit within (tmin, tmax): the reference's shadow TraceRay with
  * ACCEPT_FIRST_HIT_AND_END_SEARCH (src/shaders/rt/queries/shadow_query.slang:8-28).
  * Returns the global triangle index of the accepted occluder or UINT32_MAX. */
-static uint32_t traceAny(const Ctx* c, Ray ray) {
+static uint32_t traceAny(const Ctx* c, Ray ray, uint32_t key) {
     RayPre p = rayPrecompute(ray.d);
     const float* tris = c->b->worldTriangles;
     if (!c->bvh) {
         for (uint32_t i = 0; i < c->b->triangleCount; i++) {
             real t, u, v;
-            if (intersectTri(&tris[(size_t)i * 9u], ray.o, &p, ray.tmin, ray.tmax, &t, &u, &v)) return i;
+            if (intersectTri(&tris[(size_t)i * 9u], ray.o, &p, ray.tmin, ray.tmax, &t, &u, &v) &&
+                ALPHA_OK(c, i, u, v, key))
+                return i;
         }
         return UINT32_MAX;
     }
@@ -330,7 +379,9 @@ static uint32_t traceAny(const Ctx* c, Ray ray) {
             for (uint32_t k = 0; k < n->count; k++) {
                 uint32_t i = c->bvh->order[n->first + k];
                 real t, u, v;
-                if (intersectTri(&tris[(size_t)i * 9u], ray.o, &p, ray.tmin, ray.tmax, &t, &u, &v)) return i;
+                if (intersectTri(&tris[(size_t)i * 9u], ray.o, &p, ray.tmin, ray.tmax, &t, &u, &v) &&
+                    ALPHA_OK(c, i, u, v, key))
+                    return i;
             }
         } else {
             stack[sp++] = (uint32_t)n->left + 1u;

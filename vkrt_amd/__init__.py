@@ -350,6 +350,7 @@ class SceneBuffers(C.Structure):
         ("environmentAliasQ", C.POINTER(C.c_float)),
         ("environmentAliasIdx", C.POINTER(C.c_uint32)),
         ("environmentPdf", C.POINTER(C.c_float)),
+        ("triangleAlpha", C.POINTER(C.c_uint32)),
     ]
 
 

@@ -72,6 +72,7 @@ extern "C" void mrt_destroy(mrt_context* ctx) {
     ctx->indices.release();
     ctx->worldTris.release();
     ctx->triMesh.release();
+    ctx->triAlpha.release();
     ctx->meshTriBase.release();
     ctx->tris.release();
     ctx->nodes.release();
@@ -184,6 +185,16 @@ extern "C" int mrt_upload_materials(mrt_context* ctx, const Material* materials,
     if (!ctx->materials.upload(materials, materialCount, ctx->stream)) return -3;
     ctx->triClassDirty = true;
     return sync(ctx, "upload materials");
+}
+
+extern "C" int mrt_upload_alpha(mrt_context* ctx, const uint32_t* triangleAlpha, uint32_t triangleCount) {
+    if (!ctx) return -1;
+    if (!check(hipSetDevice(ctx->device), "hipSetDevice")) return -2;
+    ctx->alphaEnabled = false;
+    if (!triangleAlpha || triangleCount == 0) return 0;
+    if (!ctx->triAlpha.upload(triangleAlpha, triangleCount, ctx->stream)) return -3;
+    ctx->alphaEnabled = true;
+    return sync(ctx, "upload alpha");
 }
 
 extern "C" int mrt_upload_lights(mrt_context* ctx, const mrt_light_desc* l) {

@@ -154,6 +154,7 @@ struct SceneView {
     const float* envPdf;
     const float* rgb2spec;
     const ShadeTri* shadeTris;
+    const uint32_t* triAlpha;  // per-triangle alpha words; nullptr when every mesh is opaque
 };
 
 struct Hit {

@@ -18,6 +18,60 @@ constexpr int kTraceBlock = 256;
 // triangles needs far fewer; the cap only guarantees termination.
 constexpr uint32_t kMaxTraversalSteps = 1u << 22;
 
+// ------------------------------------------------------------ alpha test --
+// src/shaders/rt/alpha_test.slang:8-66. The reference's any-hit shaders draw
+// rand(rng) from the path's stream for every candidate hit the driver reports,
+// in the driver's traversal order. Here the draw is a hash of (ray key,
+// triangle): each (ray, triangle) pair gets one fixed uniform number, so the
+// closest accepted hit does not depend on traversal order and the oracle can
+// restate it, while every hit is still accepted with the reference's
+// probability. Ray key = mix(mix(path index) ^ salt), salt from the frame,
+// the RNG seed, the path depth and the ray kind (0 extension, 1 light shadow,
+// 2 environment shadow, 3 ray query).
+__host__ __device__ inline uint32_t alphaMix(uint32_t v) {
+    v ^= v >> 16;
+    v *= 0x7feb352du;
+    v ^= v >> 15;
+    v *= 0x846ca68bu;
+    v ^= v >> 16;
+    return v;
+}
+__host__ __device__ inline uint32_t alphaSalt(uint32_t frame, uint64_t seed, uint32_t depth, uint32_t kind) {
+    return alphaMix(frame * 0x9E3779B9u ^ (uint32_t)seed ^ (uint32_t)(seed >> 32) * 0x85EBCA6Bu ^
+                    ((depth << 2) | kind) * 0xC2B2AE35u);
+}
+__host__ __device__ inline uint32_t alphaRayKey(uint32_t salt, uint32_t pathIndex) {
+    return alphaMix(alphaMix(pathIndex) ^ salt);
+}
+
+// Whether a candidate hit at barycentrics (u, v) of global triangle `tri`
+// survives the alpha test. The alpha word holds the three vertex-colour alpha
+// bytes; textures are not sampled, so the base-colour texture's alpha is 1.
+DEV bool alphaAccept(const SceneView& s, uint32_t tri, float u, float v, uint32_t key) {
+    const uint32_t word = s.triAlpha[tri];
+    if (!(word >> 31)) return true;
+    const MeshInfo& mesh = s.meshes[s.triMesh[tri]];
+    const Material& m = s.materials[mesh.materialIndex];
+    const float k = 1.0f / 255.0f;
+    const float w = 1.0f - u - v;
+    const float ca = (float)(word & 0xffu) * k * w + (float)((word >> 8) & 0xffu) * k * u +
+                     (float)((word >> 16) & 0xffu) * k * v;
+    const float x = (float)(alphaMix(key ^ tri * 0x27D4EB2Fu) >> 8) * (1.0f / 16777216.0f);
+    if (m.alphaMode == VKRT_MATERIAL_ALPHA_MODE_MASK) {
+        if (ca < m.alphaCutoff) return false;
+        const float mo = fminf(fmaxf(mesh.opacity * m.opacity * ca, 0.0f), 1.0f);
+        if (mo >= 1.0f) return true;
+        if (mo <= 0.0f) return false;
+        return x <= mo;
+    }
+    const float op = fminf(fmaxf(mesh.opacity * m.opacity * ca, 0.0f), 1.0f);
+    const bool blend = m.alphaMode == VKRT_MATERIAL_ALPHA_MODE_BLEND || m.opacity < 0.999f || mesh.opacity < 0.999f;
+    if (!blend) return true;
+    if (op <= 0.0f) return false;
+    if (op >= 1.0f) return true;
+    return x <= op;
+}
+
 struct RayPre {
     int kx, ky, kz;
     float sx, sy, sz;
@@ -262,8 +316,10 @@ DEV void closestBegin(ClosestState& cs, TraversalStack& st, float tmax) {
 // nearest overlapping child is visited next and the others are pushed farthest
 // first; a popped entry whose entry distance lies beyond the closest hit found
 // so far is skipped. Returns false once the traversal has finished.
+// ALPHA = false compiles the alpha test out (scenes where every mesh is opaque).
+template <bool ALPHA = true>
 DEV bool closestStep(const SceneView& s, const TraversalRay& r, float tmin, ClosestState& cs, TraversalStack& st,
-                     uint32_t* nodeVisits = nullptr, uint32_t* triTests = nullptr) {
+                     uint32_t alphaKey, uint32_t* nodeVisits = nullptr, uint32_t* triTests = nullptr) {
     if (++cs.steps > kMaxTraversalSteps) {
         st.dropped++;
         return false;
@@ -293,8 +349,10 @@ DEV bool closestStep(const SceneView& s, const TraversalRay& r, float tmin, Clos
             const TriRecord tr = s.tris[first + i];
             float t, u, v;
             if (intersectTriangle(tr, r.o, r.pre, tmin, cs.h.t, t, u, v)) {
+                const uint32_t tri = __float_as_uint(tr.v0.w);
+                if (ALPHA && s.triAlpha && !alphaAccept(s, tri, u, v, alphaKey)) continue;
                 cs.h.t = t;
-                cs.h.tri = __float_as_uint(tr.v0.w);
+                cs.h.tri = tri;
                 cs.h.u = u;
                 cs.h.v = v;
             }
@@ -313,18 +371,19 @@ DEV bool closestStep(const SceneView& s, const TraversalRay& r, float tmin, Clos
 
 // Closest hit in (tmin, tmax).
 DEV Hit traceClosest(const SceneView& s, const TraversalRay& r, float tmin, float tmax, TraversalStack& st,
-                     uint32_t* nodeVisits = nullptr, uint32_t* triTests = nullptr) {
+                     uint32_t alphaKey, uint32_t* nodeVisits = nullptr, uint32_t* triTests = nullptr) {
     ClosestState cs;
     closestBegin(cs, st, tmax);
     if (s.nodeCount == 0) return cs.h;
-    while (closestStep(s, r, tmin, cs, st, nodeVisits, triTests)) {
+    while (closestStep<true>(s, r, tmin, cs, st, alphaKey, nodeVisits, triTests)) {
     }
     return cs.h;
 }
 
 // Any hit in (tmin, tmax): the first accepted intersection ends the search, as
 // RAY_FLAG_ACCEPT_FIRST_HIT_AND_END_SEARCH. Returns the global triangle id or kNoHit.
-DEV uint32_t traceAny(const SceneView& s, const TraversalRay& r, float tmin, float tmax, TraversalStack& st) {
+DEV uint32_t traceAny(const SceneView& s, const TraversalRay& r, float tmin, float tmax, TraversalStack& st,
+                      uint32_t alphaKey) {
     if (s.nodeCount == 0) return kNoHit;
     st.sp = 0;
     uint32_t ref = 0;
@@ -357,7 +416,10 @@ DEV uint32_t traceAny(const SceneView& s, const TraversalRay& r, float tmin, flo
             for (uint32_t i = 0; i < count; i++) {
                 const TriRecord tr = s.tris[first + i];
                 float t, u, v;
-                if (intersectTriangle(tr, r.o, r.pre, tmin, tmax, t, u, v)) return __float_as_uint(tr.v0.w);
+                if (intersectTriangle(tr, r.o, r.pre, tmin, tmax, t, u, v)) {
+                    const uint32_t tri = __float_as_uint(tr.v0.w);
+                    if (!s.triAlpha || alphaAccept(s, tri, u, v, alphaKey)) return tri;
+                }
             }
         }
         if (popNext) {

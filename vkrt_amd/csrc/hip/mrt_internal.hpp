@@ -85,6 +85,8 @@ struct mrt_context {
     mrt::DeviceArray<uint32_t> indices;
     mrt::DeviceArray<float> worldTris;
     mrt::DeviceArray<uint32_t> triMesh;
+    mrt::DeviceArray<uint32_t> triAlpha;  // per-triangle alpha words, valid when alphaEnabled
+    bool alphaEnabled = false;
     mrt::DeviceArray<uint32_t> meshTriBase;
     mrt::DeviceArray<mrt::TriRecord> tris;
     mrt::DeviceArray<mrt::BvhNode> nodes;
@@ -151,6 +153,7 @@ struct mrt_context {
         v.envPdf = envPdf.ptr;
         v.rgb2spec = rgb2spec.ptr;
         v.shadeTris = shadeTris.ptr;
+        v.triAlpha = alphaEnabled && triAlpha.count >= triCount ? triAlpha.ptr : nullptr;
         return v;
     }
 };

@@ -453,9 +453,11 @@ __global__ void kSetCount(uint32_t* counters, int slot, uint32_t value) { counte
 constexpr uint32_t kRefillLanes = 16;
 constexpr uint32_t kNoRay = 0xFFFFFFFFu;
 
-template <bool COUNT>
+// ALPHA: some mesh runs the alpha test (the opaque variant keeps six waves per SIMD).
+template <bool COUNT, bool ALPHA>
 __global__ __launch_bounds__(kTraceBlock) void kExtend(SceneView sv, const float4* ro, const float4* rd, float4* hits,
-                                                      uint32_t* counters, int countSlot, uint32_t* gstack) {
+                                                      uint32_t* counters, int countSlot, uint32_t* gstack,
+                                                      uint32_t alphaSalt, uint32_t pathBase) {
     uint32_t nodeVisits = 0, triTests = 0;
     __shared__ uint32_t lds[kLdsStackWords];
     TraversalStack st;
@@ -480,7 +482,7 @@ __global__ __launch_bounds__(kTraceBlock) void kExtend(SceneView sv, const float
     const uint32_t partSize = (n + kFetchParts - 1u) / kFetchParts;
     uint32_t part = blockIdx.x % kFetchParts;
     uint32_t partsTried = 0;  // partitions found exhausted by this wavefront
-    uint32_t rayId = kNoRay;
+    uint32_t rayId = kNoRay, alphaKey = 0u;
     TraversalRay r;
     ClosestState cs;
     cs.ref = 0;
@@ -503,6 +505,7 @@ __global__ __launch_bounds__(kTraceBlock) void kExtend(SceneView sv, const float
                     float4 a = ro[k], b = rd[k];
                     r = makeTraversalRay(xyz(a), xyz(b));
                     closestBegin(cs, st, b.w);
+                    if (ALPHA) alphaKey = alphaRayKey(alphaSalt, pathBase + __float_as_uint(a.w));
                     rayId = k;
                 }
             }
@@ -512,7 +515,8 @@ __global__ __launch_bounds__(kTraceBlock) void kExtend(SceneView sv, const float
             }
         }
         if (rayId != kNoRay) {
-            if (!closestStep(sv, r, kRayTMin, cs, st, COUNT ? &nodeVisits : nullptr, COUNT ? &triTests : nullptr)) {
+            if (!closestStep<ALPHA>(sv, r, kRayTMin, cs, st, alphaKey, COUNT ? &nodeVisits : nullptr,
+                                    COUNT ? &triTests : nullptr)) {
                 hits[rayId] = make_float4(cs.h.t, __uint_as_float(cs.h.tri), cs.h.u, cs.h.v);
                 rayId = kNoRay;
             }
@@ -545,7 +549,7 @@ DEV uint32_t classifyOccluder(const SceneView& sv, uint32_t tri) {
 template <int MODE>
 __global__ __launch_bounds__(kTraceBlock) void kShadow(SceneView sv, PathView pv, ShadowView sh, uint32_t* counters,
                                                       int countSlot, int fetchSlot, uint32_t envPass,
-                                                      uint32_t* gstack) {
+                                                      uint32_t* gstack, uint32_t alphaSalt, uint32_t pathBase) {
     __shared__ uint32_t lds[kLdsStackWords];
     TraversalStack st;
     st.lds = lds + threadIdx.x;
@@ -569,7 +573,7 @@ __global__ __launch_bounds__(kTraceBlock) void kShadow(SceneView sv, PathView pv
         float4 o = sh.o[i], d = sh.d[i];
         uint32_t q = __float_as_uint(o.w);
         TraversalRay r = makeTraversalRay(xyz(o), xyz(d));
-        uint32_t tri = traceAny(sv, r, kRayTMin, d.w, st);
+        uint32_t tri = traceAny(sv, r, kRayTMin, d.w, st, alphaRayKey(alphaSalt, pathBase + q));
         uint32_t vis = envPass ? (tri == kNoHit ? kVisVisible : kVisOccluded) : classifyOccluder(sv, tri);
         if (vis != kVisVisible && envPass) continue;  // b is zero for environment rays
         F4 c = ld4(vis == kVisVisible ? sh.a[i] : sh.b[i]);
@@ -1177,11 +1181,13 @@ __global__ __launch_bounds__(kTraceBlock) void kTraceQuery(SceneView sv, const f
     for (uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x; i < n; i += gridDim.x * kTraceBlock) {
         float4 a = rays[2 * i], b = rays[2 * i + 1];
         TraversalRay r = makeTraversalRay(xyz(a), xyz(b));
+        // a query ray has no path: its index is its key
+        const uint32_t key = alphaRayKey(alphaSalt(0u, 0ull, 0u, 3u), i);
         if (anyHit) {
-            uint32_t vis = classifyOccluder(sv, traceAny(sv, r, a.w, b.w, st));
+            uint32_t vis = classifyOccluder(sv, traceAny(sv, r, a.w, b.w, st, key));
             out[i] = make_float4(__uint_as_float(vis), 0.0f, 0.0f, 0.0f);
         } else {
-            Hit h = traceClosest(sv, r, a.w, b.w, st);
+            Hit h = traceClosest(sv, r, a.w, b.w, st, key);
             out[i] = make_float4(h.t, __uint_as_float(h.tri), h.u, h.v);
         }
     }
@@ -1205,7 +1211,9 @@ __global__ __launch_bounds__(kTraceBlock) void kPrimary(SceneView sv, SceneData 
         F3 o, d;
         primaryRay(sd, px, py, jx - 0.5f, jy - 0.5f, o, d);
         TraversalRay r = makeTraversalRay(o, d);
-        Hit h = traceClosest(sv, r, kRayTMin, kRayTMax, st);
+        // the key of this pixel's primary ray in the frame's path numbering
+        Hit h = traceClosest(sv, r, kRayTMin, kRayTMax, st,
+                             alphaRayKey(alphaSalt(sd.frameNumber, rngSeed, 0u, 0u), sampleIndex * n + i));
         if (h.tri == kNoHit) {
             outMesh[i] = VKRT_INVALID_INDEX;
             outPrim[i] = VKRT_INVALID_INDEX;
@@ -1432,7 +1440,9 @@ __global__ __launch_bounds__(kTraceBlock) void kShadeAdjoint(FrameParams fp, Sce
                     }
                     if (fp.traceShadowAlways || anyPos(nee)) {
                         TraversalRay sr = makeTraversalRay(sf.p + off * kShadowOriginOffset, ls.wi);
-                        uint32_t vis = classifyOccluder(sv, traceAny(sv, sr, kRayTMin, ls.shadowDistance, st));
+                        const uint32_t key =
+                            alphaRayKey(alphaSalt(sd.frameNumber, fp.rngSeed, fp.depth, 1u), (uint32_t)fp.pathBegin + q);
+                        uint32_t vis = classifyOccluder(sv, traceAny(sv, sr, kRayTMin, ls.shadowDistance, st, key));
                         if (vis == kVisUnsupported) clearPrevNee = true;
                         if (vis == kVisVisible && evaluated) {
                             lrem = lrem - thr3 * nee;
@@ -1471,7 +1481,9 @@ __global__ __launch_bounds__(kTraceBlock) void kShadeAdjoint(FrameParams fp, Sce
                             if (anyPos(ne)) {
                                 F3 off = wl.z > 0.0f ? sf.ng : -sf.ng;
                                 TraversalRay sr = makeTraversalRay(sf.p + off * kShadowOriginOffset, es.wi);
-                                if (traceAny(sv, sr, kRayTMin, kRayTMax, st) == kNoHit) {
+                                const uint32_t key = alphaRayKey(alphaSalt(sd.frameNumber, fp.rngSeed, fp.depth, 2u),
+                                                                 (uint32_t)fp.pathBegin + q);
+                                if (traceAny(sv, sr, kRayTMin, kRayTMax, st, key) == kNoHit) {
                                     lrem = lrem - thr3 * ne;
                                     environmentGradient(sv, sd, es.wi,
                                                         dI * thr3 * (e.value * fabsf(wl.z)) * (mis / es.pdf), av.gEnv);
@@ -1741,6 +1753,27 @@ static void queueViews(mrt_context* ctx, QueueView qv[2]) {
     }
 }
 
+// kExtend in the variant the scene needs: counting statistics or not, alpha test or not.
+static void launchExtend(mrt_context* ctx, bool countTraversal, const SceneView& sv, const QueueView& q, float4* hits,
+                         uint32_t* cnt, int slot, uint32_t salt, uint32_t pathBase) {
+    hipStream_t s = ctx->stream;
+    const bool alpha = sv.triAlpha != nullptr;
+    const dim3 grid(ctx->traceGrid), block(kTraceBlock);
+    uint32_t* stack = ctx->traversalStack.ptr;
+    if (countTraversal && alpha)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(kExtend<true, true>), grid, block, 0, s, sv, q.o, q.d, hits, cnt, slot, stack,
+                           salt, pathBase);
+    else if (countTraversal)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(kExtend<true, false>), grid, block, 0, s, sv, q.o, q.d, hits, cnt, slot, stack,
+                           salt, pathBase);
+    else if (alpha)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(kExtend<false, true>), grid, block, 0, s, sv, q.o, q.d, hits, cnt, slot,
+                           stack, salt, pathBase);
+    else
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(kExtend<false, false>), grid, block, 0, s, sv, q.o, q.d, hits, cnt, slot,
+                           stack, salt, pathBase);
+}
+
 template <int MODE>
 static bool launchChunk(mrt_context* ctx, FrameParams fp, StageEvents& ev, bool countTraversal,
                         const NrcRun* nrc = nullptr, bool keepSampleRad = false, bool sortShading = false) {
@@ -1771,12 +1804,9 @@ static bool launchChunk(mrt_context* ctx, FrameParams fp, StageEvents& ev, bool 
         fp.depth = depth;
         hipLaunchKernelGGL(kPrepDepth, dim3(1), dim3(64), 0, s, cnt, cur, nxt);
         bool timed = ev.begin(s, 0);
-        if (countTraversal)
-            hipLaunchKernelGGL(kExtend<true>, dim3(ctx->traceGrid), dim3(kTraceBlock), 0, s, sv, qv[cur].o, qv[cur].d,
-                               b.hits.ptr, cnt, cur, ctx->traversalStack.ptr);
-        else
-            hipLaunchKernelGGL(kExtend<false>, dim3(ctx->traceGrid), dim3(kTraceBlock), 0, s, sv, qv[cur].o, qv[cur].d,
-                               b.hits.ptr, cnt, cur, ctx->traversalStack.ptr);
+        const uint32_t pathBase = (uint32_t)fp.pathBegin;
+        launchExtend(ctx, countTraversal, sv, qv[cur], b.hits.ptr, cnt, cur,
+                     alphaSalt(fp.sd.frameNumber, fp.rngSeed, depth, 0u), pathBase);
         if (timed) ev.end(s);
         timed = ev.begin(s, 1);
         const uint32_t* order = nullptr;
@@ -1805,10 +1835,12 @@ static bool launchChunk(mrt_context* ctx, FrameParams fp, StageEvents& ev, bool 
             timed = ev.begin(s, 2);
             if (fp.neeEnabled)
                 hipLaunchKernelGGL(kShadow<MODE>, dim3(ctx->traceGrid), dim3(kTraceBlock), 0, s, sv, pv, shv, cnt,
-                                   kCntShadow, kCntFetchShadow, 0u, ctx->traversalStack.ptr);
+                                   kCntShadow, kCntFetchShadow, 0u, ctx->traversalStack.ptr,
+                                   alphaSalt(fp.sd.frameNumber, fp.rngSeed, depth, 1u), pathBase);
             if (fp.envNee)
                 hipLaunchKernelGGL(kShadow<MODE>, dim3(ctx->traceGrid), dim3(kTraceBlock), 0, s, sv, pv, shev, cnt,
-                                   kCntShadowEnv, kCntFetchShadowEnv, 1u, ctx->traversalStack.ptr);
+                                   kCntShadowEnv, kCntFetchShadowEnv, 1u, ctx->traversalStack.ptr,
+                                   alphaSalt(fp.sd.frameNumber, fp.rngSeed, depth, 2u), pathBase);
             if (timed) ev.end(s);
         }
         cur = nxt;
@@ -2156,8 +2188,9 @@ extern "C" int mrt_render_backward(mrt_context* ctx, const SceneData* scene, con
             int nxt = 1 - cur;
             fp.depth = depth;
             hipLaunchKernelGGL(kPrepDepth, dim3(1), dim3(64), 0, s, cnt, cur, nxt);
-            hipLaunchKernelGGL(kExtend<false>, dim3(ctx->traceGrid), dim3(kTraceBlock), 0, s, sv, qv[cur].o, qv[cur].d,
-                               b.hits.ptr, cnt, cur, ctx->traversalStack.ptr);
+            // the same alpha decisions as the primal pass: same salt, same path numbering
+            launchExtend(ctx, false, sv, qv[cur], b.hits.ptr, cnt, cur, alphaSalt(fp.sd.frameNumber, fp.rngSeed, depth, 0u),
+                         (uint32_t)fp.pathBegin);
             hipLaunchKernelGGL(kShadeAdjoint, dim3(ctx->traceGrid), dim3(kTraceBlock),
                                av.ldsMaterials ? (uint32_t)gradLdsBytes : 0u, s, fp, sv, pv, qv[cur], b.hits.ptr,
                                qv[nxt], cnt, cur, nxt, ctx->traversalStack.ptr, av);

@@ -172,6 +172,8 @@ static VKRT_Result syncScene(VKRT* vkrt, int requireDevice) {
         }
     }
     if (mrt_upload_materials(vkrt->device, p->materials, p->materialCount) != 0) return deviceError();
+    if ((geometryDirty || materialDirty) && mrt_upload_alpha(vkrt->device, p->triangleAlpha, p->triangleCount) != 0)
+        return deviceError();
     if (mrt_upload_meshes(vkrt->device, p->meshInfos, vkrt->meshCount) != 0) return deviceError();
     mrt_light_desc l = {p->emissiveMeshes, p->emissiveMeshCount, p->emissiveTriangles, p->emissiveTriangleCount,
                         p->meshAliasQ,     p->meshAliasIdx,      p->triAliasQ,         p->triAliasIdx,
@@ -452,6 +454,7 @@ VKRT_Result VKRT_exportSceneBuffers(VKRT* vkrt, VKRT_SceneBuffers* out) {
     out->rngPolicy = vkrt->rngPolicy;
     out->rngSeed = vkrt->rngSeed;
     out->environmentImportanceSampling = vkrt->envImportance;
+    out->triangleAlpha = p->triangleAlpha;
     if (out->environmentPixels) {
         out->environmentAliasQ = p->envAliasQ;
         out->environmentAliasIdx = p->envAliasIdx;

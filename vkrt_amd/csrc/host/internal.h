@@ -70,6 +70,10 @@ typedef struct PackedScene {
     uint32_t* envAliasIdx;
     float* envPdf;
     uint32_t hasTransmissive;
+    /* per global triangle, only when some mesh may reject hits (alpha test):
+     * bit 31 = the triangle's mesh may reject a hit, bits 0-23 = the alpha bytes
+     * of its three vertex colours; NULL when every mesh is opaque */
+    uint32_t* triangleAlpha;
 } PackedScene;
 
 struct VKRT {

@@ -165,6 +165,7 @@ void vkrtFreePackedScene(PackedScene* p) {
     free(p->envAliasQ);
     free(p->envAliasIdx);
     free(p->envPdf);
+    free(p->triangleAlpha);
     memset(p, 0, sizeof(*p));
 }
 
@@ -398,6 +399,38 @@ static VKRT_Result buildEnvironmentTable(VKRT* vkrt, PackedScene* p) {
     return VKRT_SUCCESS;
 }
 
+/* src/shaders/rt/alpha_test.slang:8-18 (alphaMayRejectHit): a mesh runs the
+ * alpha test when its material is MASK or BLEND or either opacity is below
+ * 0.999; src/core/render/accel/tlas.c:298-314 marks every other instance opaque. */
+static int meshMayRejectHit(const MeshInfo* info, const Material* m) {
+    return m->alphaMode == VKRT_MATERIAL_ALPHA_MODE_MASK || m->alphaMode == VKRT_MATERIAL_ALPHA_MODE_BLEND ||
+           m->opacity < 0.999f || info->opacity < 0.999f;
+}
+
+static VKRT_Result buildTriangleAlpha(VKRT* vkrt, PackedScene* p) {
+    int any = 0;
+    for (uint32_t mi = 0; mi < vkrt->meshCount && !any; mi++)
+        any = meshMayRejectHit(&p->meshInfos[mi], &p->materials[p->meshInfos[mi].materialIndex]);
+    if (!any || p->triangleCount == 0) return VKRT_SUCCESS;
+    p->triangleAlpha = (uint32_t*)calloc(p->triangleCount, sizeof(uint32_t));
+    if (!p->triangleAlpha) return VKRT_ERROR_OUT_OF_MEMORY;
+    for (uint32_t mi = 0; mi < vkrt->meshCount; mi++) {
+        const MeshInfo* info = &p->meshInfos[mi];
+        if (!meshMayRejectHit(info, &p->materials[info->materialIndex])) continue;
+        uint32_t tris = info->indexCount / 3u, tbase = p->meshTriangleBase[mi];
+        for (uint32_t t = 0; t < tris; t++) {
+            uint32_t word = 0x80000000u;
+            for (uint32_t c = 0; c < 3u; c++) {
+                uint32_t vi = p->indices[info->indexBase + t * 3u + c];
+                if (vi >= info->vertexCount) vi = 0;
+                word |= (p->vertices[info->vertexBase + vi].packedColor >> 24) << (8u * c);
+            }
+            p->triangleAlpha[tbase + t] = word;
+        }
+    }
+    return VKRT_SUCCESS;
+}
+
 VKRT_Result vkrtPrepareScene(VKRT* vkrt) {
     PackedScene next;
     memset(&next, 0, sizeof(next));
@@ -424,6 +457,7 @@ VKRT_Result vkrtPrepareScene(VKRT* vkrt) {
     VKRT_Result r = buildGeometry(vkrt, &next);
     if (r == VKRT_SUCCESS) r = buildLights(vkrt, &next);
     if (r == VKRT_SUCCESS) r = buildEnvironmentTable(vkrt, &next);
+    if (r == VKRT_SUCCESS) r = buildTriangleAlpha(vkrt, &next);
     if (r != VKRT_SUCCESS) {
         vkrtFreePackedScene(&next);
         return r;

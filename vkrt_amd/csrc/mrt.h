@@ -53,6 +53,9 @@ int mrt_build_bvh(mrt_context* ctx, mrt_bvh_stats* stats);
 
 int mrt_upload_meshes(mrt_context* ctx, const MeshInfo* meshInfos, uint32_t meshCount);
 int mrt_upload_materials(mrt_context* ctx, const Material* materials, uint32_t materialCount);
+/* Per-triangle alpha words (bit 31: the mesh runs the alpha test; bits 0-23:
+ * vertex colour alphas); NULL turns the alpha test off. */
+int mrt_upload_alpha(mrt_context* ctx, const uint32_t* triangleAlpha, uint32_t triangleCount);
 
 typedef struct mrt_light_desc {
     const EmissiveMesh* emissiveMeshes;

@@ -31,6 +31,7 @@ class MeshData:
     indices: np.ndarray  # M x 3 uint32
     material: int  # index into Scene.materials
     transform: np.ndarray | None = None  # 4 x 4 world transform (row-major), identity if None
+    opacity: float = 1.0  # mesh opacity (VKRT_setMeshOpacity)
 
 
 @dataclass
@@ -104,6 +105,33 @@ def cornell_box() -> Scene:
     s.camera = ((0.5, -1.35, 0.5), (0.5, 0.5, 0.5), (0.0, 0.0, 1.0), 40.0)
     s.environment_color = (0.0, 0.0, 0.0)
     s.environment_strength = 0.0
+    return s
+
+
+def alpha_cornell() -> Scene:
+    """The Cornell box with three surfaces that run the alpha test
+    (src/shaders/rt/alpha_test.slang): a horizontal BLEND sheet of opacity 0.5
+    under the light (shadow rays cross it), a MASK quad before the back wall
+    whose vertex-colour alpha runs 0 -> 1 from left to right (cutoff 0.4), and
+    the red wall drawn at mesh opacity 0.35 (not a box: a box's bottom face is
+    coplanar with the floor, and rays through a translucent box would meet two
+    surfaces at the same distance)."""
+    s = cornell_box()
+    sheet = default_material(baseColor=(0.9, 0.85, 0.6), roughness=0.6, metallic=0.0, specular=0.0)
+    sheet.alphaMode = 2  # BLEND
+    sheet.opacity = 0.5
+    mask = default_material(baseColor=(0.2, 0.3, 0.8), roughness=1.0, metallic=0.0, specular=0.0)
+    mask.alphaMode = 1  # MASK
+    mask.alphaCutoff = 0.4
+    s.materials += [sheet, mask]
+    zs = 0.8
+    pos, nrm, idx = _quad((0.25, 0.2, zs), (0.75, 0.2, zs), (0.75, 0.8, zs), (0.25, 0.8, zs), (0, 0, -1))
+    s.meshes.append(MeshData(make_vertices(pos, nrm), idx, len(s.materials) - 2))
+    pos, nrm, idx = _quad((0.1, 0.9, 0.1), (0.9, 0.9, 0.1), (0.9, 0.9, 0.7), (0.1, 0.9, 0.7), (0, -1, 0))
+    colors = np.ones((4, 4), dtype=np.float32)
+    colors[:, 3] = [0.0, 1.0, 1.0, 0.0]
+    s.meshes.append(MeshData(make_vertices(pos, nrm, colors), idx, len(s.materials) - 1))
+    s.meshes[1].opacity = 0.35  # the red wall
     return s
 
 
@@ -349,6 +377,8 @@ def load(renderer: Renderer, scene: Scene):
         if mesh.transform is not None:
             m = np.asarray(mesh.transform, dtype=np.float32)
             renderer.set_mesh_transform_matrix(idx, np.ascontiguousarray(m.T).reshape(-1))
+        if mesh.opacity != 1.0:
+            renderer.set_mesh_opacity(idx, mesh.opacity)
     pos, target, up, vfov = scene.camera
     renderer.set_camera(pos, target, up, vfov)
     if scene.environment is not None:
