@@ -63,6 +63,13 @@ VKRT_Result VKRT_setDebugMode(VKRT* vkrt, VKRT_DebugMode mode);                 
 VKRT_Result VKRT_setMisNeeEnabled(VKRT* vkrt, uint8_t enabled);                       /* vkrt.h:52 */
 VKRT_Result VKRT_setTimeRange(VKRT* vkrt, float timeBase, float timeStep);            /* vkrt.h:53 */
 
+/* image export: vkrt.h:54-58. The path's extension picks PNG (16-bit), JPEG
+ * or EXR (linear RGBA32F); no extension means PNG. Denoising is not available
+ * in this build: a request for it exports the raw render. */
+void VKRT_defaultRenderExportSettings(VKRT_RenderExportSettings* settings);                    /* vkrt.h:54 */
+VKRT_Result VKRT_saveRenderImageEx(VKRT* vkrt, const char* path, const VKRT_RenderExportSettings* settings); /* vkrt.h:57 */
+VKRT_Result VKRT_saveRenderImage(VKRT* vkrt, const char* path);                               /* vkrt.h:58 */
+
 /* render sessions and queries: vkrt.h:59-66 */
 VKRT_Result VKRT_startRender(VKRT* vkrt, uint32_t width, uint32_t height, uint32_t targetSamples); /* vkrt.h:59 */
 VKRT_Result VKRT_continueRender(VKRT* vkrt, uint32_t targetSamples);                            /* vkrt.h:60 */

@@ -367,6 +367,10 @@ class MeshSnapshot(C.Structure):
     ]
 
 
+class RenderExportSettings(C.Structure):
+    _fields_ = [("denoiseEnabled", C.c_uint8)]
+
+
 # include/vkrt_import.h (src/app/mesh/loader.h:5-55)
 class MaterialImportEntry(C.Structure):
     # Material holds 16-byte aligned vec4 members, so it starts at offset 16
@@ -554,6 +558,9 @@ def library() -> C.CDLL:
         "meshReleaseImportData": (None, [C.POINTER(MeshImportData)]),
         "VKRT_importMesh": (C.c_int, [P, C.c_char_p, C.c_char_p, up, up]),
         "VKRT_loadSceneFile": (C.c_int, [P, C.c_char_p]),
+        "VKRT_defaultRenderExportSettings": (None, [C.POINTER(RenderExportSettings)]),
+        "VKRT_saveRenderImageEx": (C.c_int, [P, C.c_char_p, C.POINTER(RenderExportSettings)]),
+        "VKRT_saveRenderImage": (C.c_int, [P, C.c_char_p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -732,6 +739,14 @@ class Renderer:
         c = C.c_uint32()
         _ok(self.lib.VKRT_getTextureCount(self.h, C.byref(c)), "VKRT_getTextureCount")
         return c.value
+
+    def save_image(self, path: str, denoise: bool | None = None):
+        """VKRT_saveRenderImage(Ex): PNG / JPEG / EXR by extension."""
+        if denoise is None:
+            _ok(self.lib.VKRT_saveRenderImage(self.h, os.fsencode(path)), "VKRT_saveRenderImage")
+        else:
+            st = RenderExportSettings(1 if denoise else 0)
+            _ok(self.lib.VKRT_saveRenderImageEx(self.h, os.fsencode(path), C.byref(st)), "VKRT_saveRenderImageEx")
 
     # scene input (include/vkrt_import.h)
     def import_mesh(self, path: str, name: str | None = None) -> tuple[int, int]:
