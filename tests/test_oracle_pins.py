@@ -278,6 +278,38 @@ def lut_buffers():
     r.close()
 
 
+def test_rgb2spec_interval_search_forms_agree(lut_buffers):
+    """The shading kernel finds a colour's brightness interval with two levels of
+    eight-way comparisons; the oracle (and the reference) with a binary search.
+    Both must give the same index on the shipped table for every brightness,
+    including the table's own entries and their float neighbours."""
+    table = np.ctypeslib.as_array(lut_buffers.rgb2specTable, (lut_buffers.rgb2specFloatCount,))
+    res = 64  # the payload is res scale values, then 3 * res^3 * 3 coefficients
+    assert len(table) == res + 9 * res ** 3
+    sc = table[:res].astype(np.float32)
+    assert np.all(np.diff(sc) >= 0)
+
+    def binary(x):
+        left, size = 0, res - 2
+        while size > 0:
+            half = size >> 1
+            middle = left + half + 1
+            if sc[middle] <= x:
+                left, size = middle, size - half - 1
+            else:
+                size = half
+        return min(left, res - 2)
+
+    def eight_way(x):
+        b = sum(int(sc[8 * k] <= x) for k in range(1, 8))
+        return min(8 * b + sum(int(sc[8 * b + k] <= x) for k in range(1, 8)), res - 2)
+
+    xs = np.concatenate([np.random.default_rng(3).random(20000).astype(np.float32), sc,
+                         np.nextafter(sc, np.float32(2)), np.nextafter(sc, np.float32(-1)),
+                         np.float32([0.0, 1.0, 1e-8, 4.0])])
+    assert all(binary(x) == eight_way(x) for x in xs)
+
+
 def test_rgb2spec_round_trip(lut_buffers):
     """sRGB -> coefficients -> spectrum -> CIE XYZ -> sRGB returns the input."""
     lam = np.arange(360.0, 831.0)

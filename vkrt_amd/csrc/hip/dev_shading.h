@@ -290,20 +290,22 @@ struct Spec {
 // strides let the 24 corner loads share two base addresses, which keeps the
 // hero shading kernel within its register budget.
 constexpr uint32_t kSpecRes = 64u;
+static_assert(kSpecRes == 64u, "specInterval searches 8 x 8 scale entries");
 DEV uint32_t specInterval(const Spec& sp, float x) {
-    int left = 0;
-    int size = (int)kSpecRes - 2;
-    while (size > 0) {
-        int half = size >> 1;
-        int middle = left + half + 1;
-        if (sp.table[sp.scaleOffset + (uint32_t)middle] <= x) {
-            left = middle;
-            size -= half + 1;
-        } else {
-            size = half;
-        }
-    }
-    return min((uint32_t)left, kSpecRes - 2u);
+    // The largest i in [0, res - 2] with scale[i] <= x, which the reference finds
+    // by binary search (rgb2spec.slang:14-30): six dependent loads. The scale is
+    // increasing, so two levels of eight-way comparisons give the same index
+    // with two dependent steps; the first level's addresses are the same for
+    // every lane.
+    const float* sc = sp.table + sp.scaleOffset;
+    uint32_t b = 0;
+#pragma unroll
+    for (uint32_t k = 1; k < 8u; k++) b += sc[8u * k] <= x ? 1u : 0u;
+    const float* blk = sc + 8u * b;
+    uint32_t i = 8u * b;
+#pragma unroll
+    for (uint32_t k = 1; k < 8u; k++) i += blk[k] <= x ? 1u : 0u;
+    return min(i, kSpecRes - 2u);
 }
 
 DEV F3 specFetch(const Spec& sp, F3 rgb) {
