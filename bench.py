@@ -448,7 +448,7 @@ def main():
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBPS, 4),
         "traffic": traffic,
-        "kernel": "kExtend<false, false> (closest-hit traversal, 8-wide quantised BVH)",
+        "kernel": "kExtend<false, false, false> (closest-hit traversal, 8-wide quantised BVH)",
         "launches": ext_launches,
         "avg_launch_ms": round(ext_avg_ms, 4),
         "bytes_per_ray": round(B_EXTEND_IO + b_trav, 1),

@@ -186,7 +186,12 @@ typedef struct VKRT_SceneBuffers {
     const uint32_t* environmentAliasIdx;
     const float* environmentPdf;         /* texel selection probabilities */
     const uint32_t* triangleAlpha;       /* per triangle: bit 31 = mesh runs the alpha test, bits 0-23 = the
-                                            three vertex colour alphas (8 bits each); NULL when all opaque */
+                                            three vertex colour alphas (8 bits each), bit 30 = the base-colour
+                                            texture's alpha takes part; NULL when all opaque */
+    const float* textureTexels;          /* material texture pool, linear RGBA32F; NULL without material textures */
+    uint64_t textureTexelCount;
+    const uint32_t* textureInfo;         /* per texture: first texel, width, height, 0 (width 0: unused) */
+    uint32_t textureCount;
 } VKRT_SceneBuffers;
 VKRT_Result VKRT_exportSceneBuffers(VKRT* vkrt, VKRT_SceneBuffers* out);
 

@@ -72,6 +72,19 @@ typedef vkrt_f4 vkrt_m4[4]; /* column-major: m[column][row], as cglm mat4 */
 #define VKRT_MATERIAL_ALPHA_MODE_MASK 1u
 #define VKRT_MATERIAL_ALPHA_MODE_BLEND 2u
 
+#define VKRT_MATERIAL_TEXTURE_SLOT_BASE_COLOR 0u
+#define VKRT_MATERIAL_TEXTURE_SLOT_METALLIC_ROUGHNESS 1u
+#define VKRT_MATERIAL_TEXTURE_SLOT_NORMAL 2u
+#define VKRT_MATERIAL_TEXTURE_SLOT_EMISSIVE 3u
+#define VKRT_MATERIAL_TEXTURE_SLOT_COUNT 4u
+
+/* glTF sampler wrap codes; a material's packed wrap holds U in bits 0-15 and
+ * V in bits 16-31 (src/shared/constants.h:72-76) */
+#define VKRT_TEXTURE_WRAP_REPEAT 10497u
+#define VKRT_TEXTURE_WRAP_CLAMP_TO_EDGE 33071u
+#define VKRT_TEXTURE_WRAP_MIRRORED_REPEAT 33648u
+#define VKRT_TEXTURE_WRAP_DEFAULT (VKRT_TEXTURE_WRAP_REPEAT | (VKRT_TEXTURE_WRAP_REPEAT << 16u))
+
 #define VKRT_TEXTURE_COLOR_SPACE_SRGB 0u
 #define VKRT_TEXTURE_COLOR_SPACE_LINEAR 1u
 

@@ -59,6 +59,7 @@ def lib():
         g("oracle_intersect").restype = C.c_int
         g("oracle_intersect").argtypes = [f32p, rp, rp]
         g("oracle_bsdf_eval").argtypes = [P, P, rp, rp, C.c_uint32, rt, C.c_uint32, rp, rp]
+        g("oracle_sample_material_slot").argtypes = [P, P, C.c_uint32, f32p, C.c_uint32, f32p]
         g("oracle_bsdf_eval_hero").argtypes = [P, P, rp, rp, C.c_uint32, rp, rp, rp]
         g("oracle_bsdf_sample_batch").argtypes = [P, P, rp, C.c_uint32, rt, C.c_uint32, C.c_uint32, C.c_uint32, rp]
         g("oracle_rgb2spec").restype = rt
@@ -218,6 +219,16 @@ def bsdf_sample_batch(buffers, material, wo, n, seed=1, front_face=1, wavelength
     getattr(lib(), "oracle_bsdf_sample_batch" + sfx)(C.addressof(buffers) if buffers is not None else None,
                                                     C.addressof(material), ptr(wo), front_face, wavelength, mode, seed,
                                                     n, ptr(out))
+    return out
+
+
+def sample_material_slot(buffers, material, slot, uv):
+    """A material texture slot sampled at the N x 4 coordinates uv = (u0, v0,
+    u1, v1) (both texture-coordinate sets); returns N x 4 float32."""
+    uv = np.ascontiguousarray(uv, dtype=np.float32).reshape(-1, 4)
+    out = np.zeros((len(uv), 4), dtype=np.float32)
+    lib().oracle_sample_material_slot_f32(C.addressof(buffers), C.addressof(material), slot, ptr(uv), len(uv),
+                                          ptr(out))
     return out
 
 

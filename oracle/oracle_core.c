@@ -252,7 +252,7 @@ static int slab(const ONode* n, v3 o, v3 inv, real tmax) {
  * any-hit order. Ray key = mix(mix(path index) ^ salt); salt from the frame
  * number, the RNG seed, the depth and the ray kind (0 extension, 1 light
  * shadow, 2 environment shadow, 3 query). Vertex-colour alpha is interpolated
- * in float with weights (1-u-v, u, v); texture alpha is 1 (not sampled). */
+ * in float with weights (1-u-v, u, v). */
 static uint32_t alphaSalt(uint32_t frame, uint64_t seed, uint32_t depth, uint32_t kind) {
     return hash32(frame * 0x9E3779B9u ^ (uint32_t)seed ^ (uint32_t)(seed >> 32) * 0x85EBCA6Bu ^
                   ((depth << 2) | kind) * 0xC2B2AE35u);
@@ -260,6 +260,85 @@ static uint32_t alphaSalt(uint32_t frame, uint64_t seed, uint32_t depth, uint32_
 static uint32_t alphaRayKey(uint32_t salt, uint32_t pathIndex) { return hash32(hash32(pathIndex) ^ salt); }
 static uint32_t rayKey(const Ctx* c, uint32_t pathIndex, uint32_t depth, uint32_t kind) {
     return alphaRayKey(alphaSalt(c->sd->frameNumber, c->b->rngSeed, depth, kind), pathIndex);
+}
+
+/* ------------------------------------------------ material textures -- */
+/* Mip level 0, bilinear, per-axis wrap (the samplers of
+ * src/core/scene/textures.c:141-178 sampled by
+ * src/shaders/material/textures.slang:52-56), in float: texel (x, y) of a
+ * texture is pool texel first + y * width + x, row 0 at coordinate v = 0. */
+static int wrapTexel(int i, int n, uint32_t mode) {
+    if (mode == VKRT_TEXTURE_WRAP_CLAMP_TO_EDGE) return i < 0 ? 0 : (i > n - 1 ? n - 1 : i);
+    if (mode == VKRT_TEXTURE_WRAP_MIRRORED_REPEAT) {
+        /* (size - 1) - mirror((i mod 2size) - size), mirror(a) = a >= 0 ? a : -(1 + a) */
+        int a = (int)(((long long)i % (2LL * n) + 2LL * n) % (2LL * n)) - n;
+        return (n - 1) - (a >= 0 ? a : -(1 + a));
+    }
+    return (int)(((long long)i % n + n) % n);
+}
+static float texelCoord(float uv, uint32_t size) {
+    float x = uv * (float)size - 0.5f;
+    x = fmaxf(x, -16777216.0f); /* a NaN coordinate becomes the lower bound */
+    return fminf(x, 16777216.0f);
+}
+static void sampleTexture(const Ctx* c, uint32_t tex, float u, float v, uint32_t packedWrap, float out[4]) {
+    const uint32_t* info = c->b->textureInfo + 4u * tex;
+    const int w = (int)info[1], h = (int)info[2];
+    const float x = texelCoord(u, info[1]), y = texelCoord(v, info[2]);
+    const float fx = floorf(x), fy = floorf(y);
+    const float ax = x - fx, ay = y - fy;
+    const int xs[2] = {wrapTexel((int)fx, w, packedWrap & 0xffffu), wrapTexel((int)fx + 1, w, packedWrap & 0xffffu)};
+    const int ys[2] = {wrapTexel((int)fy, h, packedWrap >> 16), wrapTexel((int)fy + 1, h, packedWrap >> 16)};
+    for (int k = 0; k < 4; k++) {
+        float t[2][2];
+        for (int j = 0; j < 2; j++)
+            for (int i = 0; i < 2; i++)
+                t[j][i] = c->b->textureTexels[((size_t)info[0] + (size_t)ys[j] * (size_t)w + (size_t)xs[i]) * 4u + k];
+        const float top = t[0][0] * (1.0f - ax) + t[0][1] * ax;
+        const float bottom = t[1][0] * (1.0f - ax) + t[1][1] * ax;
+        out[k] = top * (1.0f - ay) + bottom * ay;
+    }
+}
+/* src/shaders/geometry/surface/interpolation.slang:46-60 (texture coordinates) */
+typedef struct {
+    float uv[2][2]; /* [set][u, v] */
+} TexCoords;
+static TexCoords hitTexCoords(const Ctx* c, uint32_t mesh, uint32_t prim, float u, float v) {
+    const MeshInfo* m = &c->b->meshInfos[mesh];
+    const uint32_t tb = m->indexBase + prim * 3u;
+    const ShaderVertex* a = &c->b->vertices[c->b->indices[tb + 0] + m->vertexBase];
+    const ShaderVertex* b = &c->b->vertices[c->b->indices[tb + 1] + m->vertexBase];
+    const ShaderVertex* d = &c->b->vertices[c->b->indices[tb + 2] + m->vertexBase];
+    const float w = 1.0f - u - v;
+    TexCoords r;
+    for (int k = 0; k < 2; k++) {
+        r.uv[0][k] = a->texcoord0[k] * w + b->texcoord0[k] * u + d->texcoord0[k] * v;
+        r.uv[1][k] = a->texcoord1[k] * w + b->texcoord1[k] * u + d->texcoord1[k] * v;
+    }
+    return r;
+}
+/* sampleMaterialTexture (textures.slang:58-74): coordinate set, scale,
+ * rotation about the origin, offset; an empty slot returns the fallback. */
+static void sampleMaterialSlot(const Ctx* c, const Material* m, uint32_t slot, const TexCoords* tc,
+                               const float fallback[4], float out[4]) {
+    const uint32_t idx[4] = {m->baseColorTextureIndex, m->metallicRoughnessTextureIndex, m->normalTextureIndex,
+                             m->emissiveTextureIndex};
+    const uint32_t wrap[4] = {m->baseColorTextureWrap, m->metallicRoughnessTextureWrap, m->normalTextureWrap,
+                              m->emissiveTextureWrap};
+    const float* xf[4] = {m->baseColorTextureTransform, m->metallicRoughnessTextureTransform,
+                          m->normalTextureTransform, m->emissiveTextureTransform};
+    if (idx[slot] == VKRT_INVALID_INDEX) {
+        for (int k = 0; k < 4; k++) out[k] = fallback[k];
+        return;
+    }
+    const int set = ((m->textureTexcoordSets >> (slot * 8u)) & 0xffu) == 1u;
+    const float su = tc->uv[set][0] * xf[slot][0], sv = tc->uv[set][1] * xf[slot][1];
+    const float sn = sinf(m->textureRotations[slot]), cs = cosf(m->textureRotations[slot]);
+    sampleTexture(c, idx[slot], cs * su - sn * sv + xf[slot][2], sn * su + cs * sv + xf[slot][3], wrap[slot], out);
+}
+static int materialHasTextures(const Material* m) {
+    return m->baseColorTextureIndex != VKRT_INVALID_INDEX || m->metallicRoughnessTextureIndex != VKRT_INVALID_INDEX ||
+           m->normalTextureIndex != VKRT_INVALID_INDEX || m->emissiveTextureIndex != VKRT_INVALID_INDEX;
 }
 
 static int alphaAccept(const Ctx* c, uint32_t tri, real u, real v, uint32_t key) {
@@ -272,16 +351,29 @@ static int alphaAccept(const Ctx* c, uint32_t tri, real u, real v, uint32_t key)
     const float w = 1.0f - fu - fv;
     const float ca = (float)(word & 0xffu) * k * w + (float)((word >> 8) & 0xffu) * k * fu +
                      (float)((word >> 16) & 0xffu) * k * fv;
+    /* evaluateMaterialTextureAlpha (alpha_test.slang:20-26): MASK and BLEND
+     * materials multiply in the base-colour texture's alpha */
+    float ta = ca;
+    if (word & 0x40000000u) {
+        const uint32_t mi = c->b->triangleMesh[tri];
+        uint32_t base = 0;
+        for (uint32_t k = 0; k < mi; k++) base += c->b->meshInfos[k].indexCount / 3u;
+        const TexCoords tc = hitTexCoords(c, mi, tri - base, fu, fv);
+        const float one[4] = {1.0f, 1.0f, 1.0f, 1.0f};
+        float bc[4];
+        sampleMaterialSlot(c, m, VKRT_MATERIAL_TEXTURE_SLOT_BASE_COLOR, &tc, one, bc);
+        ta = bc[3] * ca;
+    }
     const float x = (float)(hash32(key ^ tri * 0x27D4EB2Fu) >> 8) * (1.0f / 16777216.0f);
     if (m->alphaMode == VKRT_MATERIAL_ALPHA_MODE_MASK) {
-        if (ca < m->alphaCutoff) return 0;
+        if (ta < m->alphaCutoff) return 0;
         float mo = mesh->opacity * m->opacity * ca;
         mo = mo < 0.0f ? 0.0f : (mo > 1.0f ? 1.0f : mo);
         if (mo >= 1.0f) return 1;
         if (mo <= 0.0f) return 0;
         return x <= mo;
     }
-    float op = mesh->opacity * m->opacity * ca;
+    float op = mesh->opacity * m->opacity * ta;
     op = op < 0.0f ? 0.0f : (op > 1.0f ? 1.0f : op);
     const int blend = m->alphaMode == VKRT_MATERIAL_ALPHA_MODE_BLEND || m->opacity < 0.999f || mesh->opacity < 0.999f;
     if (!blend) return 1;

@@ -26,7 +26,14 @@ TONE_NONE = 0
 TONE_ACES = 1
 RNG_HASH = 0
 RNG_PCG32 = 1
+TEXTURE_FORMAT_RGBA8_UNORM = 0
+TEXTURE_FORMAT_RGBA16_UNORM = 1
+TEXTURE_FORMAT_RGBA16_SFLOAT = 2
 TEXTURE_FORMAT_RGBA32_SFLOAT = 3
+TEXTURE_COLOR_SPACE_SRGB = 0
+TEXTURE_WRAP_REPEAT = 10497
+TEXTURE_WRAP_CLAMP_TO_EDGE = 33071
+TEXTURE_WRAP_MIRRORED_REPEAT = 33648
 TEXTURE_COLOR_SPACE_LINEAR = 1
 RENDER_PHASE_SAMPLING = 1
 RENDER_PHASE_COMPLETE_RAW = 3
@@ -351,6 +358,10 @@ class SceneBuffers(C.Structure):
         ("environmentAliasIdx", C.POINTER(C.c_uint32)),
         ("environmentPdf", C.POINTER(C.c_float)),
         ("triangleAlpha", C.POINTER(C.c_uint32)),
+        ("textureTexels", C.POINTER(C.c_float)),
+        ("textureTexelCount", C.c_uint64),
+        ("textureInfo", C.POINTER(C.c_uint32)),
+        ("textureCount", C.c_uint32),
     ]
 
 
@@ -652,7 +663,8 @@ def default_material(**kw) -> Material:
     return m
 
 
-def make_vertices(positions: np.ndarray, normals: np.ndarray, colors: np.ndarray | None = None) -> np.ndarray:
+def make_vertices(positions: np.ndarray, normals: np.ndarray, colors: np.ndarray | None = None,
+                  texcoord0: np.ndarray | None = None, texcoord1: np.ndarray | None = None) -> np.ndarray:
     """Packs positions/normals (N x 3) into an array of the reference's 80-byte Vertex."""
     n = len(positions)
     v = np.zeros((n, 20), dtype=np.float32)
@@ -667,6 +679,10 @@ def make_vertices(positions: np.ndarray, normals: np.ndarray, colors: np.ndarray
     v[:, 8:11] = t
     v[:, 11] = 1.0
     v[:, 12:16] = 1.0 if colors is None else colors
+    if texcoord0 is not None:
+        v[:, 16:18] = texcoord0
+    if texcoord1 is not None:
+        v[:, 18:20] = texcoord1
     return np.ascontiguousarray(v)
 
 
@@ -725,6 +741,18 @@ class Renderer:
         h, w, _ = pixels.shape
         up = TextureUpload(name.encode(), pixels.ctypes.data_as(C.c_void_p), w, h, TEXTURE_FORMAT_RGBA32_SFLOAT,
                            TEXTURE_COLOR_SPACE_LINEAR)
+        idx = C.c_uint32()
+        _ok(self.lib.VKRT_addTextureFromPixels(self.h, C.byref(up), C.byref(idx)), "VKRT_addTextureFromPixels")
+        return idx.value
+
+    def add_texture(self, pixels: np.ndarray, fmt: int, color_space: int, name: str = "Texture") -> int:
+        """VKRT_addTextureFromPixels with an H x W x 4 array of the format's
+        element type (uint8, uint16, float16 or float32)."""
+        dtype = {TEXTURE_FORMAT_RGBA8_UNORM: np.uint8, TEXTURE_FORMAT_RGBA16_UNORM: np.uint16,
+                 TEXTURE_FORMAT_RGBA16_SFLOAT: np.float16, TEXTURE_FORMAT_RGBA32_SFLOAT: np.float32}[fmt]
+        pixels = np.ascontiguousarray(pixels, dtype=dtype)
+        h, w, _ = pixels.shape
+        up = TextureUpload(name.encode(), pixels.ctypes.data_as(C.c_void_p), w, h, fmt, color_space)
         idx = C.c_uint32()
         _ok(self.lib.VKRT_addTextureFromPixels(self.h, C.byref(up), C.byref(idx)), "VKRT_addTextureFromPixels")
         return idx.value

@@ -73,6 +73,8 @@ extern "C" void mrt_destroy(mrt_context* ctx) {
     ctx->worldTris.release();
     ctx->triMesh.release();
     ctx->triAlpha.release();
+    ctx->texels.release();
+    ctx->texInfo.release();
     ctx->meshTriBase.release();
     ctx->tris.release();
     ctx->nodes.release();
@@ -195,6 +197,19 @@ extern "C" int mrt_upload_alpha(mrt_context* ctx, const uint32_t* triangleAlpha,
     if (!ctx->triAlpha.upload(triangleAlpha, triangleCount, ctx->stream)) return -3;
     ctx->alphaEnabled = true;
     return sync(ctx, "upload alpha");
+}
+
+extern "C" int mrt_upload_textures(mrt_context* ctx, const float* texels, uint64_t texelCount, const uint32_t* info,
+                                   uint32_t textureCount) {
+    if (!ctx) return -1;
+    if (!check(hipSetDevice(ctx->device), "hipSetDevice")) return -2;
+    ctx->texturesEnabled = false;
+    if (!texels || texelCount == 0 || !info || textureCount == 0) return 0;
+    if (!ctx->texels.upload(reinterpret_cast<const float4*>(texels), texelCount, ctx->stream) ||
+        !ctx->texInfo.upload(reinterpret_cast<const uint4*>(info), textureCount, ctx->stream))
+        return -3;
+    ctx->texturesEnabled = true;
+    return sync(ctx, "upload textures");
 }
 
 extern "C" int mrt_upload_lights(mrt_context* ctx, const mrt_light_desc* l) {

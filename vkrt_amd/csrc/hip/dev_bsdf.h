@@ -23,14 +23,16 @@ struct BMat {
 // Material fetch plus applySurfaceTextures with no texture bound
 // (src/shaders/material/textures.slang:123-141): base colour is modulated by
 // the interpolated vertex colour; emission is renormalised by its maximum.
-DEV BMat loadMaterial(const Material* mats, uint32_t idx, F3 vertexColor) {
+// The texture factors default to an empty slot's fallback sample (1).
+DEV BMat loadMaterial(const Material* mats, uint32_t idx, F3 vertexColor, F3 texBase = F3{1.0f, 1.0f, 1.0f},
+                      float texRough = 1.0f, float texMetal = 1.0f, F3 texEmission = F3{1.0f, 1.0f, 1.0f}) {
     const Material& m = mats[idx];
     BMat b;
-    b.baseColor = F3{m.baseColor[0] * (1.0f * vertexColor.x), m.baseColor[1] * (1.0f * vertexColor.y),
-                     m.baseColor[2] * (1.0f * vertexColor.z)};
-    b.roughness = sat(m.roughness * 1.0f);
+    b.baseColor = F3{m.baseColor[0] * (texBase.x * vertexColor.x), m.baseColor[1] * (texBase.y * vertexColor.y),
+                     m.baseColor[2] * (texBase.z * vertexColor.z)};
+    b.roughness = sat(m.roughness * texRough);
     b.eta = F3{m.eta[0], m.eta[1], m.eta[2]};
-    b.metallic = sat(m.metallic * 1.0f);
+    b.metallic = sat(m.metallic * texMetal);
     b.k = F3{m.k[0], m.k[1], m.k[2]};
     b.anisotropic = m.anisotropic;
     b.specular = m.specular;
@@ -45,7 +47,7 @@ DEV BMat loadMaterial(const Material* mats, uint32_t idx, F3 vertexColor) {
     b.absorptionCoefficient = m.absorptionCoefficient;
     b.attenuationColor = F3{m.attenuationColor[0], m.attenuationColor[1], m.attenuationColor[2]};
     F3 em = F3{m.emissionColor[0], m.emissionColor[1], m.emissionColor[2]} * m.emissionLuminance;
-    em = em * f3(1.0f);
+    em = em * texEmission;
     float emax = maxc(em);
     if (emax > 0.0f) {
         F3 col = em / emax;
@@ -54,6 +56,20 @@ DEV BMat loadMaterial(const Material* mats, uint32_t idx, F3 vertexColor) {
         b.emission = f3(0.0f);
     }
     return b;
+}
+
+// applySurfaceTextures (textures.slang:123-141) for a reconstructed hit: the
+// base-colour, metallic-roughness (g, b) and emissive samples scale the
+// material; untextured materials take the plain path.
+DEV BMat loadSurfaceMaterial(const SceneView& s, const Surface& sf) {
+    if (!sf.textured) return loadMaterial(s.materials, sf.materialIndex, sf.vertexColor);
+    const Material& m = s.materials[sf.materialIndex];
+    const F4 one = f4(1.0f);
+    const F4 bc = sampleMaterialSlot(s, m, VKRT_MATERIAL_TEXTURE_SLOT_BASE_COLOR, sf.tc, one);
+    const F4 mr = sampleMaterialSlot(s, m, VKRT_MATERIAL_TEXTURE_SLOT_METALLIC_ROUGHNESS, sf.tc, one);
+    const F4 em = sampleMaterialSlot(s, m, VKRT_MATERIAL_TEXTURE_SLOT_EMISSIVE, sf.tc, one);
+    return loadMaterial(s.materials, sf.materialIndex, sf.vertexColor, F3{bc.x, bc.y, bc.z}, mr.y, mr.z,
+                        F3{em.x, em.y, em.z});
 }
 
 DEV F3 diffuseColor(const BMat& m) { return m.baseColor * (1.0f - m.metallic); }

@@ -155,6 +155,9 @@ struct SceneView {
     const float* rgb2spec;
     const ShadeTri* shadeTris;
     const uint32_t* triAlpha;  // per-triangle alpha words; nullptr when every mesh is opaque
+    const float4* texels;      // material texture pool; nullptr when no material has a texture
+    const uint4* texInfo;      // per texture: first texel, width, height
+    uint32_t texCount;
 };
 
 struct Hit {

@@ -7,6 +7,7 @@
 #pragma once
 
 #include "dev_common.h"
+#include "dev_texture.h"
 
 namespace mrt {
 
@@ -196,6 +197,8 @@ struct Surface {
     uint32_t mesh;
     uint32_t materialIndex;
     F3 vertexColor;
+    bool textured;  // the material samples textures (TEX reconstruction only)
+    TexCoords tc;
 };
 
 DEV F3 bary3(F3 a, F3 b, F3 c, float u, float v) {
@@ -244,6 +247,10 @@ DEV ShadeTri buildShadeTri(const SceneView& s, uint32_t tri) {
     return r;
 }
 
+// TEX: the material's normal texture perturbs the shading normal before it is
+// sanitised (surface_state.slang:24-27, textures.slang:143-155), and the
+// texture coordinates are kept for the material's texture stage.
+template <bool TEX = false>
 DEV Surface reconstruct(const SceneView& s, F3 o, F3 d, const Hit& h) {
     Surface sf;
     const uint4* w = reinterpret_cast<const uint4*>(s.shadeTris + h.tri);
@@ -269,6 +276,22 @@ DEV Surface reconstruct(const SceneView& s, F3 o, F3 d, const Hit& h) {
     sf.vertexColor = F3{col.x, col.y, col.z};
     Basis unperturbed = makeBasis(sn, tangent);
     sn = unperturbed.n;
+    sf.textured = false;
+    if (TEX && s.texels) {
+        const Material& mt = s.materials[m.materialIndex];
+        if (materialHasTextures(mt)) {
+            sf.textured = true;
+            sf.tc = hitTexCoords(s, h.tri, mesh, h.u, h.v);
+            if (mt.normalTextureIndex != VKRT_INVALID_INDEX) {
+                const F4 ns = sampleMaterialSlot(s, mt, VKRT_MATERIAL_TEXTURE_SLOT_NORMAL, sf.tc, f4(0.0f));
+                F3 t = F3{ns.x * 2.0f - 1.0f, ns.y * 2.0f - 1.0f, ns.z * 2.0f - 1.0f};
+                t.x *= mt.normalTextureScale;
+                t.y *= mt.normalTextureScale;
+                t = safeNormalize(t);
+                sn = safeNormalize(unperturbed.t * t.x + unperturbed.b * t.y + unperturbed.n * t.z);
+            }
+        }
+    }
     {  // sanitizeShadingNormal (basis.slang:29-39)
         F3 n = safeNormalize(sn), g = safeNormalize(sf.ng), out = safeNormalize(-d);
         sn = (dot(n, g) <= 1e-4f || dot(n, out) <= 1e-4f) ? g : n;

@@ -8,6 +8,7 @@
 #pragma once
 
 #include "dev_common.h"
+#include "dev_texture.h"
 
 namespace mrt {
 
@@ -46,25 +47,34 @@ __host__ __device__ inline uint32_t alphaRayKey(uint32_t salt, uint32_t pathInde
 
 // Whether a candidate hit at barycentrics (u, v) of global triangle `tri`
 // survives the alpha test. The alpha word holds the three vertex-colour alpha
-// bytes; textures are not sampled, so the base-colour texture's alpha is 1.
+// bytes, and bit 30 when the material's base-colour texture alpha takes part
+// (MASK and BLEND materials with that texture, alpha_test.slang:20-26).
+template <bool TEXA = true>
 DEV bool alphaAccept(const SceneView& s, uint32_t tri, float u, float v, uint32_t key) {
     const uint32_t word = s.triAlpha[tri];
     if (!(word >> 31)) return true;
-    const MeshInfo& mesh = s.meshes[s.triMesh[tri]];
+    const uint32_t meshIndex = s.triMesh[tri];
+    const MeshInfo& mesh = s.meshes[meshIndex];
     const Material& m = s.materials[mesh.materialIndex];
     const float k = 1.0f / 255.0f;
     const float w = 1.0f - u - v;
     const float ca = (float)(word & 0xffu) * k * w + (float)((word >> 8) & 0xffu) * k * u +
                      (float)((word >> 16) & 0xffu) * k * v;
+    float ta = ca;
+    if (TEXA && (word & 0x40000000u) && s.texels)
+        ta = sampleMaterialSlot(s, m, VKRT_MATERIAL_TEXTURE_SLOT_BASE_COLOR, hitTexCoords(s, tri, meshIndex, u, v),
+                                f4(1.0f))
+                 .w *
+             ca;
     const float x = (float)(alphaMix(key ^ tri * 0x27D4EB2Fu) >> 8) * (1.0f / 16777216.0f);
     if (m.alphaMode == VKRT_MATERIAL_ALPHA_MODE_MASK) {
-        if (ca < m.alphaCutoff) return false;
+        if (ta < m.alphaCutoff) return false;
         const float mo = fminf(fmaxf(mesh.opacity * m.opacity * ca, 0.0f), 1.0f);
         if (mo >= 1.0f) return true;
         if (mo <= 0.0f) return false;
         return x <= mo;
     }
-    const float op = fminf(fmaxf(mesh.opacity * m.opacity * ca, 0.0f), 1.0f);
+    const float op = fminf(fmaxf(mesh.opacity * m.opacity * ta, 0.0f), 1.0f);
     const bool blend = m.alphaMode == VKRT_MATERIAL_ALPHA_MODE_BLEND || m.opacity < 0.999f || mesh.opacity < 0.999f;
     if (!blend) return true;
     if (op <= 0.0f) return false;
@@ -316,8 +326,9 @@ DEV void closestBegin(ClosestState& cs, TraversalStack& st, float tmax) {
 // nearest overlapping child is visited next and the others are pushed farthest
 // first; a popped entry whose entry distance lies beyond the closest hit found
 // so far is skipped. Returns false once the traversal has finished.
-// ALPHA = false compiles the alpha test out (scenes where every mesh is opaque).
-template <bool ALPHA = true>
+// ALPHA = false compiles the alpha test out (scenes where every mesh is opaque);
+// TEXA = false compiles out its texture-alpha lookup (no material textures).
+template <bool ALPHA = true, bool TEXA = true>
 DEV bool closestStep(const SceneView& s, const TraversalRay& r, float tmin, ClosestState& cs, TraversalStack& st,
                      uint32_t alphaKey, uint32_t* nodeVisits = nullptr, uint32_t* triTests = nullptr) {
     if (++cs.steps > kMaxTraversalSteps) {
@@ -350,7 +361,7 @@ DEV bool closestStep(const SceneView& s, const TraversalRay& r, float tmin, Clos
             float t, u, v;
             if (intersectTriangle(tr, r.o, r.pre, tmin, cs.h.t, t, u, v)) {
                 const uint32_t tri = __float_as_uint(tr.v0.w);
-                if (ALPHA && s.triAlpha && !alphaAccept(s, tri, u, v, alphaKey)) continue;
+                if (ALPHA && s.triAlpha && !alphaAccept<TEXA>(s, tri, u, v, alphaKey)) continue;
                 cs.h.t = t;
                 cs.h.tri = tri;
                 cs.h.u = u;

@@ -87,6 +87,9 @@ struct mrt_context {
     mrt::DeviceArray<uint32_t> triMesh;
     mrt::DeviceArray<uint32_t> triAlpha;  // per-triangle alpha words, valid when alphaEnabled
     bool alphaEnabled = false;
+    mrt::DeviceArray<float4> texels;  // material texture pool (linear RGBA)
+    mrt::DeviceArray<uint4> texInfo;  // per texture: first texel, width, height
+    bool texturesEnabled = false;     // some material references a texture
     mrt::DeviceArray<uint32_t> meshTriBase;
     mrt::DeviceArray<mrt::TriRecord> tris;
     mrt::DeviceArray<mrt::BvhNode> nodes;
@@ -154,6 +157,9 @@ struct mrt_context {
         v.rgb2spec = rgb2spec.ptr;
         v.shadeTris = shadeTris.ptr;
         v.triAlpha = alphaEnabled && triAlpha.count >= triCount ? triAlpha.ptr : nullptr;
+        v.texels = texturesEnabled ? texels.ptr : nullptr;
+        v.texInfo = texturesEnabled ? texInfo.ptr : nullptr;
+        v.texCount = texturesEnabled ? (uint32_t)texInfo.count : 0u;
         return v;
     }
 };

@@ -453,8 +453,9 @@ __global__ void kSetCount(uint32_t* counters, int slot, uint32_t value) { counte
 constexpr uint32_t kRefillLanes = 16;
 constexpr uint32_t kNoRay = 0xFFFFFFFFu;
 
-// ALPHA: some mesh runs the alpha test (the opaque variant keeps six waves per SIMD).
-template <bool COUNT, bool ALPHA>
+// ALPHA: some mesh runs the alpha test (the opaque variant keeps six waves per SIMD);
+// TEXA: some material has textures, so the test may read a texture's alpha.
+template <bool COUNT, bool ALPHA, bool TEXA = false>
 __global__ __launch_bounds__(kTraceBlock) void kExtend(SceneView sv, const float4* ro, const float4* rd, float4* hits,
                                                       uint32_t* counters, int countSlot, uint32_t* gstack,
                                                       uint32_t alphaSalt, uint32_t pathBase) {
@@ -515,7 +516,7 @@ __global__ __launch_bounds__(kTraceBlock) void kExtend(SceneView sv, const float
             }
         }
         if (rayId != kNoRay) {
-            if (!closestStep<ALPHA>(sv, r, kRayTMin, cs, st, alphaKey, COUNT ? &nodeVisits : nullptr,
+            if (!closestStep<ALPHA, TEXA>(sv, r, kRayTMin, cs, st, alphaKey, COUNT ? &nodeVisits : nullptr,
                                     COUNT ? &triTests : nullptr)) {
                 hits[rayId] = make_float4(cs.h.t, __uint_as_float(cs.h.tri), cs.h.u, cs.h.v);
                 rayId = kNoRay;
@@ -643,7 +644,7 @@ __global__ __launch_bounds__(kBlock) void kNrcTargets(const float4* record, cons
 // bit. What keeps the hero variant inside the budget: the rgb2spec resolution is
 // a compile-time constant, medium coefficients and the scalar-wavelength state
 // are read only where used, and shadow rays are queued before BSDF sampling.
-template <int MODE, bool NRC = false>
+template <int MODE, bool NRC = false, bool TEX = false>
 __global__ __launch_bounds__(kBlock, 3) void kShade(FrameParams fp, SceneView sv, PathView pv, QueueView in,
                                                  const float4* hits, QueueView out, ShadowView sh, ShadowView she,
                                                  uint32_t* counters, int inSlot, int outSlot, NrcView nv,
@@ -747,8 +748,8 @@ __global__ __launch_bounds__(kBlock, 3) void kShade(FrameParams fp, SceneView sv
                              : (MODE == kModeSingle ? !(thr.x > 0.0f) : (heroActive ? !anyPos(thr) : !(misc.y > 0.0f))))
             break;
 
-        Surface sf = reconstruct(sv, o, d, h);
-        BMat mat = loadMaterial(sv.materials, sf.materialIndex, sf.vertexColor);
+        Surface sf = reconstruct<TEX>(sv, o, d, h);
+        BMat mat = TEX ? loadSurfaceMaterial(sv, sf) : loadMaterial(sv.materials, sf.materialIndex, sf.vertexColor);
         F3 wo = toLocal(-d, sf.basis);
         const float stateLambda = MODE == kModeRgb ? 0.0f : (MODE == kModeSingle ? lam.x : (heroActive ? lam.x : misc.z));
         BState st = makeState(mat, wo, sf.frontFace, stateLambda, MODE == kModeRgb ? 0u : 1u);
@@ -1761,14 +1762,17 @@ static void launchExtend(mrt_context* ctx, bool countTraversal, const SceneView&
     const dim3 grid(ctx->traceGrid), block(kTraceBlock);
     uint32_t* stack = ctx->traversalStack.ptr;
     if (countTraversal && alpha)
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(kExtend<true, true>), grid, block, 0, s, sv, q.o, q.d, hits, cnt, slot, stack,
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(kExtend<true, true, true>), grid, block, 0, s, sv, q.o, q.d, hits, cnt, slot, stack,
                            salt, pathBase);
     else if (countTraversal)
         hipLaunchKernelGGL(HIP_KERNEL_NAME(kExtend<true, false>), grid, block, 0, s, sv, q.o, q.d, hits, cnt, slot, stack,
                            salt, pathBase);
+    else if (alpha && sv.texels)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(kExtend<false, true, true>), grid, block, 0, s, sv, q.o, q.d, hits, cnt,
+                           slot, stack, salt, pathBase);
     else if (alpha)
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(kExtend<false, true>), grid, block, 0, s, sv, q.o, q.d, hits, cnt, slot,
-                           stack, salt, pathBase);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(kExtend<false, true, false>), grid, block, 0, s, sv, q.o, q.d, hits, cnt,
+                           slot, stack, salt, pathBase);
     else
         hipLaunchKernelGGL(HIP_KERNEL_NAME(kExtend<false, false>), grid, block, 0, s, sv, q.o, q.d, hits, cnt, slot,
                            stack, salt, pathBase);
@@ -1823,9 +1827,19 @@ static bool launchChunk(mrt_context* ctx, FrameParams fp, StageEvents& ev, bool 
                                b.sortOffset.ptr, nb, b.sortPerm.ptr);
             order = b.sortPerm.ptr;
         }
-        if (MODE == kModeRgb && nrc)
+        // textured materials take a separate variant so that untextured scenes
+        // keep the plain kernel's register budget
+        const bool tex = sv.texels != nullptr;
+        if (MODE == kModeRgb && nrc && tex)
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(kShade<kModeRgb, true, true>), dim3(blocksFor(fp.count)), dim3(kBlock),
+                               0, s, fp, sv, pv, qv[cur], b.hits.ptr, qv[nxt], shv, shev, cnt, cur, nxt, nrc->view,
+                               order);
+        else if (MODE == kModeRgb && nrc)
             hipLaunchKernelGGL(HIP_KERNEL_NAME(kShade<kModeRgb, true>), dim3(blocksFor(fp.count)), dim3(kBlock), 0, s,
                                fp, sv, pv, qv[cur], b.hits.ptr, qv[nxt], shv, shev, cnt, cur, nxt, nrc->view, order);
+        else if (tex)
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(kShade<MODE, false, true>), dim3(blocksFor(fp.count)), dim3(kBlock), 0,
+                               s, fp, sv, pv, qv[cur], b.hits.ptr, qv[nxt], shv, shev, cnt, cur, nxt, NrcView{}, order);
         else
             hipLaunchKernelGGL(HIP_KERNEL_NAME(kShade<MODE, false>), dim3(blocksFor(fp.count)), dim3(kBlock), 0, s, fp,
                                sv, pv, qv[cur], b.hits.ptr, qv[nxt], shv, shev, cnt, cur, nxt, NrcView{}, order);
@@ -2135,6 +2149,10 @@ extern "C" int mrt_render_backward(mrt_context* ctx, const SceneData* scene, con
     if (rc != 0) return rc;
     if (fs.mode != kModeRgb) {
         setError("render_backward supports the RGB render mode");
+        return -1;
+    }
+    if (ctx->texturesEnabled) {
+        setError("render_backward does not differentiate through material textures");
         return -1;
     }
     if (g->materialCount != (uint32_t)ctx->materials.count) {

@@ -174,6 +174,9 @@ static VKRT_Result syncScene(VKRT* vkrt, int requireDevice) {
     if (mrt_upload_materials(vkrt->device, p->materials, p->materialCount) != 0) return deviceError();
     if ((geometryDirty || materialDirty) && mrt_upload_alpha(vkrt->device, p->triangleAlpha, p->triangleCount) != 0)
         return deviceError();
+    if ((geometryDirty || materialDirty) &&
+        mrt_upload_textures(vkrt->device, p->textureTexels, p->textureTexelCount, p->textureInfo, p->textureCount) != 0)
+        return deviceError();
     if (mrt_upload_meshes(vkrt->device, p->meshInfos, vkrt->meshCount) != 0) return deviceError();
     mrt_light_desc l = {p->emissiveMeshes, p->emissiveMeshCount, p->emissiveTriangles, p->emissiveTriangleCount,
                         p->meshAliasQ,     p->meshAliasIdx,      p->triAliasQ,         p->triAliasIdx,
@@ -370,6 +373,10 @@ VKRT_Result VKRT_renderBackward(VKRT* vkrt, const float* dLossdImage, VKRT_Gradi
         return VKRT_ERROR_INVALID_ARGUMENT;
     }
     if (out->materialCount != vkrt->packed.materialCount) return VKRT_ERROR_INVALID_ARGUMENT;
+    if (vkrt->packed.textureTexelCount) {
+        VKRT_LOG_ERROR("VKRT_renderBackward does not differentiate through material textures");
+        return VKRT_ERROR_INVALID_ARGUMENT;
+    }
     mrt_render_params rp;
     fillRenderParams(vkrt, &rp);
     if (mrt_resize_accumulation(vkrt->device, vkrt->renderWidth, vkrt->renderHeight) != 0) return deviceError();
@@ -455,6 +462,10 @@ VKRT_Result VKRT_exportSceneBuffers(VKRT* vkrt, VKRT_SceneBuffers* out) {
     out->rngSeed = vkrt->rngSeed;
     out->environmentImportanceSampling = vkrt->envImportance;
     out->triangleAlpha = p->triangleAlpha;
+    out->textureTexels = p->textureTexels;
+    out->textureTexelCount = p->textureTexelCount;
+    out->textureInfo = p->textureInfo;
+    out->textureCount = p->textureCount;
     if (out->environmentPixels) {
         out->environmentAliasQ = p->envAliasQ;
         out->environmentAliasIdx = p->envAliasIdx;

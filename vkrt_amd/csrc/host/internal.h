@@ -74,6 +74,13 @@ typedef struct PackedScene {
      * bit 31 = the triangle's mesh may reject a hit, bits 0-23 = the alpha bytes
      * of its three vertex colours; NULL when every mesh is opaque */
     uint32_t* triangleAlpha;
+    /* material texture pool: linear RGBA32F texels of every texture a material
+     * references, and per texture {first texel, width, height, 0} (width 0 for
+     * textures no material uses) */
+    float* textureTexels;
+    uint64_t textureTexelCount;
+    uint32_t* textureInfo;
+    uint32_t textureCount;
 } PackedScene;
 
 struct VKRT {

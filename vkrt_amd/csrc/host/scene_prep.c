@@ -166,6 +166,8 @@ void vkrtFreePackedScene(PackedScene* p) {
     free(p->envAliasIdx);
     free(p->envPdf);
     free(p->triangleAlpha);
+    free(p->textureTexels);
+    free(p->textureInfo);
     memset(p, 0, sizeof(*p));
 }
 
@@ -407,6 +409,13 @@ static int meshMayRejectHit(const MeshInfo* info, const Material* m) {
            m->opacity < 0.999f || info->opacity < 0.999f;
 }
 
+/* The alpha test reads the base-colour texture's alpha for MASK and BLEND
+ * materials (src/shaders/rt/alpha_test.slang:20-26); other materials use the
+ * vertex alpha alone. */
+static int meshSamplesTextureAlpha(const Material* m) {
+    return m->alphaMode != VKRT_MATERIAL_ALPHA_MODE_OPAQUE && m->baseColorTextureIndex != VKRT_INVALID_INDEX;
+}
+
 static VKRT_Result buildTriangleAlpha(VKRT* vkrt, PackedScene* p) {
     int any = 0;
     for (uint32_t mi = 0; mi < vkrt->meshCount && !any; mi++)
@@ -419,7 +428,7 @@ static VKRT_Result buildTriangleAlpha(VKRT* vkrt, PackedScene* p) {
         if (!meshMayRejectHit(info, &p->materials[info->materialIndex])) continue;
         uint32_t tris = info->indexCount / 3u, tbase = p->meshTriangleBase[mi];
         for (uint32_t t = 0; t < tris; t++) {
-            uint32_t word = 0x80000000u;
+            uint32_t word = 0x80000000u | (meshSamplesTextureAlpha(&p->materials[info->materialIndex]) ? 0x40000000u : 0u);
             for (uint32_t c = 0; c < 3u; c++) {
                 uint32_t vi = p->indices[info->indexBase + t * 3u + c];
                 if (vi >= info->vertexCount) vi = 0;
@@ -428,6 +437,63 @@ static VKRT_Result buildTriangleAlpha(VKRT* vkrt, PackedScene* p) {
             p->triangleAlpha[tbase + t] = word;
         }
     }
+    return VKRT_SUCCESS;
+}
+
+/* Texture indices a material holds that name no texture are treated as
+ * empty slots; the pool then holds each referenced texture once, row-major
+ * from row 0 (texture coordinate v = 0), as the device samples it. */
+static VKRT_Result buildTexturePool(VKRT* vkrt, PackedScene* p) {
+    uint32_t tc = vkrt->textureCount;
+    for (uint32_t i = 0; i < p->materialCount; i++) {
+        Material* m = &p->materials[i];
+        uint32_t* slots[4] = {&m->baseColorTextureIndex, &m->metallicRoughnessTextureIndex, &m->normalTextureIndex,
+                              &m->emissiveTextureIndex};
+        for (int k = 0; k < 4; k++)
+            if (*slots[k] >= tc) *slots[k] = VKRT_INVALID_INDEX;
+    }
+    if (tc == 0) return VKRT_SUCCESS;
+    uint8_t* used = (uint8_t*)calloc(tc, 1);
+    p->textureInfo = (uint32_t*)calloc((size_t)tc * 4u, sizeof(uint32_t));
+    if (!used || !p->textureInfo) {
+        free(used);
+        return VKRT_ERROR_OUT_OF_MEMORY;
+    }
+    p->textureCount = tc;
+    for (uint32_t i = 0; i < p->materialCount; i++) {
+        const Material* m = &p->materials[i];
+        uint32_t slots[4] = {m->baseColorTextureIndex, m->metallicRoughnessTextureIndex, m->normalTextureIndex,
+                             m->emissiveTextureIndex};
+        for (int k = 0; k < 4; k++)
+            if (slots[k] != VKRT_INVALID_INDEX) used[slots[k]] = 1;
+    }
+    uint64_t total = 0;
+    for (uint32_t t = 0; t < tc; t++) {
+        if (!used[t]) continue;
+        const HostTexture* tex = &vkrt->textures[t];
+        if (total + (uint64_t)tex->width * tex->height > 0xFFFFFFFFull) {
+            free(used);
+            VKRT_LOG_ERROR("material textures exceed 2^32 texels");
+            return VKRT_ERROR_OUT_OF_MEMORY;
+        }
+        p->textureInfo[4u * t + 0u] = (uint32_t)total;
+        p->textureInfo[4u * t + 1u] = tex->width;
+        p->textureInfo[4u * t + 2u] = tex->height;
+        total += (uint64_t)tex->width * tex->height;
+    }
+    p->textureTexelCount = total;
+    if (total) {
+        p->textureTexels = (float*)malloc((size_t)total * 4u * sizeof(float));
+        if (!p->textureTexels) {
+            free(used);
+            return VKRT_ERROR_OUT_OF_MEMORY;
+        }
+        for (uint32_t t = 0; t < tc; t++)
+            if (used[t])
+                memcpy(p->textureTexels + (size_t)p->textureInfo[4u * t] * 4u, vkrt->textures[t].rgba,
+                       (size_t)vkrt->textures[t].width * vkrt->textures[t].height * 4u * sizeof(float));
+    }
+    free(used);
     return VKRT_SUCCESS;
 }
 
@@ -457,6 +523,7 @@ VKRT_Result vkrtPrepareScene(VKRT* vkrt) {
     VKRT_Result r = buildGeometry(vkrt, &next);
     if (r == VKRT_SUCCESS) r = buildLights(vkrt, &next);
     if (r == VKRT_SUCCESS) r = buildEnvironmentTable(vkrt, &next);
+    if (r == VKRT_SUCCESS) r = buildTexturePool(vkrt, &next);
     if (r == VKRT_SUCCESS) r = buildTriangleAlpha(vkrt, &next);
     if (r != VKRT_SUCCESS) {
         vkrtFreePackedScene(&next);

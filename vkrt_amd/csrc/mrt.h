@@ -56,6 +56,10 @@ int mrt_upload_materials(mrt_context* ctx, const Material* materials, uint32_t m
 /* Per-triangle alpha words (bit 31: the mesh runs the alpha test; bits 0-23:
  * vertex colour alphas); NULL turns the alpha test off. */
 int mrt_upload_alpha(mrt_context* ctx, const uint32_t* triangleAlpha, uint32_t triangleCount);
+/* Material texture pool: texelCount linear RGBA32F texels and 4 words per
+ * texture (first texel, width, height, 0); no texels turns texturing off. */
+int mrt_upload_textures(mrt_context* ctx, const float* texels, uint64_t texelCount, const uint32_t* info,
+                        uint32_t textureCount);
 
 typedef struct mrt_light_desc {
     const EmissiveMesh* emissiveMeshes;

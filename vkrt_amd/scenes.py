@@ -38,6 +38,7 @@ class MeshData:
 class Scene:
     meshes: list = field(default_factory=list)
     materials: list = field(default_factory=list)
+    textures: list = field(default_factory=list)  # (H x W x 4 pixels, format, colour space), added first
     camera: tuple = ((0.0, -1.0, 0.0), (0.0, 0.0, 0.0), (0.0, 0.0, 1.0), 40.0)
     environment: np.ndarray | None = None  # H x W x 4 float32
     environment_color: tuple = (0.0, 0.0, 0.0)
@@ -132,6 +133,126 @@ def alpha_cornell() -> Scene:
     colors[:, 3] = [0.0, 1.0, 1.0, 0.0]
     s.meshes.append(MeshData(make_vertices(pos, nrm, colors), idx, len(s.materials) - 1))
     s.meshes[1].opacity = 0.35  # the red wall
+    return s
+
+
+def _wrap(u, v):
+    return int(u) | (int(v) << 16)
+
+
+def _textured_quad(corners, normal, uv_scale1=2.0):
+    pos, nrm, idx = _quad(*corners, normal)
+    uv = np.array([[0, 0], [1, 0], [1, 1], [0, 1]], dtype=np.float32)
+    return make_vertices(pos, nrm, texcoord0=uv, texcoord1=uv * uv_scale1), idx
+
+
+def textured_cornell() -> Scene:
+    """A Cornell box whose materials sample every kind of material texture
+    (src/shaders/material/textures.slang): an sRGB checker on the floor with a
+    scaled, rotated and offset UV transform; a normal map and a 16-bit
+    metallic-roughness map on a glossy back wall (mirrored / clamped wrap); a
+    half-float base colour on the red wall read through texture-coordinate
+    set 1; an emissive panel with a mirrored-repeat emissive map; a MASK quad
+    cut by its base-colour texture's alpha and a BLEND sheet whose opacity is
+    that alpha. Texture-coordinate set 0 runs 0 -> 1 across every quad, set 1
+    twice as fast."""
+    import vkrt_amd as V
+    n = 32
+    y, x = np.mgrid[0:n, 0:n]
+    checker = np.zeros((n, n, 4), np.uint8)
+    on = ((x // 4 + y // 4) % 2) == 1
+    checker[..., 0] = np.where(on, 230, 40)
+    checker[..., 1] = np.where(on, 200, 90 + 4 * x)
+    checker[..., 2] = np.where(on, 60, 50 + 5 * y)
+    checker[..., 3] = 255
+    bump = np.zeros((n, n, 4), np.uint8)  # tangent-space normals of a sinusoidal height field
+    gx = 0.6 * np.cos(2 * np.pi * x / n * 3)
+    gy = 0.6 * np.sin(2 * np.pi * y / n * 2)
+    nv = np.stack([-gx, -gy, np.ones_like(gx)], -1)
+    nv /= np.linalg.norm(nv, axis=-1, keepdims=True)
+    bump[..., :3] = np.round((nv * 0.5 + 0.5) * 255).astype(np.uint8)
+    bump[..., 3] = 255
+    m = 16
+    yy, xx = np.mgrid[0:m, 0:m]
+    mr = np.zeros((m, m, 4), np.uint16)
+    mr[..., 1] = (65535 * (0.15 + 0.85 * xx / (m - 1))).astype(np.uint16)  # roughness factor
+    mr[..., 2] = np.where((yy // 4) % 2 == 0, 65535, 6000).astype(np.uint16)  # metallic factor
+    mr[..., 3] = 65535
+    red = np.zeros((8, 8, 4), np.float16)
+    gy8, gx8 = np.mgrid[0:8, 0:8]
+    red[..., 0] = 0.5 + 0.5 * (gx8 / 7)
+    red[..., 1] = 0.2 + 0.6 * ((gx8 + gy8) % 2)
+    red[..., 2] = 0.3 * gy8 / 7
+    red[..., 3] = 1.0
+    emis = np.zeros((8, 8, 4), np.uint8)
+    emis[..., 0] = 255
+    emis[..., 1] = (gx8 * 36).astype(np.uint8)
+    emis[..., 2] = ((gx8 + gy8) % 2 * 255).astype(np.uint8)
+    emis[..., 3] = 255
+    holes = np.full((m, m, 4), 255, np.uint8)
+    cx = (xx % 8) - 3.5
+    cy = (yy % 8) - 3.5
+    holes[..., 3] = np.where(cx * cx + cy * cy < 7.0, 0, np.clip(40 + 13 * xx, 0, 255)).astype(np.uint8)
+    holes[..., 0] = 90
+    holes[..., 1] = 140
+    holes[..., 2] = 220
+    srgb, lin = V.TEXTURE_COLOR_SPACE_SRGB, V.TEXTURE_COLOR_SPACE_LINEAR
+    s = Scene()
+    s.textures = [(checker, V.TEXTURE_FORMAT_RGBA8_UNORM, srgb), (bump, V.TEXTURE_FORMAT_RGBA8_UNORM, lin),
+                  (mr, V.TEXTURE_FORMAT_RGBA16_UNORM, lin), (red, V.TEXTURE_FORMAT_RGBA16_SFLOAT, lin),
+                  (emis, V.TEXTURE_FORMAT_RGBA8_UNORM, srgb), (holes, V.TEXTURE_FORMAT_RGBA8_UNORM, srgb)]
+    floor = lambert((0.9, 0.9, 0.9))
+    floor.baseColorTextureIndex = 0
+    floor.baseColorTextureTransform[:] = (3.0, 3.0, 0.1, 0.2)
+    floor.textureRotations[0] = 0.4
+    back = default_material(baseColor=(0.8, 0.8, 0.75), roughness=0.6, metallic=0.9, specular=0.5)
+    back.normalTextureIndex = 1
+    back.normalTextureScale = 0.8
+    back.normalTextureWrap = _wrap(V.TEXTURE_WRAP_MIRRORED_REPEAT, V.TEXTURE_WRAP_CLAMP_TO_EDGE)
+    back.normalTextureTransform[:] = (2.5, 1.5, -0.3, 0.0)
+    back.metallicRoughnessTextureIndex = 2
+    back.metallicRoughnessTextureWrap = _wrap(V.TEXTURE_WRAP_CLAMP_TO_EDGE, V.TEXTURE_WRAP_CLAMP_TO_EDGE)
+    back.metallicRoughnessTextureTransform[:] = (1.4, 1.4, -0.2, -0.2)
+    redm = lambert((1.0, 0.3, 0.3))
+    redm.baseColorTextureIndex = 3
+    redm.textureTexcoordSets = 1 << (8 * 0)  # base colour from set 1
+    redm.baseColorTextureWrap = _wrap(V.TEXTURE_WRAP_MIRRORED_REPEAT, V.TEXTURE_WRAP_REPEAT)
+    panel = lambert((0.2, 0.2, 0.2), emission=4.0)
+    panel.emissiveTextureIndex = 4
+    panel.emissiveTextureWrap = _wrap(V.TEXTURE_WRAP_MIRRORED_REPEAT, V.TEXTURE_WRAP_MIRRORED_REPEAT)
+    panel.emissiveTextureTransform[:] = (2.0, 2.0, 0.0, 0.0)
+    mask = lambert((1.0, 1.0, 1.0))
+    mask.baseColorTextureIndex = 5
+    mask.alphaMode = 1  # MASK
+    mask.alphaCutoff = 0.5
+    blend = lambert((1.0, 0.9, 0.7))
+    blend.baseColorTextureIndex = 5
+    blend.alphaMode = 2  # BLEND
+    blend.baseColorTextureTransform[:] = (2.0, 2.0, 0.0, 0.0)
+    s.materials = [floor, back, redm, lambert((0.14, 0.45, 0.09)), lambert((0.73, 0.73, 0.73), emission=15.0),
+                   panel, mask, blend, lambert((0.73, 0.73, 0.73))]
+    quads = [
+        (((0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0)), (0, 0, 1), 0),  # floor
+        (((0, 0, 1), (0, 1, 1), (1, 1, 1), (1, 0, 1)), (0, 0, -1), 8),  # ceiling
+        (((0, 1, 0), (1, 1, 0), (1, 1, 1), (0, 1, 1)), (0, -1, 0), 1),  # back
+        (((0, 0, 0), (0, 1, 0), (0, 1, 1), (0, 0, 1)), (1, 0, 0), 2),  # left (red)
+        (((1, 0, 0), (1, 0, 1), (1, 1, 1), (1, 1, 0)), (-1, 0, 0), 3),  # right (green)
+    ]
+    for corners, normal, mat in quads:
+        s.meshes.append(MeshData(*_textured_quad(corners, normal), mat))
+    zl = 1.0 - 1e-3
+    light = ((0.37, 0.37, zl), (0.63, 0.37, zl), (0.63, 0.63, zl), (0.37, 0.63, zl))
+    s.meshes.append(MeshData(*_textured_quad(light, (0, 0, -1)), 4))
+    panel_q = ((0.02, 0.55, 0.15), (0.02, 0.85, 0.15), (0.02, 0.85, 0.45), (0.02, 0.55, 0.45))
+    s.meshes.append(MeshData(*_textured_quad(panel_q, (1, 0, 0)), 5))
+    mask_q = ((0.15, 0.85, 0.05), (0.85, 0.85, 0.05), (0.85, 0.85, 0.75), (0.15, 0.85, 0.05 + 0.7))
+    s.meshes.append(MeshData(*_textured_quad(mask_q, (0, -1, 0)), 6))
+    zs = 0.82
+    sheet = ((0.3, 0.25, zs), (0.7, 0.25, zs), (0.7, 0.65, zs), (0.3, 0.65, zs))
+    s.meshes.append(MeshData(*_textured_quad(sheet, (0, 0, -1)), 7))
+    s.camera = ((0.5, -1.35, 0.5), (0.5, 0.5, 0.5), (0.0, 0.0, 1.0), 40.0)
+    s.environment_color = (0.0, 0.0, 0.0)
+    s.environment_strength = 0.0
     return s
 
 
@@ -370,6 +491,8 @@ def sky_environment(width: int = 2048, height: int = 1024, sun_elevation_deg: fl
 
 def load(renderer: Renderer, scene: Scene):
     """Creates the scene's materials and meshes through the C API."""
+    for i, (pixels, fmt, space) in enumerate(scene.textures):
+        renderer.add_texture(pixels, fmt, space, f"t{i}")
     mat_ids = [renderer.add_material(m, f"m{i}") for i, m in enumerate(scene.materials)]
     for mesh in scene.meshes:
         idx = renderer.add_mesh(mesh.vertices, mesh.indices)
