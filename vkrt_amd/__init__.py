@@ -701,7 +701,17 @@ class Renderer:
             info.width, info.height, info.headless = width, height, 1
             _ok(self.lib.VKRT_initWithCreateInfo(self.h, C.byref(info)), "VKRT_initWithCreateInfo")
             self.device_ready = True
+        # a fixed sample count per frame, as the reference's offline benchmark
+        # locks it (src/app/render/benchmark.c:55-67); auto_spp() turns the
+        # frame-time controller back on
+        _ok(self.lib.VKRT_setAutoSPPEnabled(self.h, 0), "VKRT_setAutoSPPEnabled")
         self._keep = []
+
+    def auto_spp(self, enabled: bool, target_fps: int | None = None):
+        """VKRT_setAutoSPPEnabled / VKRT_setAutoSPPTargetFPS."""
+        if target_fps is not None:
+            _ok(self.lib.VKRT_setAutoSPPTargetFPS(self.h, target_fps), "VKRT_setAutoSPPTargetFPS")
+        _ok(self.lib.VKRT_setAutoSPPEnabled(self.h, 1 if enabled else 0), "VKRT_setAutoSPPEnabled")
 
     def close(self):
         if self.h:
@@ -850,6 +860,7 @@ class Renderer:
                   rng=None, seed=0, env_importance=None, nee=None):
         L = self.lib
         if spp is not None:
+            _ok(L.VKRT_setAutoSPPEnabled(self.h, 0), "VKRT_setAutoSPPEnabled")  # an explicit count is kept
             _ok(L.VKRT_setSamplesPerPixel(self.h, spp), "setSamplesPerPixel")
         if max_depth is not None:
             _ok(L.VKRT_setPathDepth(self.h, min_depth if min_depth is not None else 4, max_depth), "setPathDepth")

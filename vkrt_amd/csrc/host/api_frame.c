@@ -284,6 +284,7 @@ VKRT_Result VKRT_endFrame(VKRT* vkrt) {
     if (!vkrt) return VKRT_ERROR_INVALID_ARGUMENT;
     if (vkrt->frameTraced && sessionSampling(vkrt)) {
         uint32_t spp = vkrt->sceneData.samplesPerPixel ? vkrt->sceneData.samplesPerPixel : 1u;
+        vkrtUpdateAutoSPP(vkrt); /* src/core/api/frame.c:380-381 */
         vkrt->renderStatus.accumulationFrame++;
         vkrt->renderStatus.totalSamples += spp;
         vkrt->sceneData.frameNumber++;
@@ -317,6 +318,8 @@ VKRT_Result VKRT_startRender(VKRT* vkrt, uint32_t width, uint32_t height, uint32
     vkrt->renderHeight = height;
     vkrt->renderStatus.renderPhase = VKRT_RENDER_PHASE_SAMPLING;
     vkrt->renderStatus.renderTargetSamples = targetSamples;
+    vkrtResetAutoSPPState(vkrt, 1); /* src/core/api/render.c:275 */
+    vkrtWriteSceneStateUniform(vkrt);
     return VKRT_setRenderViewport(vkrt, 0, 0, width, height);
 }
 

@@ -75,7 +75,65 @@ void vkrtWriteSceneStateUniform(VKRT* vkrt) {
 }
 
 /* src/core/scene/uniform.c:227-245: any change restarts accumulation. */
+/* src/core/scene/timing.c:62-75: a scene change restarts the controller at
+ * one sample per pixel; a settings change only forgets its measurements. */
+void vkrtResetAutoSPPState(VKRT* vkrt, int resetSamplesPerPixel) {
+    if (resetSamplesPerPixel && vkrt->settings.autoSPPEnabled) vkrt->settings.samplesPerPixel = 1u;
+    vkrt->autoSPPControlMs = 0.0f;
+    vkrt->renderStatus.displayTimeMs = 0.0f;
+    vkrt->renderStatus.renderTimeMs = 0.0f;
+    vkrt->renderStatus.displayRenderTimeMs = 0.0f;
+    vkrt->renderStatus.displayFrameTimeMs = 0.0f;
+    vkrt->renderStatus.framesPerSecond = 0u;
+}
+
+/* src/core/scene/timing.c:121-178, run once per frame that added samples:
+ * the trace time per sample is smoothed, the sample count aimed at 90 % of
+ * the frame budget, moves inside a dead band are ignored, and one step is
+ * limited to x1.25 up and x0.6 down, within [1, 2048]. */
+void vkrtUpdateAutoSPP(VKRT* vkrt) {
+    if (!vkrt->settings.autoSPPEnabled) return;
+    const float smoothing = 0.35f, budgetScale = 0.90f, upDeadband = 0.18f, downDeadband = 0.08f;
+    const float maxUp = 1.25f, maxDown = 0.60f;
+    float targetMs = vkrt->autoSPPTargetFrameMs;
+    if (!(targetMs > 0.0f)) {
+        uint32_t fps = vkrt->settings.autoSPPTargetFPS ? vkrt->settings.autoSPPTargetFPS : 60u;
+        targetMs = 1000.0f / (float)fps;
+    }
+    float controlMs = vkrt->renderStatus.renderTimeMs;
+    if (!(controlMs > 0.0f)) controlMs = vkrt->renderStatus.displayRenderTimeMs;
+    if (!(controlMs > 0.0f)) controlMs = vkrt->renderStatus.displayFrameTimeMs;
+    if (!(controlMs > 0.0f)) return;
+    uint32_t spp = vkrt->settings.samplesPerPixel ? vkrt->settings.samplesPerPixel : 1u;
+    float sppf = (float)spp;
+    float perSample = controlMs / sppf;
+    if (!(perSample > 0.0f)) return;
+    vkrt->autoSPPControlMs = vkrt->autoSPPControlMs <= 0.0f
+                                 ? perSample
+                                 : vkrt->autoSPPControlMs * (1.0f - smoothing) + perSample * smoothing;
+    float desired = (targetMs * budgetScale) / vkrt->autoSPPControlMs;
+    desired = fminf(fmaxf(desired, 1.0f), 2048.0f);
+    float delta = desired - sppf;
+    float deadband = fmaxf(sppf * (delta > 0.0f ? upDeadband : downDeadband), 1.0f);
+    if (fabsf(delta) <= deadband) return;
+    uint32_t next;
+    if (delta > 0.0f) {
+        next = (uint32_t)floorf(fminf(desired, ceilf(sppf * maxUp)));
+        if (next <= spp && spp < 2048u) next = spp + 1u;
+    } else {
+        next = (uint32_t)ceilf(fmaxf(desired, floorf(sppf * maxDown)));
+        if (next >= spp && spp > 1u) next = spp - 1u;
+    }
+    if (next < 1u) next = 1u;
+    if (next > 2048u) next = 2048u;
+    if (next != spp) {
+        vkrt->settings.samplesPerPixel = next;
+        vkrtWriteSceneStateUniform(vkrt); /* the accumulation continues */
+    }
+}
+
 void vkrtResetSceneData(VKRT* vkrt) {
+    vkrtResetAutoSPPState(vkrt, 1);
     vkrt->sceneData.frameNumber = 0;
     if (vkrt->renderStatus.renderPhase != VKRT_RENDER_PHASE_INACTIVE) {
         vkrt->renderStatus.renderPhase = VKRT_RENDER_PHASE_SAMPLING;
@@ -118,14 +176,18 @@ VKRT_Result VKRT_setPathDepth(VKRT* vkrt, uint32_t rrMinDepth, uint32_t rrMaxDep
 VKRT_Result VKRT_setAutoSPPEnabled(VKRT* vkrt, uint8_t enabled) {
     if (!vkrt) return VKRT_ERROR_INVALID_ARGUMENT;
     vkrt->settings.autoSPPEnabled = enabled ? 1u : 0u;
+    vkrtResetAutoSPPState(vkrt, 0);
     return VKRT_SUCCESS;
 }
 
 VKRT_Result VKRT_setAutoSPPTargetFPS(VKRT* vkrt, uint32_t targetFPS) {
     if (!vkrt) return VKRT_ERROR_INVALID_ARGUMENT;
+    if (targetFPS == 0) targetFPS = 60; /* the display refresh rate; no display here */
     if (targetFPS < 30) targetFPS = 30;
     if (targetFPS > 360) targetFPS = 360;
     vkrt->settings.autoSPPTargetFPS = targetFPS;
+    vkrt->autoSPPTargetFrameMs = 1000.0f / (float)targetFPS;
+    vkrtResetAutoSPPState(vkrt, 0);
     return VKRT_SUCCESS;
 }
 

@@ -85,6 +85,10 @@ typedef struct PackedScene {
 
 struct VKRT {
     VKRT_SceneSettingsSnapshot settings;
+    /* auto-SPP controller (src/core/scene/timing.c:44-178): smoothed trace
+     * time per sample, and the frame budget set through the target FPS */
+    float autoSPPControlMs;
+    float autoSPPTargetFrameMs;
     SceneData sceneData;
     uint32_t renderWidth;
     uint32_t renderHeight;
@@ -145,6 +149,8 @@ void vkrtMeshWorldTransformFromInfo(const MeshInfo* info, float out[4][4]);
 
 /* settings (api_settings.c) */
 void vkrtResetSceneData(VKRT* vkrt);
+void vkrtResetAutoSPPState(VKRT* vkrt, int resetSamplesPerPixel);
+void vkrtUpdateAutoSPP(VKRT* vkrt);
 void vkrtWriteSceneStateUniform(VKRT* vkrt);
 void vkrtInitDefaultSettings(VKRT* vkrt, uint32_t width, uint32_t height);
 
