@@ -24,6 +24,7 @@
  */
 #include "internal.h"
 #include "json.h"
+#include "png_decode.h"
 #include "vkrt_import.h"
 
 #include <float.h>
@@ -534,6 +535,163 @@ static Material buildMaterial(const JsonValue* src, uint32_t* outTextureRefs) {
     }
     if (outTextureRefs) *outTextureRefs += refs;
     return m;
+}
+
+static char* duplicateString(const char* s);
+static const char* jsonName(const JsonValue* object);
+
+/* ------------------------------------------------------------- textures */
+
+static float srgbToLinear(float c) { return c <= 0.04045f ? c / 12.92f : powf((c + 0.055f) / 1.055f, 2.4f); }
+
+/* One image of the file as a texture entry (src/app/mesh/loader.c:432-499),
+ * shared by every view of the same image in the same colour space. PNG
+ * images are decoded; 16-bit sRGB images are stored as linear floats, since
+ * a 16-bit texture upload is linear only. */
+static uint32_t importTexture(GltfDoc* doc, MeshImportData* out, uint32_t* keys, uint32_t textureIndex,
+                              uint32_t colorSpace, uint32_t* outWrap) {
+    const JsonValue* tex = vkrtJsonAt(vkrtJsonGet(&doc->root, "textures"), textureIndex);
+    uint32_t source = 0, samplerIndex = 0;
+    *outWrap = VKRT_TEXTURE_WRAP_DEFAULT;
+    if (!tex || !vkrtJsonToU32(vkrtJsonGet(tex, "source"), &source)) return VKRT_INVALID_INDEX;
+    if (vkrtJsonToU32(vkrtJsonGet(tex, "sampler"), &samplerIndex)) {
+        const JsonValue* smp = vkrtJsonAt(vkrtJsonGet(&doc->root, "samplers"), samplerIndex);
+        uint32_t ws = VKRT_TEXTURE_WRAP_REPEAT, wt = VKRT_TEXTURE_WRAP_REPEAT; /* loader.c:362-366 */
+        uint32_t v = 0;
+        if (smp && vkrtJsonToU32(vkrtJsonGet(smp, "wrapS"), &v)) ws = v;
+        if (smp && vkrtJsonToU32(vkrtJsonGet(smp, "wrapT"), &v)) wt = v;
+        *outWrap = ws | (wt << 16);
+    }
+    const uint32_t key = (source << 1) | (colorSpace == VKRT_TEXTURE_COLOR_SPACE_SRGB);
+    for (uint32_t i = 0; i < out->textureCount; i++)
+        if (keys[i] == key) return i;
+    const JsonValue* image = vkrtJsonAt(vkrtJsonGet(&doc->root, "images"), source);
+    if (!image) return VKRT_INVALID_INDEX;
+    const JsonValue* uri = vkrtJsonGet(image, "uri");
+    uint8_t* owned = NULL;
+    const uint8_t* bytes = NULL;
+    size_t size = 0;
+    uint32_t view = 0;
+    if (uri && uri->type == JSON_STRING && strncmp(uri->string, "data:", 5) == 0) {
+        const char* comma = strchr(uri->string, ',');
+        if (comma) owned = decodeBase64(comma + 1, &size);
+        bytes = owned;
+    } else if (uri && uri->type == JSON_STRING) {
+        char rel[4096], full[8192];
+        if (decodeUriPath(uri->string, rel, sizeof(rel))) {
+            snprintf(full, sizeof(full), "%s/%s", doc->directory, rel);
+            owned = readWholeFile(full, &size);
+        }
+        bytes = owned;
+    } else if (vkrtJsonToU32(vkrtJsonGet(image, "bufferView"), &view)) {
+        uint32_t length = 0;
+        size_t stride = 0;
+        const JsonValue* bv = vkrtJsonAt(doc->bufferViews, view);
+        if (bv && vkrtJsonToU32(vkrtJsonGet(bv, "byteLength"), &length) && length &&
+            resolveView(doc, view, 0, 1, length, 0, &bytes, &stride) == 0)
+            size = length;
+    }
+    PngImage png;
+    int decoded = bytes && size && vkrtDecodePng(bytes, size, &png) == 0;
+    free(owned);
+    if (!decoded) {
+        GLTF_LOG("glTF image %u is not a PNG image this importer decodes; its texture is skipped", source);
+        return VKRT_INVALID_INDEX;
+    }
+    TextureImportEntry e;
+    memset(&e, 0, sizeof(e));
+    e.width = png.width;
+    e.height = png.height;
+    e.pixels = png.pixels;
+    e.format = VKRT_TEXTURE_FORMAT_RGBA8_UNORM;
+    e.colorSpace = colorSpace;
+    if (png.bits == 16) {
+        e.format = VKRT_TEXTURE_FORMAT_RGBA16_UNORM;
+        if (colorSpace == VKRT_TEXTURE_COLOR_SPACE_SRGB) {
+            const size_t n = (size_t)png.width * png.height * 4u;
+            float* f = (float*)malloc(n * sizeof(float));
+            if (!f) {
+                free(png.pixels);
+                return VKRT_INVALID_INDEX;
+            }
+            for (size_t i = 0; i < n; i++) {
+                const float v = (float)((uint16_t*)png.pixels)[i] / 65535.0f;
+                f[i] = (i & 3u) == 3u ? v : srgbToLinear(v);
+            }
+            free(png.pixels);
+            e.pixels = f;
+            e.format = VKRT_TEXTURE_FORMAT_RGBA32_SFLOAT;
+            e.colorSpace = VKRT_TEXTURE_COLOR_SPACE_LINEAR;
+        }
+    }
+    const char* name = jsonName(image) ? jsonName(image) : (jsonName(tex) ? jsonName(tex) : "Texture");
+    e.name = duplicateString(name);
+    if (out->textureCount == out->textureCapacity) {
+        uint32_t cap = out->textureCapacity ? out->textureCapacity * 2u : 8u;
+        TextureImportEntry* grown = (TextureImportEntry*)realloc(out->textures, cap * sizeof(TextureImportEntry));
+        if (!grown || !e.name) {
+            if (grown) out->textures = grown;
+            free(e.pixels);
+            free(e.name);
+            return VKRT_INVALID_INDEX;
+        }
+        out->textures = grown;
+        out->textureCapacity = cap;
+    }
+    keys[out->textureCount] = key;
+    out->textures[out->textureCount] = e;
+    return out->textureCount++;
+}
+
+/* A material texture view (loader.c:380-404, 907-1010): texture, sampler
+ * wrap, texture-coordinate set and KHR_texture_transform (its texCoord
+ * overrides the view's); sets above 1 are not supported and are skipped. */
+static void attachTexture(GltfDoc* doc, MeshImportData* out, uint32_t* keys, Material* m, const JsonValue* view,
+                          uint32_t slot, uint32_t colorSpace) {
+    uint32_t index = 0, set = 0, wrap = 0;
+    if (!view || !vkrtJsonToU32(vkrtJsonGet(view, "index"), &index)) return;
+    const JsonValue* tt = vkrtJsonGet(vkrtJsonGet(view, "extensions"), "KHR_texture_transform");
+    if (!(tt && vkrtJsonToU32(vkrtJsonGet(tt, "texCoord"), &set)) &&
+        !vkrtJsonToU32(vkrtJsonGet(view, "texCoord"), &set))
+        set = 0;
+    if (set > 1u) {
+        GLTF_LOG("texture coordinate set %u is not supported; the texture is skipped", set);
+        return;
+    }
+    const uint32_t t = importTexture(doc, out, keys, index, colorSpace, &wrap);
+    if (t == VKRT_INVALID_INDEX) return;
+    float xf[4] = {1.0f, 1.0f, 0.0f, 0.0f};
+    float rotation = 0.0f;
+    if (tt) {
+        static const float one2[2] = {1.0f, 1.0f}, zero2[2] = {0.0f, 0.0f};
+        readFloatsOr(tt, "scale", xf, 2, one2);
+        readFloatsOr(tt, "offset", xf + 2, 2, zero2);
+        rotation = readFloatOr(tt, "rotation", 0.0f);
+    }
+    uint32_t* idx[4] = {&m->baseColorTextureIndex, &m->metallicRoughnessTextureIndex, &m->normalTextureIndex,
+                        &m->emissiveTextureIndex};
+    uint32_t* wraps[4] = {&m->baseColorTextureWrap, &m->metallicRoughnessTextureWrap, &m->normalTextureWrap,
+                          &m->emissiveTextureWrap};
+    float* xfs[4] = {m->baseColorTextureTransform, m->metallicRoughnessTextureTransform, m->normalTextureTransform,
+                     m->emissiveTextureTransform};
+    *idx[slot] = t;
+    *wraps[slot] = wrap;
+    memcpy(xfs[slot], xf, sizeof(xf));
+    m->textureRotations[slot] = rotation;
+    m->textureTexcoordSets = (m->textureTexcoordSets & ~(0xffu << (8u * slot))) | (set << (8u * slot));
+}
+
+static void attachMaterialTextures(GltfDoc* doc, MeshImportData* out, uint32_t* keys, Material* m,
+                                   const JsonValue* src) {
+    const JsonValue* pbr = vkrtJsonGet(src, "pbrMetallicRoughness");
+    attachTexture(doc, out, keys, m, vkrtJsonGet(pbr, "baseColorTexture"), VKRT_MATERIAL_TEXTURE_SLOT_BASE_COLOR,
+                  VKRT_TEXTURE_COLOR_SPACE_SRGB);
+    attachTexture(doc, out, keys, m, vkrtJsonGet(pbr, "metallicRoughnessTexture"),
+                  VKRT_MATERIAL_TEXTURE_SLOT_METALLIC_ROUGHNESS, VKRT_TEXTURE_COLOR_SPACE_LINEAR);
+    attachTexture(doc, out, keys, m, vkrtJsonGet(src, "normalTexture"), VKRT_MATERIAL_TEXTURE_SLOT_NORMAL,
+                  VKRT_TEXTURE_COLOR_SPACE_LINEAR);
+    attachTexture(doc, out, keys, m, vkrtJsonGet(src, "emissiveTexture"), VKRT_MATERIAL_TEXTURE_SLOT_EMISSIVE,
+                  VKRT_TEXTURE_COLOR_SPACE_SRGB);
 }
 
 /* ------------------------------------------------------------- geometry */
@@ -1222,7 +1380,10 @@ int meshLoadFromFile(const char* filePath, MeshImportData* out) {
     directoryOf(filePath, doc->directory, sizeof(doc->directory));
     int result = parseContainer(doc);
     uint32_t textureRefs = 0;
+    /* (image, colour space) of each imported texture; at most four per material */
+    uint32_t* keys = NULL;
     if (result == 0 && doc->materials && doc->materials->type == JSON_ARRAY && doc->materials->count > 0) {
+        keys = (uint32_t*)calloc((size_t)doc->materials->count * 4u, sizeof(uint32_t));
         out->materials = (MaterialImportEntry*)calloc(doc->materials->count, sizeof(MaterialImportEntry));
         if (!out->materials) result = -1;
         for (uint32_t i = 0; result == 0 && i < doc->materials->count; i++) {
@@ -1233,6 +1394,7 @@ int meshLoadFromFile(const char* filePath, MeshImportData* out) {
             else
                 snprintf(name, sizeof(name), "Material %u", i);
             out->materials[i].material = buildMaterial(src, &textureRefs);
+            if (keys) attachMaterialTextures(doc, out, keys, &out->materials[i].material, src);
             out->materials[i].name = duplicateString(name);
             out->materialCount = i + 1u;
             if (!out->materials[i].name) result = -1;
@@ -1243,9 +1405,10 @@ int meshLoadFromFile(const char* filePath, MeshImportData* out) {
         GLTF_LOG("No triangle mesh primitives were found in '%s'", filePath);
         result = -1;
     }
-    if (result == 0 && textureRefs > 0)
-        GLTF_LOG("'%s': %u material texture reference(s) ignored (image textures are not imported)", filePath,
-                 textureRefs);
+    if (result == 0 && textureRefs > out->textureCount)
+        GLTF_LOG("'%s': %u material texture reference(s), %u texture(s) imported", filePath, textureRefs,
+                 out->textureCount);
+    free(keys);
     releaseDoc(doc);
     free(doc);
     if (result != 0) {

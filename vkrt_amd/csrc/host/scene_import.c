@@ -64,6 +64,28 @@ VKRT_Result VKRT_importMesh(VKRT* vkrt, const char* path, const char* importName
     if (result == VKRT_SUCCESS) result = VKRT_uploadMeshDataBatch(vkrt, uploads, data.count);
     free(uploads);
 
+    /* decoded textures first; the materials' slot indices are local to the file */
+    if (result == VKRT_SUCCESS && data.textureCount) {
+        VKRT_TextureUpload* tu = (VKRT_TextureUpload*)calloc(data.textureCount, sizeof(VKRT_TextureUpload));
+        if (!tu) result = VKRT_ERROR_OUT_OF_MEMORY;
+        for (uint32_t t = 0; tu && t < data.textureCount; t++) {
+            tu[t].name = data.textures[t].name;
+            tu[t].pixels = data.textures[t].pixels;
+            tu[t].width = data.textures[t].width;
+            tu[t].height = data.textures[t].height;
+            tu[t].format = data.textures[t].format;
+            tu[t].colorSpace = data.textures[t].colorSpace;
+        }
+        if (tu) result = VKRT_addTexturesBatch(vkrt, tu, data.textureCount, NULL);
+        free(tu);
+        for (uint32_t m = 0; result == VKRT_SUCCESS && m < data.materialCount; m++) {
+            Material* mat = &data.materials[m].material;
+            uint32_t* slots[4] = {&mat->baseColorTextureIndex, &mat->metallicRoughnessTextureIndex,
+                                  &mat->normalTextureIndex, &mat->emissiveTextureIndex};
+            for (int k = 0; k < 4; k++)
+                if (*slots[k] != VKRT_INVALID_INDEX) *slots[k] += textureBase;
+        }
+    }
     if (result == VKRT_SUCCESS && data.materialCount) {
         materialMap = (uint32_t*)malloc(sizeof(uint32_t) * data.materialCount);
         if (!materialMap) result = VKRT_ERROR_OUT_OF_MEMORY;
