@@ -132,7 +132,11 @@ def cpu_baseline(renderer, rows: int, spp: int, depth: int):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
-    threads = max(1, min(16, os.cpu_count() or 1))
+    # every core this process may run on, within the host share the job is given
+    # (OMP_NUM_THREADS, 16 per GPU on a one-GPU box)
+    visible = len(os.sched_getaffinity(0))
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    threads = max(1, min(visible, share) if share > 0 else visible)
     buffers = renderer.export()
     t0 = time.time()
     osc = O.OracleScene(buffers)
@@ -150,7 +154,8 @@ def cpu_baseline(renderer, rows: int, spp: int, depth: int):
         "cores": threads,
         "kind": "port",
         "sample": f"rows {r0}-{r0 + rows - 1} of the 1920x1080 frame, all {WIDTH} columns, {spp} spp, depth "
-                  f"{depth}, same scene, mode and seeds; FP32 C oracle with OpenMP; "
+                  f"{depth}, same scene, mode and seeds; FP32 C oracle with OpenMP on {threads} threads "
+                  f"({visible} cores in the affinity mask, host share {share or 'unset'}); "
                   f"{bounces} bounces in {dt:.2f} s; oracle BVH build ({build_s:.1f} s) excluded",
     }
 

@@ -45,6 +45,10 @@ VKRT_Result VKRT_uploadMeshData(VKRT* vkrt, const Vertex* vertices, size_t verte
 VKRT_Result VKRT_uploadMeshDataBatch(VKRT* vkrt, const VKRT_MeshUpload* uploads, size_t uploadCount); /* vkrt.h:34 */
 VKRT_Result VKRT_removeMesh(VKRT* vkrt, uint32_t meshIndex);                                          /* vkrt.h:35 */
 
+/* Orbit / pan / zoom of the viewport camera from one frame of mouse input;
+ * src/core/scene/camera.c:90-120. */
+VKRT_Result VKRT_applyCameraInput(VKRT* vkrt, const VKRT_CameraInput* input);          /* vkrt.h:36 */
+
 /* settings: vkrt.h:37-53 */
 VKRT_Result VKRT_invalidateAccumulation(VKRT* vkrt);                                  /* vkrt.h:37 */
 VKRT_Result VKRT_setSamplesPerPixel(VKRT* vkrt, uint32_t samplesPerPixel);            /* vkrt.h:38 */
@@ -58,6 +62,9 @@ VKRT_Result VKRT_setExposure(VKRT* vkrt, float exposure);                       
 VKRT_Result VKRT_setAutoExposureEnabled(VKRT* vkrt, uint8_t enabled);                 /* vkrt.h:46 */
 VKRT_Result VKRT_setEnvironmentLight(VKRT* vkrt, const float color[3], float strength); /* vkrt.h:47 */
 VKRT_Result VKRT_setEnvironmentRotation(VKRT* vkrt, float rotationDegrees);           /* vkrt.h:48 */
+/* Loads a PNG, JPEG or EXR file as a linear texture and makes it the
+ * environment map (src/core/scene/environment.c:42-60). */
+VKRT_Result VKRT_setEnvironmentTextureFromFile(VKRT* vkrt, const char* path);           /* vkrt.h:49 */
 VKRT_Result VKRT_clearEnvironmentTexture(VKRT* vkrt);                                 /* vkrt.h:50 */
 VKRT_Result VKRT_setDebugMode(VKRT* vkrt, VKRT_DebugMode mode);                       /* vkrt.h:51 */
 VKRT_Result VKRT_setMisNeeEnabled(VKRT* vkrt, uint8_t enabled);                       /* vkrt.h:52 */
@@ -67,6 +74,12 @@ VKRT_Result VKRT_setTimeRange(VKRT* vkrt, float timeBase, float timeStep);      
  * or EXR (linear RGBA32F); no extension means PNG. Denoising is not available
  * in this build: a request for it exports the raw render. */
 void VKRT_defaultRenderExportSettings(VKRT_RenderExportSettings* settings);                    /* vkrt.h:54 */
+/* The flag is kept and reported as in the reference (src/core/api/render.c:27-37).
+ * This build carries no denoiser, so a finished render always reaches
+ * VKRT_RENDER_PHASE_COMPLETE_RAW: the outcome the reference reports when its
+ * denoise step fails (src/core/utility/export/api.c:160-167). */
+VKRT_Result VKRT_setRenderDenoiseEnabled(VKRT* vkrt, uint8_t enabled);                         /* vkrt.h:55 */
+VKRT_Result VKRT_denoiseRenderToViewport(VKRT* vkrt);                                          /* vkrt.h:56 */
 VKRT_Result VKRT_saveRenderImageEx(VKRT* vkrt, const char* path, const VKRT_RenderExportSettings* settings); /* vkrt.h:57 */
 VKRT_Result VKRT_saveRenderImage(VKRT* vkrt, const char* path);                               /* vkrt.h:58 */
 
@@ -77,6 +90,7 @@ VKRT_Result VKRT_stopRenderSampling(VKRT* vkrt);                                
 VKRT_Result VKRT_stopRender(VKRT* vkrt);                                                        /* vkrt.h:62 */
 VKRT_Result VKRT_getSceneSettings(const VKRT* vkrt, VKRT_SceneSettingsSnapshot* outSettings);   /* vkrt.h:63 */
 VKRT_Result VKRT_getRenderStatus(const VKRT* vkrt, VKRT_RenderStatusSnapshot* outStatus);       /* vkrt.h:64 */
+VKRT_Result VKRT_getRuntimeSnapshot(const VKRT* vkrt, VKRT_RuntimeSnapshot* outRuntime);       /* vkrt.h:65 */
 VKRT_Result VKRT_getSystemInfo(const VKRT* vkrt, VKRT_SystemInfo* outSystemInfo);               /* vkrt.h:66 */
 
 /* scene objects: vkrt.h:73-112 */
@@ -86,11 +100,19 @@ VKRT_Result VKRT_getMaterialCount(const VKRT* vkrt, uint32_t* outMaterialCount);
 VKRT_Result VKRT_getMaterialSnapshot(const VKRT* vkrt, uint32_t materialIndex,
                                      VKRT_MaterialSnapshot* outMaterial);                             /* vkrt.h:76 */
 VKRT_Result VKRT_getTextureCount(const VKRT* vkrt, uint32_t* outTextureCount);                        /* vkrt.h:77 */
+VKRT_Result VKRT_getTextureSnapshot(const VKRT* vkrt, uint32_t textureIndex,
+                                    VKRT_TextureSnapshot* outTexture);                               /* vkrt.h:78 */
 VKRT_Result VKRT_addTextureFromPixels(VKRT* vkrt, const VKRT_TextureUpload* upload,
                                       uint32_t* outTextureIndex);                                     /* vkrt.h:79 */
+/* PNG (libspng's RGBA8 / RGBA16 output rules), baseline JPEG (RGBA8) or EXR
+ * (RGBA16F when every channel is half, else RGBA32F); src/core/utility/image.c:123-399. */
+VKRT_Result VKRT_addTextureFromFile(VKRT* vkrt, const char* path, const char* name, uint32_t colorSpace,
+                                    uint32_t* outTextureIndex);                                      /* vkrt.h:80 */
 VKRT_Result VKRT_removeTexture(VKRT* vkrt, uint32_t textureIndex);                                  /* vkrt.h:87 */
 VKRT_Result VKRT_addTexturesBatch(VKRT* vkrt, const VKRT_TextureUpload* uploads, size_t uploadCount,
                                   uint32_t* outTextureIndices);                                      /* vkrt.h:88 */
+VKRT_Result VKRT_setMaterialTexture(VKRT* vkrt, uint32_t materialIndex, uint32_t textureSlot,
+                                    uint32_t textureIndex);                                          /* vkrt.h:94 */
 VKRT_Result VKRT_addMaterial(VKRT* vkrt, const Material* material, const char* name,
                              uint32_t* outMaterialIndex);                                             /* vkrt.h:95 */
 VKRT_Result VKRT_removeMaterial(VKRT* vkrt, uint32_t materialIndex);                                 /* vkrt.h:96 */

@@ -49,6 +49,18 @@ typedef struct Camera {
     float nearZ, farZ, vfov;
 } Camera;
 
+/* src/core/api/vkrt_types.h:37-46: one frame of viewport mouse input */
+typedef struct VKRT_CameraInput {
+    float orbitDx;
+    float orbitDy;
+    float panDx;
+    float panDy;
+    float scroll;
+    uint8_t orbiting;
+    uint8_t panning;
+    uint8_t captureMouse;
+} VKRT_CameraInput;
+
 typedef struct VKRT_CreateInfo {
     uint32_t width;
     uint32_t height;
@@ -126,6 +138,57 @@ typedef struct VKRT_RenderStatusSnapshot {
     float displayFrameTimeMs;
 } VKRT_RenderStatusSnapshot;
 
+/* Phase and status predicates of src/core/api/vkrt_types.h:172-227 (the
+ * offline benchmark and the render controller poll these). */
+static inline uint8_t VKRT_renderPhaseIsActive(VKRT_RenderPhase phase) { return phase != VKRT_RENDER_PHASE_INACTIVE; }
+static inline uint8_t VKRT_renderPhaseIsSampling(VKRT_RenderPhase phase) { return phase == VKRT_RENDER_PHASE_SAMPLING; }
+static inline uint8_t VKRT_renderPhaseIsDenoising(VKRT_RenderPhase phase) { return phase == VKRT_RENDER_PHASE_DENOISING; }
+static inline uint8_t VKRT_renderPhaseIsBusy(VKRT_RenderPhase phase) { return VKRT_renderPhaseIsDenoising(phase); }
+static inline uint8_t VKRT_renderPhaseIsComplete(VKRT_RenderPhase phase) {
+    return phase == VKRT_RENDER_PHASE_COMPLETE_RAW || phase == VKRT_RENDER_PHASE_COMPLETE_DENOISED;
+}
+static inline uint8_t VKRT_renderPhaseSamplingFinished(VKRT_RenderPhase phase) {
+    return VKRT_renderPhaseIsBusy(phase) || VKRT_renderPhaseIsComplete(phase);
+}
+static inline uint8_t VKRT_renderPhaseIsDenoised(VKRT_RenderPhase phase) {
+    return phase == VKRT_RENDER_PHASE_COMPLETE_DENOISED;
+}
+static inline uint8_t VKRT_renderStatusIsActive(const VKRT_RenderStatusSnapshot* s) {
+    return s && VKRT_renderPhaseIsActive(s->renderPhase);
+}
+static inline uint8_t VKRT_renderStatusIsSampling(const VKRT_RenderStatusSnapshot* s) {
+    return s && VKRT_renderPhaseIsSampling(s->renderPhase);
+}
+static inline uint8_t VKRT_renderStatusIsDenoising(const VKRT_RenderStatusSnapshot* s) {
+    return s && VKRT_renderPhaseIsDenoising(s->renderPhase);
+}
+static inline uint8_t VKRT_renderStatusIsBusy(const VKRT_RenderStatusSnapshot* s) {
+    return s && VKRT_renderPhaseIsBusy(s->renderPhase);
+}
+static inline uint8_t VKRT_renderStatusIsComplete(const VKRT_RenderStatusSnapshot* s) {
+    return s && VKRT_renderPhaseIsComplete(s->renderPhase);
+}
+static inline uint8_t VKRT_renderStatusSamplingFinished(const VKRT_RenderStatusSnapshot* s) {
+    return s && VKRT_renderPhaseSamplingFinished(s->renderPhase);
+}
+static inline uint8_t VKRT_renderStatusIsDenoised(const VKRT_RenderStatusSnapshot* s) {
+    return s && VKRT_renderPhaseIsDenoised(s->renderPhase);
+}
+
+/* src/core/api/vkrt_types.h:229-238. Headless: the display and swapchain
+ * extents are the render extent and presentMode is 0. */
+typedef struct VKRT_RuntimeSnapshot {
+    uint32_t displayWidth;
+    uint32_t displayHeight;
+    uint32_t swapchainWidth;
+    uint32_t swapchainHeight;
+    uint32_t renderWidth;
+    uint32_t renderHeight;
+    uint32_t displayViewportRect[4];
+    uint32_t presentMode;
+    float displayRefreshHz;
+} VKRT_RuntimeSnapshot;
+
 typedef struct VKRT_RenderExportSettings {
     uint8_t denoiseEnabled;
 } VKRT_RenderExportSettings;
@@ -151,6 +214,16 @@ typedef struct VKRT_MaterialSnapshot {
     uint32_t useCount;
     char name[VKRT_NAME_LEN];
 } VKRT_MaterialSnapshot;
+
+/* src/core/api/vkrt_types.h:265-272 */
+typedef struct VKRT_TextureSnapshot {
+    uint32_t width;
+    uint32_t height;
+    uint32_t format;
+    uint32_t colorSpace;
+    uint32_t useCount;
+    char name[VKRT_NAME_LEN];
+} VKRT_TextureSnapshot;
 
 /* Default material of the reference (src/core/api/vkrt_types.h:200-242). */
 static inline Material VKRT_materialDefault(void) {

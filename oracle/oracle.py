@@ -55,6 +55,7 @@ def lib():
         g("oracle_pcg32_raw").argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, u32p]
         g("oracle_hash").restype = C.c_uint32
         g("oracle_hash").argtypes = [C.c_uint32]
+        g("oracle_hash_states").argtypes = [C.c_int, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, u32p]
         g("oracle_camera_ray").argtypes = [P, C.c_int, C.c_int, rt, rt, rp]
         g("oracle_intersect").restype = C.c_int
         g("oracle_intersect").argtypes = [f32p, rp, rp]
@@ -177,6 +178,13 @@ def pcg32_raw(init_state, init_seq, n):
 
 def oracle_hash(v):
     return lib().oracle_hash_f32(v)
+
+
+def hash_states(px, py, frame, sample, n):
+    """Hash-policy RNG states: the pixel seed, then the state after each draw."""
+    out = np.zeros(n, dtype=np.uint32)
+    lib().oracle_hash_states_f32(px, py, frame, sample, n, ptr(out))
+    return out
 
 
 def intersect(tri9, ray8, precision=32):

@@ -161,3 +161,61 @@ def test_sheen_is_rejected_to_zero(api):
     mats = np.ctypeslib.as_array(C.cast(b.materials, C.POINTER(C.c_float)), shape=(b.materialCount * 68,))
     w = V.Material.sheenTintWeight.offset // 4 + 3
     assert mats.reshape(-1, 68)[idx, w] == 0.0
+
+
+REFERENCE_APP = "/root/reference/src/app"
+# The editor's viewport overlay, mesh picking and zoom/pan view state: the
+# reference's interactive UI, which this headless trace step does not carry.
+EDITOR_ONLY = {"VKRT_consumeSelectedMesh", "VKRT_getOverlayInfo", "VKRT_getRenderViewCrop", "VKRT_getRenderViewState",
+               "VKRT_getSelectedMesh", "VKRT_registerAppHooks", "VKRT_requestSelectionAtPixel",
+               "VKRT_setRenderViewState", "VKRT_setSelectedMesh"}
+
+
+def _exported():
+    out = subprocess.run(["nm", "-D", "--defined-only", V.LIB_PATH], capture_output=True, text=True, check=True).stdout
+    return {line.split()[-1] for line in out.splitlines() if line.split()[-1].startswith("VKRT_")}
+
+
+def _inline_helpers():
+    names = set()
+    for fn in os.listdir(INCLUDE):
+        if fn.endswith(".h"):
+            names |= set(re.findall(r"static inline [\w ]+?\b(VKRT_\w+)\s*\(", open(os.path.join(INCLUDE, fn)).read()))
+    return names
+
+
+@pytest.mark.skipif(not os.path.isdir(REFERENCE_APP), reason="reference sources are not mounted")
+def test_reference_offline_benchmark_compiles_and_links_against_the_headers(tmp_path):
+    """src/app/render/benchmark.c (the --render-headless driver) compiles
+    unchanged with include/ in place of src/core/api, and every VKRT_ symbol
+    its object needs is exported by the library."""
+    obj = tmp_path / "benchmark.o"
+    dirs = [INCLUDE, REFERENCE_APP, os.path.join(REFERENCE_APP, "render"), "/root/reference/src/core/utility"]
+    subprocess.run(["gcc", "-std=gnu11", "-c", os.path.join(REFERENCE_APP, "render", "benchmark.c"), "-o", str(obj)] +
+                   [f"-I{d}" for d in dirs], check=True)
+    undefined = subprocess.run(["nm", "-u", str(obj)], capture_output=True, text=True, check=True).stdout
+    needed = {line.split()[-1] for line in undefined.splitlines() if line.split()[-1].startswith("VKRT_")}
+    assert needed and needed <= _exported(), sorted(needed - _exported())
+
+
+@pytest.mark.skipif(not os.path.isdir(REFERENCE_APP), reason="reference sources are not mounted")
+def test_every_api_call_of_the_reference_app_resolves():
+    """The render, scene, mesh, session and CLI controllers of the reference
+    application (which also include GLFW, Vulkan or imgui headers absent here,
+    so they are scanned rather than compiled) call only functions the library
+    exports or the headers define inline; the editor's calls outside that set
+    are exactly its overlay, picking and zoom/pan functions."""
+    def calls(paths):
+        found = set()
+        for p in paths:
+            found |= set(re.findall(r"\b(VKRT_[A-Za-z0-9]+)\s*\(", open(p).read()))
+        return found
+
+    available = _exported() | _inline_helpers()
+    app = [os.path.join(d, f) for d, _, fs in os.walk(REFERENCE_APP) for f in fs
+           if f.endswith(".c") and "/editor" not in d]
+    assert len(app) >= 8
+    assert calls(app) - available == set()
+    editor = [os.path.join(d, f) for d, _, fs in os.walk(os.path.join(REFERENCE_APP, "editor")) for f in fs
+              if f.endswith(".c")]
+    assert calls(editor) - available == EDITOR_ONLY

@@ -116,3 +116,56 @@ def test_gltf_png_textures_import(tmp_path):
     p[..., :3] = srgb(p[..., :3])
     np.testing.assert_allclose(texels(3), p, atol=2e-7)
     r.close()
+
+
+def _quad_with_texture(image, tex_coord=0):
+    b = Builder()
+    pos = np.array([[0, 0, 0], [1, 0, 0], [1, 0, 1], [0, 0, 1]], np.float32)
+    uv = np.array([[0, 0], [1, 0], [1, 1], [0, 1]], np.float32)
+    prim = {"attributes": {"POSITION": b.accessor(pos, "VEC3"), "TEXCOORD_0": b.accessor(uv, "VEC2")},
+            "indices": b.accessor(np.array([0, 1, 2, 0, 2, 3], np.uint16), "SCALAR"), "material": 0}
+    b.doc["images"] = [image(b)]
+    b.doc["textures"] = [{"source": 0}]
+    b.doc["materials"] = [{"pbrMetallicRoughness": {"baseColorTexture": {"index": 0, "texCoord": tex_coord}}}]
+    b.doc["meshes"] = [{"primitives": [prim]}]
+    b.doc["nodes"] = [{"mesh": 0}]
+    b.doc["scenes"] = [{"nodes": [0]}]
+    return b
+
+
+def test_gltf_jpeg_texture_and_high_texcoord_set(tmp_path):
+    """loader.c:432-499: a JPEG image in a buffer view decodes like the
+    reference's turbojpeg call (fixture pixels from libjpeg-turbo); a texture
+    coordinate set above 1 is stored in its 8 bits (loader.c:372-378)."""
+    import os
+    golden = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "images")
+    jpg = open(os.path.join(golden, "rgb_420_progressive.jpg"), "rb").read()
+    want = np.load(os.path.join(golden, "rgb_420_progressive_rgb.npy")) / 255.0
+    b = _quad_with_texture(lambda b: {"bufferView": b.view(jpg), "mimeType": "image/jpeg"}, tex_coord=2)
+    path = str(tmp_path / "jpeg.glb")
+    b.finish(path)
+    r = V.Renderer(init_device=False)
+    r.import_mesh(path)
+    buf = r.export()
+    m = buf.materials[buf.meshInfos[0].materialIndex]
+    assert m.baseColorTextureIndex == 0 and m.textureTexcoordSets == 2
+    first, w, h, _ = np.ctypeslib.as_array(buf.textureInfo, (4,))
+    tex = np.ctypeslib.as_array(buf.textureTexels, (buf.textureTexelCount * 4,)).reshape(-1, 4)
+    got = tex[first:first + w * h].reshape(h, w, 4)
+    np.testing.assert_allclose(got[..., :3], srgb(want), atol=2e-7)
+    r.close()
+
+
+def test_gltf_image_that_does_not_decode_fails_the_import(tmp_path):
+    """loader.c:508-520: a texture image that cannot be decoded fails the
+    import instead of leaving the material untextured."""
+    b = _quad_with_texture(lambda b: {"bufferView": b.view(b"\x89PNG\r\n\x1a\n broken"), "mimeType": "image/png"})
+    path = str(tmp_path / "broken.glb")
+    b.finish(path)
+    r = V.Renderer(init_device=False)
+    try:
+        V.load_gltf(path)
+        raise AssertionError("import should fail")
+    except V.VkrtError:
+        pass
+    r.close()

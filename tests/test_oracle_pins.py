@@ -72,6 +72,53 @@ def test_uniforms_are_24_bit_and_uniform(policy):
     assert abs(np.corrcoef(u[:-1], u[1:])[0, 1]) < 0.02
 
 
+M32 = 0xFFFFFFFF
+
+
+def lowbias32_inverse(x):
+    """Wellons' published inverse of lowbias32 (nullprogram.com/blog/2018/07/31,
+    'lowbias32_r'): multipliers 0x43021123 and 0x1d69e2a5 undo 0x846ca68b and
+    0x7feb352d, and x ^= x>>15 ^ x>>30 undoes x ^= x>>15."""
+    x ^= x >> 16
+    x = (x * 0x43021123) & M32
+    x ^= (x >> 15) ^ (x >> 30)
+    x = (x * 0x1D69E2A5) & M32
+    x ^= x >> 16
+    return x
+
+
+def test_lowbias32_inverse_constants_are_the_modular_inverses():
+    assert pow(0x7FEB352D, -1, 1 << 32) == 0x1D69E2A5
+    assert pow(0x846CA68B, -1, 1 << 32) == 0x43021123
+
+
+def test_hash_is_lowbias32():
+    """random.slang:7-14. Inverting the oracle's hash with the published inverse
+    must give back every input; a wrong multiplier, shift or order breaks it."""
+    rng = np.random.default_rng(11)
+    xs = [0, 1, 2, 0x9E3779B9, M32, 0x80000000] + [int(v) for v in rng.integers(0, 1 << 32, 2000, dtype=np.uint64)]
+    for x in xs:
+        assert lowbias32_inverse(int(O.oracle_hash(x))) == x
+    # the hash is not the identity or an xorshift alone
+    assert O.oracle_hash(1) != 1 and O.oracle_hash(0) == 0
+
+
+@pytest.mark.parametrize("px,py,frame,sample", [(0, 0, 0, 0), (3, 4, 0, 0), (1919, 1079, 7, 63),
+                                                (255, 0, 123456, 15), (17, 999, 0xFFFFFFFF, 0xFFFFFFFF)])
+def test_hash_pixel_seed_and_steps(px, py, frame, sample):
+    """initPixelSeed (random.slang:21-27): inverse(seed) is the xor of the four
+    products. rand (random.slang:16-19): inverse(next) = state + 0x9e3779b9 and
+    the uniform is the low 24 bits of the new state over 2^24."""
+    n = 9
+    states = [int(s) for s in O.hash_states(px, py, frame, sample, n)]
+    want = ((px * 73856093) ^ (py * 19349663) ^ (frame * 83492791) ^ (sample * 2654435761)) & M32
+    assert lowbias32_inverse(states[0]) == want
+    for k in range(1, n):
+        assert lowbias32_inverse(states[k]) == (states[k - 1] + 0x9E3779B9) & M32
+    u = O.rand_sequence(V.RNG_HASH, 1, px, py, frame, sample, n - 1, precision=64)
+    assert np.array_equal(u, np.array([(s & 0xFFFFFF) / 16777216.0 for s in states[1:]]))
+
+
 def test_streams_differ_by_pixel_frame_and_sample():
     base = O.rand_sequence(V.RNG_PCG32, 1, 3, 4, 0, 0, 8)
     for args in ((4, 4, 0, 0), (3, 5, 0, 0), (3, 4, 1, 0), (3, 4, 0, 1)):

@@ -235,3 +235,59 @@ def test_material_and_mesh_setters_follow_the_reference():
     r.clear_mesh_material(1)
     assert r.mesh(1).materialIndex == 0
     r.close()
+
+
+def test_scene_file_keeps_imported_textures_and_loads_texture_files(tmp_path):
+    """A scene file whose mesh import brings a textured GLB, plus a standalone
+    texture file and an environment map file (controller.c:1274-1312): slot
+    indices of the GLB's textures are kept, the standalone texture's saved
+    index is remapped to the new one (remapStandaloneTextureIndices,
+    controller.c:640-661), and the environment map is set from its file."""
+    from test_gltf_textures import png
+    from test_image_io import encode_exr
+
+    rng = np.random.default_rng(8)
+    b = G.Builder()
+    pos = np.array([[0, 0, 0], [1, 0, 0], [1, 0, 1], [0, 0, 1]], np.float32)
+    uv = np.array([[0, 0], [1, 0], [1, 1], [0, 1]], np.float32)
+    prim = {"attributes": {"POSITION": b.accessor(pos, "VEC3"), "TEXCOORD_0": b.accessor(uv, "VEC2")},
+            "indices": b.accessor(np.array([0, 1, 2, 0, 2, 3], np.uint16), "SCALAR"), "material": 0}
+    b.doc["images"] = [{"bufferView": b.view(png(rng.integers(0, 256, (4, 4, 4)), 6)), "mimeType": "image/png"}]
+    b.doc["textures"] = [{"source": 0}]
+    b.doc["materials"] = [{"pbrMetallicRoughness": {"baseColorTexture": {"index": 0}}}]
+    b.doc["meshes"] = [{"primitives": [prim]}]
+    b.doc["nodes"] = [{"mesh": 0}]
+    b.doc["scenes"] = [{"nodes": [0]}]
+    b.finish(str(tmp_path / "textured.glb"))
+    (tmp_path / "glow.png").write_bytes(png(rng.integers(0, 256, (3, 5, 3)), 2))
+    (tmp_path / "sky.exr").write_bytes(encode_exr([(c, 2, np.full((4, 8), 0.5, np.float32)) for c in "RGB"], 3, 8, 4))
+    doc = {
+        "format": "vkrt.scene", "version": 1,
+        "meshImports": ["textured.glb"],
+        "textureImports": [{"path": "glow.png", "index": 7, "colorSpace": 0}],
+        "environmentTexturePath": "sky.exr",
+        "materials": [{"index": 1, "name": "textured",
+                       "material": {"baseColor": [1, 1, 1], "baseColorTextureIndex": 0,
+                                    "emissiveTextureIndex": 7}}],
+        "meshes": [{"name": "quad", "materialIndex": 1, "hasMaterialAssignment": True, "renderBackfaces": False,
+                    "opacity": 1, "importIndex": 0, "importLocalIndex": 0}],
+        "sceneObjects": [{"name": "quad", "parentIndex": None, "meshIndex": 0, "localPosition": [0, 0, 0],
+                          "localRotation": [0, 0, 0], "localScale": [1, 1, 1]}],
+        "sceneSettings": {"camera": {"position": [0, -3, 0.5], "target": [0, 0, 0.5], "up": [0, 0, 1], "vfov": 40}},
+    }
+    path = tmp_path / "scene.json"
+    path.write_text(json.dumps(doc))
+    r = V.Renderer(init_device=False)
+    r.load_scene_file(str(path))
+    assert r.texture_count() == 3  # GLB texture 0, glow.png 1, sky.exr 2
+    m = r.material(r.mesh(0).materialIndex)
+    assert m.baseColorTextureIndex == 0  # the GLB's texture, not cleared
+    assert m.emissiveTextureIndex == 1  # saved index 7 -> the new index of glow.png
+    assert r.scene_settings().environmentTextureIndex == 2
+    # a texture file that does not load fails the whole document and restores the scene
+    doc["textureImports"][0]["path"] = "missing.png"
+    path.write_text(json.dumps(doc))
+    with pytest.raises(V.VkrtError):
+        r.load_scene_file(str(path))
+    assert r.texture_count() == 3 and r.scene_settings().environmentTextureIndex == 2
+    r.close()

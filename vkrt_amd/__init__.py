@@ -18,6 +18,9 @@ LIB_PATH = os.path.join(_HERE, "libvkrt_amd.so")
 
 VKRT_SUCCESS = 0
 VKRT_INVALID_INDEX = 0xFFFFFFFF
+VKRT_ERROR_INVALID_ARGUMENT = -1
+VKRT_ERROR_OPERATION_FAILED = -2
+VKRT_ERROR_OUT_OF_MEMORY = -3
 RENDER_MODE_RGB = 0
 RENDER_MODE_SPECTRAL = 1
 SPECTRAL_SINGLE = 0
@@ -35,8 +38,16 @@ TEXTURE_WRAP_REPEAT = 10497
 TEXTURE_WRAP_CLAMP_TO_EDGE = 33071
 TEXTURE_WRAP_MIRRORED_REPEAT = 33648
 TEXTURE_COLOR_SPACE_LINEAR = 1
+TEXTURE_WRAP_DEFAULT = TEXTURE_WRAP_REPEAT | (TEXTURE_WRAP_REPEAT << 16)
+MATERIAL_TEXTURE_SLOT_BASE_COLOR = 0
+MATERIAL_TEXTURE_SLOT_METALLIC_ROUGHNESS = 1
+MATERIAL_TEXTURE_SLOT_NORMAL = 2
+MATERIAL_TEXTURE_SLOT_EMISSIVE = 3
+RENDER_PHASE_INACTIVE = 0
 RENDER_PHASE_SAMPLING = 1
+RENDER_PHASE_DENOISING = 2
 RENDER_PHASE_COMPLETE_RAW = 3
+RENDER_PHASE_COMPLETE_DENOISED = 4
 
 
 class Vertex(C.Structure):
@@ -264,6 +275,22 @@ class SystemInfo(C.Structure):
     _fields_ = [("deviceName", C.c_char * 256), ("vendorID", C.c_uint32), ("driverVersion", C.c_uint32)]
 
 
+class CameraInput(C.Structure):
+    _fields_ = [("orbitDx", C.c_float), ("orbitDy", C.c_float), ("panDx", C.c_float), ("panDy", C.c_float),
+                ("scroll", C.c_float), ("orbiting", C.c_uint8), ("panning", C.c_uint8), ("captureMouse", C.c_uint8)]
+
+
+class RuntimeSnapshot(C.Structure):
+    _fields_ = [("displayWidth", C.c_uint32), ("displayHeight", C.c_uint32), ("swapchainWidth", C.c_uint32),
+                ("swapchainHeight", C.c_uint32), ("renderWidth", C.c_uint32), ("renderHeight", C.c_uint32),
+                ("displayViewportRect", C.c_uint32 * 4), ("presentMode", C.c_uint32), ("displayRefreshHz", C.c_float)]
+
+
+class TextureSnapshot(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("format", C.c_uint32), ("colorSpace", C.c_uint32),
+                ("useCount", C.c_uint32), ("name", C.c_char * 256)]
+
+
 class TraceStats(C.Structure):
     _fields_ = [
         ("extensionRays", C.c_uint64),
@@ -460,6 +487,9 @@ _EXPECTED_SIZES = {
     MeshImportEntry: 88,
     NodeImportEntry: 120,
     MeshImportData: 64,
+    CameraInput: 24,
+    RuntimeSnapshot: 48,
+    TextureSnapshot: 276,
 }
 for _cls, _size in _EXPECTED_SIZES.items():
     assert C.sizeof(_cls) == _size, f"{_cls.__name__} is {C.sizeof(_cls)} bytes, expected {_size}"
@@ -572,6 +602,14 @@ def library() -> C.CDLL:
         "VKRT_defaultRenderExportSettings": (None, [C.POINTER(RenderExportSettings)]),
         "VKRT_saveRenderImageEx": (C.c_int, [P, C.c_char_p, C.POINTER(RenderExportSettings)]),
         "VKRT_saveRenderImage": (C.c_int, [P, C.c_char_p]),
+        "VKRT_applyCameraInput": (C.c_int, [P, C.POINTER(CameraInput)]),
+        "VKRT_setEnvironmentTextureFromFile": (C.c_int, [P, C.c_char_p]),
+        "VKRT_setRenderDenoiseEnabled": (C.c_int, [P, C.c_uint8]),
+        "VKRT_denoiseRenderToViewport": (C.c_int, [P]),
+        "VKRT_getRuntimeSnapshot": (C.c_int, [P, C.POINTER(RuntimeSnapshot)]),
+        "VKRT_getTextureSnapshot": (C.c_int, [P, u32, C.POINTER(TextureSnapshot)]),
+        "VKRT_addTextureFromFile": (C.c_int, [P, C.c_char_p, C.c_char_p, u32, up]),
+        "VKRT_setMaterialTexture": (C.c_int, [P, u32, u32, u32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -42,6 +42,7 @@ VKRT_Result VKRT_create(VKRT** outVkrt) {
     VKRT_defaultNeuralCacheSettings(&vkrt->nrc);
     vkrt->materialSort = 0;
     vkrt->renderStatus.renderPhase = VKRT_RENDER_PHASE_INACTIVE;
+    vkrt->finalImageDenoiseEnabled = 1u;
     *outVkrt = vkrt;
     return VKRT_SUCCESS;
 }
@@ -309,6 +310,40 @@ VKRT_Result VKRT_draw(VKRT* vkrt) {
     return r;
 }
 
+/* src/core/api/render.c:27-37 */
+VKRT_Result VKRT_setRenderDenoiseEnabled(VKRT* vkrt, uint8_t enabled) {
+    if (!vkrt) return VKRT_ERROR_INVALID_ARGUMENT;
+    vkrt->finalImageDenoiseEnabled = enabled ? 1u : 0u;
+    if (vkrt->renderStatus.renderPhase == VKRT_RENDER_PHASE_SAMPLING)
+        vkrt->renderStatus.renderDenoiseEnabled = vkrt->finalImageDenoiseEnabled;
+    return VKRT_SUCCESS;
+}
+
+/* src/core/api/render.c:46-81. Valid once sampling has finished and with no
+ * debug view. The reference then denoises on a worker thread and reports
+ * COMPLETE_RAW when that fails (src/core/utility/export/api.c:160-167); this
+ * build has no denoiser, so the request completes at once as COMPLETE_RAW. */
+VKRT_Result VKRT_denoiseRenderToViewport(VKRT* vkrt) {
+    if (!vkrt) return VKRT_ERROR_INVALID_ARGUMENT;
+    const VKRT_RenderPhase phase = vkrt->renderStatus.renderPhase;
+    if (!VKRT_renderPhaseIsActive(phase) || !VKRT_renderPhaseSamplingFinished(phase)) return VKRT_ERROR_OPERATION_FAILED;
+    if (VKRT_renderPhaseIsBusy(phase)) return VKRT_SUCCESS;
+    if (vkrt->settings.debugMode != VKRT_DEBUG_MODE_NONE) return VKRT_ERROR_OPERATION_FAILED;
+    vkrt->renderStatus.renderPhase = VKRT_RENDER_PHASE_COMPLETE_RAW;
+    return VKRT_SUCCESS;
+}
+
+/* src/core/api/query.c:25-38. Headless: the display and swapchain are the
+ * render target, with no present mode or refresh rate. */
+VKRT_Result VKRT_getRuntimeSnapshot(const VKRT* vkrt, VKRT_RuntimeSnapshot* out) {
+    if (!vkrt || !out) return VKRT_ERROR_INVALID_ARGUMENT;
+    memset(out, 0, sizeof(*out));
+    out->displayWidth = out->swapchainWidth = out->renderWidth = vkrt->renderWidth;
+    out->displayHeight = out->swapchainHeight = out->renderHeight = vkrt->renderHeight;
+    memcpy(out->displayViewportRect, vkrt->displayViewport, sizeof(out->displayViewportRect));
+    return VKRT_SUCCESS;
+}
+
 /* src/core/api/render.c:248-265 */
 VKRT_Result VKRT_startRender(VKRT* vkrt, uint32_t width, uint32_t height, uint32_t targetSamples) {
     if (!vkrt || width == 0 || height == 0) return VKRT_ERROR_INVALID_ARGUMENT;
@@ -317,6 +352,7 @@ VKRT_Result VKRT_startRender(VKRT* vkrt, uint32_t width, uint32_t height, uint32
     vkrt->renderWidth = width;
     vkrt->renderHeight = height;
     vkrt->renderStatus.renderPhase = VKRT_RENDER_PHASE_SAMPLING;
+    vkrt->renderStatus.renderDenoiseEnabled = vkrt->finalImageDenoiseEnabled; /* render.c:214 */
     vkrt->renderStatus.renderTargetSamples = targetSamples;
     vkrtResetAutoSPPState(vkrt, 1); /* src/core/api/render.c:275 */
     vkrtWriteSceneStateUniform(vkrt);
@@ -329,6 +365,7 @@ VKRT_Result VKRT_continueRender(VKRT* vkrt, uint32_t targetSamples) {
         vkrt->renderStatus.renderPhase != VKRT_RENDER_PHASE_COMPLETE_DENOISED)
         return VKRT_ERROR_OPERATION_FAILED;
     vkrt->renderStatus.renderPhase = VKRT_RENDER_PHASE_SAMPLING;
+    vkrt->renderStatus.renderDenoiseEnabled = vkrt->finalImageDenoiseEnabled;
     vkrt->renderStatus.renderTargetSamples = targetSamples;
     vkrtWriteSceneStateUniform(vkrt);
     return VKRT_SUCCESS;

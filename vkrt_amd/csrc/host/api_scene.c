@@ -8,6 +8,7 @@
  *  - every change restarts accumulation (resetSceneData).
  */
 #include "internal.h"
+#include "image_io.h"
 
 #include <math.h>
 #include <stdlib.h>
@@ -337,6 +338,92 @@ VKRT_Result VKRT_addTextureFromPixels(VKRT* vkrt, const VKRT_TextureUpload* uplo
     return VKRT_SUCCESS;
 }
 
+/* src/core/scene/textures.c:374-381: material slots that name the texture,
+ * plus one when it is the environment map. */
+static uint32_t textureUsers(const VKRT* vkrt, uint32_t textureIndex) {
+    uint32_t users = vkrt->settings.environmentTextureIndex == textureIndex ? 1u : 0u;
+    for (uint32_t m = 0; m < vkrt->materialCount; m++) {
+        const Material* mat = &vkrt->materials[m].material;
+        users += (mat->baseColorTextureIndex == textureIndex) + (mat->metallicRoughnessTextureIndex == textureIndex) +
+                 (mat->normalTextureIndex == textureIndex) + (mat->emissiveTextureIndex == textureIndex);
+    }
+    return users;
+}
+
+/* src/core/api/query.c:110-124 */
+VKRT_Result VKRT_getTextureSnapshot(const VKRT* vkrt, uint32_t textureIndex, VKRT_TextureSnapshot* out) {
+    if (!vkrt || !out || textureIndex >= vkrt->textureCount) return VKRT_ERROR_INVALID_ARGUMENT;
+    const HostTexture* t = &vkrt->textures[textureIndex];
+    out->width = t->width;
+    out->height = t->height;
+    out->format = t->format;
+    out->colorSpace = t->colorSpace;
+    out->useCount = textureUsers(vkrt, textureIndex);
+    snprintf(out->name, sizeof(out->name), "%s", t->name);
+    return VKRT_SUCCESS;
+}
+
+/* src/core/api/texture.c:16-28 and src/core/scene/textures.c:484-515: the file
+ * must exist; the texture is named after the file unless a name is given. */
+VKRT_Result VKRT_addTextureFromFile(VKRT* vkrt, const char* path, const char* name, uint32_t colorSpace,
+                                    uint32_t* outTextureIndex) {
+    if (outTextureIndex) *outTextureIndex = VKRT_INVALID_INDEX;
+    if (!vkrt || !path || !path[0] || colorSpace > VKRT_TEXTURE_COLOR_SPACE_LINEAR) return VKRT_ERROR_INVALID_ARGUMENT;
+    FILE* probe = fopen(path, "rb");
+    if (!probe) return VKRT_ERROR_INVALID_ARGUMENT;
+    fclose(probe);
+    VkrtLoadedImage image;
+    if (vkrtLoadImageFromFile(path, colorSpace, &image) != 0) return VKRT_ERROR_OPERATION_FAILED;
+    const char* base = strrchr(path, '/');
+    VKRT_TextureUpload upload;
+    memset(&upload, 0, sizeof(upload));
+    upload.name = name && name[0] ? name : (base ? base + 1 : path);
+    upload.pixels = image.pixels;
+    upload.width = image.width;
+    upload.height = image.height;
+    upload.format = image.format;
+    upload.colorSpace = image.colorSpace;
+    VKRT_Result r = VKRT_addTextureFromPixels(vkrt, &upload, outTextureIndex);
+    free(image.pixels);
+    return r;
+}
+
+/* src/core/api/texture.c:48-53 and src/core/scene/textures.c:580-622: colour
+ * slots (base colour, emissive) take sRGB or linear textures, the others
+ * linear only; binding resets the slot's wrap, coordinate set, transform and
+ * rotation, and a texture left without users is released. */
+VKRT_Result VKRT_setMaterialTexture(VKRT* vkrt, uint32_t materialIndex, uint32_t textureSlot, uint32_t textureIndex) {
+    if (!vkrt || textureSlot >= VKRT_MATERIAL_TEXTURE_SLOT_COUNT || materialIndex >= vkrt->materialCount)
+        return VKRT_ERROR_INVALID_ARGUMENT;
+    Material* m = &vkrt->materials[materialIndex].material;
+    uint32_t* index[4] = {&m->baseColorTextureIndex, &m->metallicRoughnessTextureIndex, &m->normalTextureIndex,
+                          &m->emissiveTextureIndex};
+    uint32_t* wrap[4] = {&m->baseColorTextureWrap, &m->metallicRoughnessTextureWrap, &m->normalTextureWrap,
+                         &m->emissiveTextureWrap};
+    float* xf[4] = {m->baseColorTextureTransform, m->metallicRoughnessTextureTransform, m->normalTextureTransform,
+                    m->emissiveTextureTransform};
+    if (textureIndex != VKRT_INVALID_INDEX) {
+        if (textureIndex >= vkrt->textureCount) return VKRT_ERROR_INVALID_ARGUMENT;
+        const int colourSlot =
+            textureSlot == VKRT_MATERIAL_TEXTURE_SLOT_BASE_COLOR || textureSlot == VKRT_MATERIAL_TEXTURE_SLOT_EMISSIVE;
+        if (!colourSlot && vkrt->textures[textureIndex].colorSpace != VKRT_TEXTURE_COLOR_SPACE_LINEAR)
+            return VKRT_ERROR_INVALID_ARGUMENT;
+    }
+    if (*index[textureSlot] == textureIndex) return VKRT_SUCCESS;
+    const uint32_t previous = *index[textureSlot];
+    *index[textureSlot] = textureIndex;
+    *wrap[textureSlot] = VKRT_TEXTURE_WRAP_DEFAULT;
+    m->textureTexcoordSets &= ~(0xffu << (textureSlot * 8u));
+    xf[textureSlot][0] = xf[textureSlot][1] = 1.0f;
+    xf[textureSlot][2] = xf[textureSlot][3] = 0.0f;
+    m->textureRotations[textureSlot] = 0.0f;
+    vkrt->materialRevision++;
+    vkrtResetSceneData(vkrt);
+    if (previous != VKRT_INVALID_INDEX && previous < vkrt->textureCount && textureUsers(vkrt, previous) == 0u)
+        return VKRT_removeTexture(vkrt, previous);
+    return VKRT_SUCCESS;
+}
+
 VKRT_Result VKRT_getTextureCount(const VKRT* vkrt, uint32_t* outTextureCount) {
     if (!vkrt || !outTextureCount) return VKRT_ERROR_INVALID_ARGUMENT;
     *outTextureCount = vkrt->textureCount;
@@ -400,7 +487,40 @@ VKRT_Result VKRT_setEnvironmentTexture(VKRT* vkrt, uint32_t textureIndex) {
     return VKRT_SUCCESS;
 }
 
-VKRT_Result VKRT_clearEnvironmentTexture(VKRT* vkrt) { return VKRT_setEnvironmentTexture(vkrt, VKRT_INVALID_INDEX); }
+/* src/core/scene/environment.c:11-38,63-65: the cleared map is released when
+ * no material uses it. */
+VKRT_Result VKRT_clearEnvironmentTexture(VKRT* vkrt) {
+    if (!vkrt) return VKRT_ERROR_INVALID_ARGUMENT;
+    const uint32_t previous = vkrt->settings.environmentTextureIndex;
+    VKRT_Result r = VKRT_setEnvironmentTexture(vkrt, VKRT_INVALID_INDEX);
+    if (r == VKRT_SUCCESS && previous != VKRT_INVALID_INDEX && previous < vkrt->textureCount &&
+        textureUsers(vkrt, previous) == 0u)
+        r = VKRT_removeTexture(vkrt, previous);
+    return r;
+}
+
+/* src/core/scene/environment.c:11-60: the file loads as a linear texture and
+ * replaces the environment map; the previous map is released when nothing
+ * else uses it, and the new texture is released again if it cannot be used. */
+VKRT_Result VKRT_setEnvironmentTextureFromFile(VKRT* vkrt, const char* path) {
+    if (!vkrt || !path || !path[0]) return VKRT_ERROR_INVALID_ARGUMENT;
+    uint32_t index = VKRT_INVALID_INDEX;
+    VKRT_Result r = VKRT_addTextureFromFile(vkrt, path, NULL, VKRT_TEXTURE_COLOR_SPACE_LINEAR, &index);
+    if (r != VKRT_SUCCESS) return r;
+    if (vkrt->textures[index].colorSpace != VKRT_TEXTURE_COLOR_SPACE_LINEAR) {
+        VKRT_removeTexture(vkrt, index);
+        return VKRT_ERROR_INVALID_ARGUMENT;
+    }
+    const uint32_t previous = vkrt->settings.environmentTextureIndex;
+    r = VKRT_setEnvironmentTexture(vkrt, index);
+    if (r != VKRT_SUCCESS) {
+        VKRT_removeTexture(vkrt, index);
+        return r;
+    }
+    if (previous != VKRT_INVALID_INDEX && previous < vkrt->textureCount && textureUsers(vkrt, previous) == 0u)
+        return VKRT_removeTexture(vkrt, previous);
+    return VKRT_SUCCESS;
+}
 
 /* src/core/api/texture.c:35-46: an empty batch is invalid; a batch is added
  * whole or not at all. */

@@ -319,6 +319,66 @@ VKRT_Result VKRT_cameraGetPose(const VKRT* vkrt, float position[3], float target
     return VKRT_SUCCESS;
 }
 
+/* src/core/api/settings.c:15-21 and src/core/scene/camera.c:12-120: pan moves
+ * position and target along the view's right/up axes by 0.001 x distance per
+ * unit; orbit turns the position about the target by 0.004 rad per unit with
+ * the pitch held 0.001 rad short of the poles; scroll moves the position along
+ * the (unnormalised) view vector by 0.075 per unit. Each step recomputes the
+ * camera and restarts accumulation; nothing moves while the mouse is captured. */
+VKRT_Result VKRT_applyCameraInput(VKRT* vkrt, const VKRT_CameraInput* input) {
+    if (!vkrt || !input) return VKRT_ERROR_INVALID_ARGUMENT;
+    if (input->captureMouse) return VKRT_SUCCESS;
+    float* pos = vkrt->settings.camera.pos;
+    float* tgt = vkrt->settings.camera.target;
+    const float view[3] = {tgt[0] - pos[0], tgt[1] - pos[1], tgt[2] - pos[2]};
+    /* right = normalize(view x +z), or +y when looking straight up or down; up = normalize(right x view) */
+    float right[3] = {view[1], -view[0], 0.0f};
+    float rl = sqrtf(right[0] * right[0] + right[1] * right[1]);
+    if (rl < 1e-6f) {
+        right[0] = 0.0f;
+        right[1] = 1.0f;
+    } else {
+        right[0] /= rl;
+        right[1] /= rl;
+    }
+    float up[3] = {right[1] * view[2] - right[2] * view[1], right[2] * view[0] - right[0] * view[2],
+                   right[0] * view[1] - right[1] * view[0]};
+    float ul = sqrtf(up[0] * up[0] + up[1] * up[1] + up[2] * up[2]);
+    if (ul > 0.0f)
+        for (int a = 0; a < 3; a++) up[a] /= ul;
+    const float dist = sqrtf(view[0] * view[0] + view[1] * view[1] + view[2] * view[2]);
+    int moved = 0;
+    if (input->panning && (input->panDx != 0.0f || input->panDy != 0.0f)) {
+        const float scale = 0.001f * dist, mx = -input->panDx * scale, my = input->panDy * scale;
+        for (int a = 0; a < 3; a++) {
+            const float delta = right[a] * mx + up[a] * my;
+            pos[a] += delta;
+            tgt[a] += delta;
+        }
+        moved = 1;
+    }
+    if (input->orbiting && (input->orbitDx != 0.0f || input->orbitDy != 0.0f)) {
+        const float halfPi = 1.5707963267948966f, eps = 0.001f;
+        float yaw = atan2f(view[1], view[0]) - input->orbitDx * 0.004f;
+        float pitch = atan2f(view[2], sqrtf(view[0] * view[0] + view[1] * view[1])) - input->orbitDy * 0.004f;
+        if (pitch < -halfPi + eps) pitch = -halfPi + eps;
+        if (pitch > halfPi - eps) pitch = halfPi - eps;
+        const float forward[3] = {cosf(pitch) * cosf(yaw), cosf(pitch) * sinf(yaw), sinf(pitch)};
+        for (int a = 0; a < 3; a++) pos[a] = tgt[a] - forward[a] * dist;
+        moved = 1;
+    }
+    if (input->scroll != 0.0f) {
+        const float step = input->scroll * -0.075f;
+        for (int a = 0; a < 3; a++) pos[a] -= view[a] * step;
+        moved = 1;
+    }
+    if (moved) {
+        vkrtSyncCameraMatrices(vkrt);
+        vkrtResetSceneData(vkrt);
+    }
+    return VKRT_SUCCESS;
+}
+
 VKRT_Result VKRT_setRenderViewport(VKRT* vkrt, uint32_t x, uint32_t y, uint32_t width, uint32_t height) {
     if (!vkrt) return VKRT_ERROR_INVALID_ARGUMENT;
     if (x >= vkrt->renderWidth) x = vkrt->renderWidth ? vkrt->renderWidth - 1 : 0;

@@ -24,7 +24,7 @@
  */
 #include "internal.h"
 #include "json.h"
-#include "png_decode.h"
+#include "image_io.h"
 #include "vkrt_import.h"
 
 #include <float.h>
@@ -64,6 +64,7 @@ typedef struct GltfDoc {
     const JsonValue* materials;
     const JsonValue* meshes;
     const JsonValue* nodes;
+    int textureFailed; /* an image that does not decode fails the import, as in the reference */
 } GltfDoc;
 
 /* ---------------------------------------------------------------- files */
@@ -542,18 +543,19 @@ static const char* jsonName(const JsonValue* object);
 
 /* ------------------------------------------------------------- textures */
 
-static float srgbToLinear(float c) { return c <= 0.04045f ? c / 12.92f : powf((c + 0.055f) / 1.055f, 2.4f); }
-
 /* One image of the file as a texture entry (src/app/mesh/loader.c:432-499),
- * shared by every view of the same image in the same colour space. PNG
- * images are decoded; 16-bit sRGB images are stored as linear floats, since
- * a 16-bit texture upload is linear only. */
-static uint32_t importTexture(GltfDoc* doc, MeshImportData* out, uint32_t* keys, uint32_t textureIndex,
+ * shared by every view of the same image in the same colour space. The image
+ * is decoded from its file, data URI or buffer view by the library's image
+ * reader (PNG, JPEG, EXR: the reference's formats and output rules); an image
+ * that does not decode fails the whole import, as the reference's does. */
+static uint32_t importTexture(GltfDoc* doc, MeshImportData* out, uint64_t* keys, uint32_t textureIndex,
                               uint32_t colorSpace, uint32_t* outWrap) {
     const JsonValue* tex = vkrtJsonAt(vkrtJsonGet(&doc->root, "textures"), textureIndex);
     uint32_t source = 0, samplerIndex = 0;
     *outWrap = VKRT_TEXTURE_WRAP_DEFAULT;
     if (!tex || !vkrtJsonToU32(vkrtJsonGet(tex, "source"), &source)) return VKRT_INVALID_INDEX;
+    const JsonValue* image = vkrtJsonAt(vkrtJsonGet(&doc->root, "images"), source);
+    if (!image) return VKRT_INVALID_INDEX;
     if (vkrtJsonToU32(vkrtJsonGet(tex, "sampler"), &samplerIndex)) {
         const JsonValue* smp = vkrtJsonAt(vkrtJsonGet(&doc->root, "samplers"), samplerIndex);
         uint32_t ws = VKRT_TEXTURE_WRAP_REPEAT, wt = VKRT_TEXTURE_WRAP_REPEAT; /* loader.c:362-366 */
@@ -562,18 +564,22 @@ static uint32_t importTexture(GltfDoc* doc, MeshImportData* out, uint32_t* keys,
         if (smp && vkrtJsonToU32(vkrtJsonGet(smp, "wrapT"), &v)) wt = v;
         *outWrap = ws | (wt << 16);
     }
-    const uint32_t key = (source << 1) | (colorSpace == VKRT_TEXTURE_COLOR_SPACE_SRGB);
+    const uint64_t key = ((uint64_t)source << 1) | (colorSpace == VKRT_TEXTURE_COLOR_SPACE_SRGB);
     for (uint32_t i = 0; i < out->textureCount; i++)
         if (keys[i] == key) return i;
-    const JsonValue* image = vkrtJsonAt(vkrtJsonGet(&doc->root, "images"), source);
-    if (!image) return VKRT_INVALID_INDEX;
     const JsonValue* uri = vkrtJsonGet(image, "uri");
+    const JsonValue* mime = vkrtJsonGet(image, "mimeType");
+    char mimeType[64] = "";
+    if (mime && mime->type == JSON_STRING) snprintf(mimeType, sizeof(mimeType), "%s", mime->string);
     uint8_t* owned = NULL;
     const uint8_t* bytes = NULL;
     size_t size = 0;
     uint32_t view = 0;
     if (uri && uri->type == JSON_STRING && strncmp(uri->string, "data:", 5) == 0) {
         const char* comma = strchr(uri->string, ',');
+        const char* semi = strchr(uri->string, ';');
+        if (semi && comma && semi < comma && (size_t)(semi - uri->string - 5) < sizeof(mimeType))
+            snprintf(mimeType, sizeof(mimeType), "%.*s", (int)(semi - uri->string - 5), uri->string + 5);
         if (comma) owned = decodeBase64(comma + 1, &size);
         bytes = owned;
     } else if (uri && uri->type == JSON_STRING) {
@@ -583,6 +589,7 @@ static uint32_t importTexture(GltfDoc* doc, MeshImportData* out, uint32_t* keys,
             owned = readWholeFile(full, &size);
         }
         bytes = owned;
+        mimeType[0] = 0; /* loader.c:443-449: files are decoded by their contents */
     } else if (vkrtJsonToU32(vkrtJsonGet(image, "bufferView"), &view)) {
         uint32_t length = 0;
         size_t stride = 0;
@@ -591,39 +598,23 @@ static uint32_t importTexture(GltfDoc* doc, MeshImportData* out, uint32_t* keys,
             resolveView(doc, view, 0, 1, length, 0, &bytes, &stride) == 0)
             size = length;
     }
-    PngImage png;
-    int decoded = bytes && size && vkrtDecodePng(bytes, size, &png) == 0;
+    VkrtLoadedImage decoded;
+    char label[64];
+    snprintf(label, sizeof(label), "glTF image %u", source);
+    int ok = bytes && size && vkrtLoadImageFromMemory(bytes, size, mimeType, colorSpace, &decoded, label) == 0;
     free(owned);
-    if (!decoded) {
-        GLTF_LOG("glTF image %u is not a PNG image this importer decodes; its texture is skipped", source);
+    if (!ok) {
+        GLTF_LOG("glTF image %u could not be decoded", source);
+        doc->textureFailed = 1;
         return VKRT_INVALID_INDEX;
     }
     TextureImportEntry e;
     memset(&e, 0, sizeof(e));
-    e.width = png.width;
-    e.height = png.height;
-    e.pixels = png.pixels;
-    e.format = VKRT_TEXTURE_FORMAT_RGBA8_UNORM;
-    e.colorSpace = colorSpace;
-    if (png.bits == 16) {
-        e.format = VKRT_TEXTURE_FORMAT_RGBA16_UNORM;
-        if (colorSpace == VKRT_TEXTURE_COLOR_SPACE_SRGB) {
-            const size_t n = (size_t)png.width * png.height * 4u;
-            float* f = (float*)malloc(n * sizeof(float));
-            if (!f) {
-                free(png.pixels);
-                return VKRT_INVALID_INDEX;
-            }
-            for (size_t i = 0; i < n; i++) {
-                const float v = (float)((uint16_t*)png.pixels)[i] / 65535.0f;
-                f[i] = (i & 3u) == 3u ? v : srgbToLinear(v);
-            }
-            free(png.pixels);
-            e.pixels = f;
-            e.format = VKRT_TEXTURE_FORMAT_RGBA32_SFLOAT;
-            e.colorSpace = VKRT_TEXTURE_COLOR_SPACE_LINEAR;
-        }
-    }
+    e.width = decoded.width;
+    e.height = decoded.height;
+    e.pixels = decoded.pixels;
+    e.format = decoded.format;
+    e.colorSpace = decoded.colorSpace;
     const char* name = jsonName(image) ? jsonName(image) : (jsonName(tex) ? jsonName(tex) : "Texture");
     e.name = duplicateString(name);
     if (out->textureCount == out->textureCapacity) {
@@ -643,10 +634,12 @@ static uint32_t importTexture(GltfDoc* doc, MeshImportData* out, uint32_t* keys,
     return out->textureCount++;
 }
 
-/* A material texture view (loader.c:380-404, 907-1010): texture, sampler
+/* A material texture view (loader.c:372-404, 907-1010): texture, sampler
  * wrap, texture-coordinate set and KHR_texture_transform (its texCoord
- * overrides the view's); sets above 1 are not supported and are skipped. */
-static void attachTexture(GltfDoc* doc, MeshImportData* out, uint32_t* keys, Material* m, const JsonValue* view,
+ * overrides the view's). The set is stored in 8 bits as the reference does;
+ * shading reads TEXCOORD_1 for set 1 and TEXCOORD_0 for any other set
+ * (textures.slang). */
+static void attachTexture(GltfDoc* doc, MeshImportData* out, uint64_t* keys, Material* m, const JsonValue* view,
                           uint32_t slot, uint32_t colorSpace) {
     uint32_t index = 0, set = 0, wrap = 0;
     if (!view || !vkrtJsonToU32(vkrtJsonGet(view, "index"), &index)) return;
@@ -654,10 +647,6 @@ static void attachTexture(GltfDoc* doc, MeshImportData* out, uint32_t* keys, Mat
     if (!(tt && vkrtJsonToU32(vkrtJsonGet(tt, "texCoord"), &set)) &&
         !vkrtJsonToU32(vkrtJsonGet(view, "texCoord"), &set))
         set = 0;
-    if (set > 1u) {
-        GLTF_LOG("texture coordinate set %u is not supported; the texture is skipped", set);
-        return;
-    }
     const uint32_t t = importTexture(doc, out, keys, index, colorSpace, &wrap);
     if (t == VKRT_INVALID_INDEX) return;
     float xf[4] = {1.0f, 1.0f, 0.0f, 0.0f};
@@ -678,10 +667,10 @@ static void attachTexture(GltfDoc* doc, MeshImportData* out, uint32_t* keys, Mat
     *wraps[slot] = wrap;
     memcpy(xfs[slot], xf, sizeof(xf));
     m->textureRotations[slot] = rotation;
-    m->textureTexcoordSets = (m->textureTexcoordSets & ~(0xffu << (8u * slot))) | (set << (8u * slot));
+    m->textureTexcoordSets = (m->textureTexcoordSets & ~(0xffu << (8u * slot))) | ((set & 0xffu) << (8u * slot));
 }
 
-static void attachMaterialTextures(GltfDoc* doc, MeshImportData* out, uint32_t* keys, Material* m,
+static void attachMaterialTextures(GltfDoc* doc, MeshImportData* out, uint64_t* keys, Material* m,
                                    const JsonValue* src) {
     const JsonValue* pbr = vkrtJsonGet(src, "pbrMetallicRoughness");
     attachTexture(doc, out, keys, m, vkrtJsonGet(pbr, "baseColorTexture"), VKRT_MATERIAL_TEXTURE_SLOT_BASE_COLOR,
@@ -1381,9 +1370,9 @@ int meshLoadFromFile(const char* filePath, MeshImportData* out) {
     int result = parseContainer(doc);
     uint32_t textureRefs = 0;
     /* (image, colour space) of each imported texture; at most four per material */
-    uint32_t* keys = NULL;
+    uint64_t* keys = NULL;
     if (result == 0 && doc->materials && doc->materials->type == JSON_ARRAY && doc->materials->count > 0) {
-        keys = (uint32_t*)calloc((size_t)doc->materials->count * 4u, sizeof(uint32_t));
+        keys = (uint64_t*)calloc((size_t)doc->materials->count * 4u, sizeof(uint64_t));
         out->materials = (MaterialImportEntry*)calloc(doc->materials->count, sizeof(MaterialImportEntry));
         if (!out->materials) result = -1;
         for (uint32_t i = 0; result == 0 && i < doc->materials->count; i++) {
@@ -1400,6 +1389,7 @@ int meshLoadFromFile(const char* filePath, MeshImportData* out) {
             if (!out->materials[i].name) result = -1;
         }
     }
+    if (result == 0 && doc->textureFailed) result = -1;
     if (result == 0) result = collectRoots(doc, out);
     if (result == 0 && out->count == 0) {
         GLTF_LOG("No triangle mesh primitives were found in '%s'", filePath);

@@ -124,6 +124,7 @@ struct VKRT {
 
     VKRT_RenderStatusSnapshot renderStatus;
     uint8_t frameTraced;
+    uint8_t finalImageDenoiseEnabled; /* src/core/api/lifecycle.c:338 (default on) */
 
     mrt_context* device;
     VKRT_SystemInfo systemInfo;

@@ -423,6 +423,7 @@ VKRT_Result VKRT_loadSceneFile(VKRT* vkrt, const char* path) {
     uint32_t* savedToLoaded = NULL;
     uint8_t* keep = NULL;
     SceneObjectRecord* objects = NULL;
+    uint32_t* textureMap = NULL;
 
     /* src/app/scene/controller.c:1541-1556 */
     const JsonValue* format = vkrtJsonGet(&root, "format");
@@ -465,12 +466,41 @@ VKRT_Result VKRT_loadSceneFile(VKRT* vkrt, const char* path) {
             SCENE_FAIL("Scene '%s': importing '%s' failed", path, resolved);
     }
 
-    /* loadStandaloneTextures / loadEnvironmentTexture (controller.c:1274-1312):
-     * image files are not decoded by this library */
-    if (textureImports && textureImports->count > 0)
-        SCENE_FAIL("Scene '%s': texture files are not supported (add textures with VKRT_addTextureFromPixels)", path);
-    if (envPath && envPath->type == JSON_STRING)
-        SCENE_FAIL("Scene '%s': environment texture files are not supported (use VKRT_setEnvironmentTexture)", path);
+    /* querySavedTextureIndexMapCount / loadStandaloneTextures (controller.c:690-716,
+     * 1274-1298): each saved standalone texture is loaded from its file and the
+     * saved index is mapped to the new one */
+    uint32_t textureMapCount = 0;
+    for (uint32_t t = 0; textureImports && t < textureImports->count; t++) {
+        const JsonValue* to = &textureImports->items[t];
+        uint32_t saved = 0, colorSpace = 0;
+        const JsonValue* tp = vkrtJsonGet(to, "path");
+        if (to->type != JSON_OBJECT || !vkrtJsonToU32(vkrtJsonGet(to, "index"), &saved) || !tp ||
+            tp->type != JSON_STRING || !vkrtJsonToU32(vkrtJsonGet(to, "colorSpace"), &colorSpace))
+            SCENE_FAIL("Scene '%s': texture import %u is malformed", path, t);
+        if (saved >= (1u << 20)) SCENE_FAIL("Scene '%s': texture index %u too large", path, saved);
+        if (saved + 1u > textureMapCount) textureMapCount = saved + 1u;
+    }
+    textureMap = (uint32_t*)malloc(sizeof(uint32_t) * (textureMapCount + 1u));
+    if (!textureMap) SCENE_FAIL("out of memory");
+    for (uint32_t t = 0; t < textureMapCount; t++) textureMap[t] = VKRT_INVALID_INDEX;
+    for (uint32_t t = 0; textureImports && t < textureImports->count; t++) {
+        const JsonValue* to = &textureImports->items[t];
+        uint32_t saved = 0, colorSpace = 0, loaded = VKRT_INVALID_INDEX;
+        char resolved[8192];
+        vkrtJsonToU32(vkrtJsonGet(to, "index"), &saved);
+        vkrtJsonToU32(vkrtJsonGet(to, "colorSpace"), &colorSpace);
+        if (!resolveAssetPath(path, vkrtJsonGet(to, "path")->string, resolved, sizeof(resolved)) ||
+            VKRT_addTextureFromFile(vkrt, resolved, NULL, colorSpace, &loaded) != VKRT_SUCCESS)
+            SCENE_FAIL("Scene '%s': texture import %u could not be loaded", path, t);
+        textureMap[saved] = loaded;
+    }
+    /* loadEnvironmentTexture (controller.c:1300-1312) */
+    if (envPath && envPath->type == JSON_STRING) {
+        char resolved[8192];
+        if (!resolveAssetPath(path, envPath->string, resolved, sizeof(resolved)) ||
+            VKRT_setEnvironmentTextureFromFile(vkrt, resolved) != VKRT_SUCCESS)
+            SCENE_FAIL("Scene '%s': environment texture '%s' could not be loaded", path, envPath->string);
+    }
 
     /* buildSavedMeshIndexMap (controller.c:1314-1337) */
     uint32_t savedMeshCount = meshes->count;
@@ -526,9 +556,15 @@ VKRT_Result VKRT_loadSceneFile(VKRT* vkrt, const char* path) {
         if ((explicitIndex && !vkrtJsonToU32(explicitIndex, &index)) || !name || name->type != JSON_STRING ||
             !parseMaterial(vkrtJsonGet(mo, "material"), &mat))
             SCENE_FAIL("Scene '%s': material entry %u is malformed", path, i);
-        /* standalone textures are not loaded, so saved texture slots cannot be resolved */
-        mat.baseColorTextureIndex = mat.metallicRoughnessTextureIndex = VKRT_INVALID_INDEX;
-        mat.normalTextureIndex = mat.emissiveTextureIndex = VKRT_INVALID_INDEX;
+        /* remapStandaloneTextureIndices (controller.c:640-661): slots that name a
+         * saved standalone texture move to its new index; the others keep the
+         * index of the texture the mesh imports created */
+        uint32_t* slots[4] = {&mat.baseColorTextureIndex, &mat.metallicRoughnessTextureIndex,
+                              &mat.normalTextureIndex, &mat.emissiveTextureIndex};
+        for (int k = 0; k < 4; k++)
+            if (*slots[k] != VKRT_INVALID_INDEX && *slots[k] < textureMapCount &&
+                textureMap[*slots[k]] != VKRT_INVALID_INDEX)
+                *slots[k] = textureMap[*slots[k]];
         if (VKRT_setMaterial(vkrt, index, &mat) != VKRT_SUCCESS ||
             VKRT_setMaterialName(vkrt, index, name->string) != VKRT_SUCCESS)
             SCENE_FAIL("Scene '%s': setting material %u failed", path, index);
@@ -632,6 +668,7 @@ VKRT_Result VKRT_loadSceneFile(VKRT* vkrt, const char* path) {
     free(savedToLoaded);
     free(keep);
     free(objects);
+    free(textureMap);
     vkrtJsonFree(&root);
     dropBackup(&backup);
     return VKRT_SUCCESS;
@@ -642,6 +679,7 @@ fail:
     free(savedToLoaded);
     free(keep);
     free(objects);
+    free(textureMap);
     vkrtJsonFree(&root);
     if (asideDone) restoreScene(vkrt, &backup);
     return VKRT_ERROR_OPERATION_FAILED;
