@@ -84,6 +84,8 @@ def parse_args():
                         "-1: no PRB variant)")
     p.add_argument("--nrc-frames", type=int, default=240,
                    help="orbit frames of the neural radiance cache variant (0 skips it)")
+    p.add_argument("--no-stress", dest="stress", action="store_false",
+                   help="skip the 20.8M-triangle stress variant of the default run")
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                    help="only run device-to-device copies of MIB mebibytes (reference kernel for HBM counters)")
     return p.parse_args()
@@ -124,6 +126,56 @@ def build_workload(kind):
     r.start(WIDTH, HEIGHT, 0)
     log(f"{kind} scene ready in {time.time() - t0:.1f} s: {scene.triangle_count} triangles")
     return r, scene
+
+
+def stress_variant(steps: int = 4, warmup: int = 2):
+    """The 20,781,120-triangle incoherent-traversal configuration at 1080p,
+    16 spp, depth 16: BVH build time, Gbounces/s and the traversal kernel's
+    fraction of the HBM roof on the same algorithmic-bytes model."""
+    r, scene = build_workload("stress")
+    for _ in range(warmup):
+        r.draw()
+    st = r.stats()
+    build_ms = st.bvhBuildMs
+    r.synchronize()
+    t0 = time.perf_counter()
+    bounces = ext_ms = launches = 0
+    for _ in range(steps):
+        r.draw()
+        s = r.stats()
+        bounces += s.extensionRays
+        ext_ms += s.extendMs
+        launches += s.extendLaunches
+    r.synchronize()
+    elapsed = time.perf_counter() - t0
+    r.set_traversal_statistics(True)
+    r.draw()
+    s = r.stats()
+    r.set_traversal_statistics(False)
+    nodes = s.nodeVisits / max(1, s.extensionRays)
+    tris = s.triangleTests / max(1, s.extensionRays)
+    per_ray = B_EXTEND_IO + nodes * NODE_BYTES + tris * TRI_BYTES
+    achieved = bounces * per_ray / (ext_ms * 1e-3) / 1e9 if ext_ms > 0 else 0.0
+    out = {
+        "workload": WORKLOADS["stress"]["name"],
+        "triangles": scene.triangle_count,
+        "value": round(bounces / elapsed / 1e9, 4),
+        "unit": "Gbounces/s",
+        "rays_per_s": round(bounces / elapsed),
+        "steps": steps,
+        "ms_per_step": round(elapsed * 1e3 / steps, 3),
+        "bounces_per_step": bounces // steps,
+        "bvh": {"build_ms": round(build_ms, 2), "nodes": st.bvhNodeCount, "depth": st.bvhMaxDepth},
+        "kExtend": {"achieved": round(achieved, 1), "frac": round(achieved / HBM_PEAK_GBPS, 4), "unit": "GB/s",
+                    "avg_launch_ms": round(ext_ms / max(1, launches), 4), "launches": launches,
+                    "bytes_per_ray": round(per_ray, 1), "nodes_per_ray": round(nodes, 3),
+                    "tris_per_ray": round(tris, 3)},
+    }
+    log(f"stress: {scene.triangle_count} triangles, BVH build {build_ms:.1f} ms, {out['value']:.3f} Gbounces/s, "
+        f"kExtend {achieved:.0f} GB/s ({achieved / HBM_PEAK_GBPS:.3f} of peak), {nodes:.2f} nodes/ray, "
+        f"{tris:.2f} triangles/ray")
+    r.close()
+    return out
 
 
 def cpu_baseline(renderer, rows: int, spp: int, depth: int):
@@ -503,6 +555,9 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(r, args.cpu_rows if args.workload == "spectral" else 64, SPP, MAX_DEPTH)
+    r.close()  # frees the path state before the stress variant allocates its own
+    if args.workload == "spectral" and args.stress and args.steps > 0:
+        variants["stress"] = stress_variant()
 
     line = {
         "metric": "Gbounces/s at 1080p spectral" if args.workload == "spectral" else "Gbounces/s at 1080p, 20.8M-triangle stress",
@@ -535,7 +590,6 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
-    r.close()
 
 
 if __name__ == "__main__":

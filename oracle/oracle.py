@@ -56,6 +56,9 @@ def lib():
         g("oracle_hash").restype = C.c_uint32
         g("oracle_hash").argtypes = [C.c_uint32]
         g("oracle_hash_states").argtypes = [C.c_int, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, u32p]
+        g("oracle_sheen_ltc").restype = rt
+        g("oracle_sheen_ltc").argtypes = [rt, rt, C.c_uint32]
+        g("oracle_sheen_eval").argtypes = [rt, rp, rp, rp]
         g("oracle_camera_ray").argtypes = [P, C.c_int, C.c_int, rt, rt, rp]
         g("oracle_intersect").restype = C.c_int
         g("oracle_intersect").argtypes = [f32p, rp, rp]
@@ -137,7 +140,7 @@ class OracleScene:
         bounces = C.c_uint64()
         rc = getattr(lib(), "oracle_render" + sfx)(
             self.addr, self.bvh, width, height, ptr(accum), ptr(out) if out is not None else None, r0, r1,
-            threads or os.cpu_count() or 1, C.byref(bounces))
+            threads or default_threads(), C.byref(bounces))
         assert rc == 0
         return (accum, out, bounces.value) if output else (accum, bounces.value)
 
@@ -146,7 +149,7 @@ class OracleScene:
         prim = np.zeros(width * height, dtype=np.uint32)
         _, sfx = _dtype(precision)
         rc = getattr(lib(), "oracle_primary_hits" + sfx)(self.addr, self.bvh, width, height, sample_index,
-                                                         prev_samples, ptr(mesh), ptr(prim), threads or os.cpu_count())
+                                                         prev_samples, ptr(mesh), ptr(prim), threads or default_threads())
         assert rc == 0
         return mesh, prim
 
@@ -159,8 +162,28 @@ class OracleScene:
         bary = np.zeros((n, 2), dtype=np.float32)
         _, sfx = _dtype(precision)
         getattr(lib(), "oracle_trace_rays" + sfx)(self.addr, self.bvh, ptr(rays), n, ptr(mesh), ptr(prim), ptr(t),
-                                                  ptr(bary), threads or os.cpu_count())
+                                                  ptr(bary), threads or default_threads())
         return mesh, prim, t, bary
+
+
+def default_threads():
+    """OpenMP threads for the oracle: the cores this process may use, capped by
+    OMP_NUM_THREADS when the host share is set."""
+    n = len(os.sched_getaffinity(0))
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return max(1, min(n, share) if share > 0 else n)
+
+
+def sheen_ltc(cos_theta, roughness, layer):
+    return float(lib().oracle_sheen_ltc_f64(cos_theta, roughness, layer))
+
+
+def sheen_eval(roughness, wo, wi):
+    """The bare sheen lobe, unit colour: (albedo * density, density)."""
+    out = np.zeros(2)
+    lib().oracle_sheen_eval_f64(roughness, ptr(np.ascontiguousarray(wo, dtype=np.float64)),
+                                ptr(np.ascontiguousarray(wi, dtype=np.float64)), ptr(out))
+    return float(out[0]), float(out[1])
 
 
 def rand_sequence(policy, seed, px, py, frame, sample, n, precision=32):

@@ -153,14 +153,17 @@ def test_trace_without_device_fails_loudly(api):
     assert api.lib.VKRT_traceRays(api.h, None, 0, None, None, None, None) != 0
 
 
-def test_sheen_is_rejected_to_zero(api):
+def test_sheen_is_kept_and_clamped_like_the_reference(api):
+    # src/core/api/mesh.c:112-131: tint and weight clamped to [0, 1], non-finite -> 0
     m = V.default_material()
-    m.sheenTintWeight[3] = 0.7
+    m.sheenTintWeight[:] = (0.2, 1.5, float("nan"), 0.7)
+    m.sheenRoughness = -3.0
     idx = api.add_material(m)
     b = api.export()
     mats = np.ctypeslib.as_array(C.cast(b.materials, C.POINTER(C.c_float)), shape=(b.materialCount * 68,))
-    w = V.Material.sheenTintWeight.offset // 4 + 3
-    assert mats.reshape(-1, 68)[idx, w] == 0.0
+    o = V.Material.sheenTintWeight.offset // 4
+    np.testing.assert_array_equal(mats.reshape(-1, 68)[idx, o:o + 4], np.float32([0.2, 1.0, 0.0, 0.7]))
+    assert mats.reshape(-1, 68)[idx, V.Material.sheenRoughness.offset // 4] == 0.0
 
 
 REFERENCE_APP = "/root/reference/src/app"

@@ -454,6 +454,10 @@ MATERIALS = {
     "smooth_metal": dict(baseColor=(0.9, 0.9, 0.9), roughness=0.1, metallic=1.0),
     "clearcoat": dict(baseColor=(0.3, 0.5, 0.7), roughness=0.6, clearcoat=1.0, clearcoatGloss=0.7),
     "rough_glass": dict(baseColor=(1.0, 1.0, 1.0), roughness=0.4, transmission=1.0, ior=1.5),
+    "sheen_diffuse": dict(baseColor=(0.5, 0.2, 0.2), roughness=1.0, specular=0.0, sheenTintWeight=(1.0, 0.9, 0.8, 1.0),
+                          sheenRoughness=0.5),
+    "sheen_clearcoat": dict(baseColor=(0.2, 0.3, 0.6), roughness=0.5, sheenTintWeight=(0.6, 0.6, 1.0, 0.7),
+                            sheenRoughness=0.2, clearcoat=0.5, clearcoatGloss=0.6),
 }
 
 
@@ -482,6 +486,76 @@ def test_bsdf_sampling_agrees_with_eval(lut_buffers, name):
     integral = pdfs.mean() * 4.0 * math.pi
     assert integral <= 1.0 + 0.03
     assert integral == pytest.approx(usable.mean(), abs=0.04)
+
+
+# ---------------------------------------------------------------- sheen --
+SHEEN_REFERENCE_TABLE = "/root/reference/src/shaders/bsdf/data/sheen_ltc.slang"
+
+
+def carried_sheen_table():
+    import re
+    text = open(os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                             "vkrt_amd", "csrc", "sheen_ltc_table.inc")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return np.array([float(v) for v in re.findall(r"-?\d+\.\d+", text)], np.float32).reshape(3, 32, 32)
+
+
+@pytest.mark.skipif(not os.path.exists(SHEEN_REFERENCE_TABLE), reason="reference sources are not mounted")
+def test_sheen_table_is_the_reference_table():
+    import re
+    text = open(SHEEN_REFERENCE_TABLE).read()
+    body = text[text.index("SHEEN_LTC_TABLE[3072]"):]
+    body = body[body.index("{") + 1:body.index("}")]
+    ref = np.array([float(v) for v in re.findall(r"-?\d+\.\d+(?:e-?\d+)?", body)], np.float32)
+    np.testing.assert_array_equal(carried_sheen_table().ravel(), ref)
+
+
+def test_sheen_lookup_hits_table_nodes_and_interpolates():
+    """sheen.slang:22-39: rows are roughness, columns cos(theta), layers a, b,
+    albedo; exact at the nodes, bilinear between them, clamped outside."""
+    t = carried_sheen_table()
+    assert t[2].min() >= 0.0 and t[2].max() <= 1.0  # a directional albedo
+    for layer in range(3):
+        for i, j in ((0, 0), (5, 17), (31, 31), (12, 3), (20, 30)):
+            assert O.sheen_ltc(j / 31.0, i / 31.0, layer) == pytest.approx(float(t[layer, i, j]), abs=1e-7)
+        c, r = (7.25) / 31.0, (9.5) / 31.0
+        want = ((t[layer, 9, 7] * 0.75 + t[layer, 9, 8] * 0.25) * 0.5 + (t[layer, 10, 7] * 0.75 + t[layer, 10, 8] * 0.25) * 0.5)
+        assert O.sheen_ltc(c, r, layer) == pytest.approx(float(want), rel=1e-5, abs=1e-7)
+        assert O.sheen_ltc(1.7, -0.3, layer) == pytest.approx(float(t[layer, 0, 31]), abs=1e-7)
+
+
+@pytest.mark.parametrize("theta,rough", [(10, 0.3), (45, 0.5), (70, 0.8), (30, 1.0)])
+def test_sheen_density_is_a_normalised_ltc(theta, rough):
+    """The lobe's density is a linearly transformed clamped cosine, which
+    integrates to 1 over the upper hemisphere whatever the transform; the
+    value is the directional albedo times it, so the lobe's energy is the
+    table's albedo (at most 1)."""
+    wo = local_dir(theta, 30)
+    dirs = sphere_dirs(200000)
+    dirs = dirs[dirs[:, 2] > 0]
+    vals = np.array([O.sheen_eval(rough, wo, d) for d in dirs])
+    integral = vals[:, 1].mean() * 2.0 * math.pi
+    assert integral == pytest.approx(1.0, abs=0.01)
+    albedo = O.sheen_ltc(math.cos(math.radians(theta)), rough, 2)
+    np.testing.assert_allclose(vals[:, 0], albedo * vals[:, 1], rtol=1e-12, atol=0)
+    assert 0.0 < albedo <= 1.0
+    # below the horizon on either side there is no sheen
+    assert O.sheen_eval(rough, wo, local_dir(120)) == (0.0, 0.0)
+    assert O.sheen_eval(rough, local_dir(100), local_dir(20)) == (0.0, 0.0)
+
+
+def test_sheen_attenuates_the_base_by_its_albedo(lut_buffers):
+    """weights.slang:27-62 / eval_rgb.slang:92-100: a sheen layer adds its lobe
+    and scales everything under it by 1 - weight * albedo(wo)."""
+    base = dict(baseColor=(0.6, 0.6, 0.6), **LAMBERT)
+    wo, wi = local_dir(35, 0), local_dir(50, 140)
+    v0, _ = O.bsdf_eval(lut_buffers, V.default_material(**base), wo, wi)
+    rough, w = 0.4, 0.8
+    m = V.default_material(sheenTintWeight=(1.0, 1.0, 1.0, w), sheenRoughness=rough, **base)
+    v1, _ = O.bsdf_eval(lut_buffers, m, wo, wi)
+    att = 1.0 - w * O.sheen_ltc(wo[2], rough, 2)
+    lobe = w * O.sheen_eval(rough, wo, wi)[0]
+    np.testing.assert_allclose(v1, v0 * att + lobe, rtol=1e-6)
 
 
 def test_metal_reciprocity_and_energy(lut_buffers):

@@ -185,6 +185,11 @@ extern "C" int mrt_upload_materials(mrt_context* ctx, const Material* materials,
     if (!ctx) return -1;
     if (!check(hipSetDevice(ctx->device), "hipSetDevice")) return -2;
     if (!ctx->materials.upload(materials, materialCount, ctx->stream)) return -3;
+    ctx->sheenEnabled = false;
+    for (uint32_t i = 0; i < materialCount; i++) {
+        const float* tw = materials[i].sheenTintWeight;
+        if (tw[3] > 0.0f && (tw[0] > 0.0f || tw[1] > 0.0f || tw[2] > 0.0f)) ctx->sheenEnabled = true;
+    }
     ctx->triClassDirty = true;
     return sync(ctx, "upload materials");
 }

@@ -1,10 +1,19 @@
-// Layered principled BSDF on the GPU (src/shaders/bsdf/): clearcoat over a
-// metal / dielectric / diffuse+subsurface base, one-sample lobe selection,
-// RGB, single-wavelength and four-wavelength (hero) evaluation. Sheen is not
-// carried (the host rejects sheen weights), so the sheen layer is identity.
+// Layered principled BSDF on the GPU (src/shaders/bsdf/): microfiber sheen
+// over clearcoat over a metal / dielectric / diffuse+subsurface base,
+// one-sample lobe selection, RGB, single-wavelength and four-wavelength (hero)
+// evaluation.
 #pragma once
 
 namespace mrt {
+
+// Microfiber sheen LTC fit (Zeltner, Burley, Chiang 2022, as tabulated by
+// Blender Cycles): [layer][roughness row][cos-theta column], 32 x 32 per
+// layer; layers are the LTC transform a, transform b and directional albedo
+// (src/shaders/bsdf/data/sheen_ltc.slang). 12 KB, read through the caches.
+constexpr uint32_t kSheenLtcSize = 32u;
+static __constant__ float kSheenLtc[3u * kSheenLtcSize * kSheenLtcSize] = {
+#include "../sheen_ltc_table.inc"
+};
 
 // src/shaders/bsdf/base/material.slang:4-48 (fields the lobes read)
 struct BMat {
@@ -18,6 +27,8 @@ struct BMat {
     float absorptionCoefficient;
     F3 attenuationColor;
     F3 emission;  // emissionColor * emissionLuminance after the texture stage
+    F3 sheenColor;  // saturate(tint) * saturate(weight), src/shaders/bsdf/base/material.slang:63-73
+    float sheenRoughness;
 };
 
 // Material fetch plus applySurfaceTextures with no texture bound
@@ -46,6 +57,8 @@ DEV BMat loadMaterial(const Material* mats, uint32_t idx, F3 vertexColor, F3 tex
     b.subsurface = m.subsurface;
     b.absorptionCoefficient = m.absorptionCoefficient;
     b.attenuationColor = F3{m.attenuationColor[0], m.attenuationColor[1], m.attenuationColor[2]};
+    b.sheenColor = sat(F3{m.sheenTintWeight[0], m.sheenTintWeight[1], m.sheenTintWeight[2]}) * sat(m.sheenTintWeight[3]);
+    b.sheenRoughness = m.sheenRoughness;
     F3 em = F3{m.emissionColor[0], m.emissionColor[1], m.emissionColor[2]} * m.emissionLuminance;
     em = em * texEmission;
     float emax = maxc(em);
@@ -361,8 +374,73 @@ DEV float evalTransLane(const BMat& m, F3 wo, F3 wi, uint32_t ff, const GGX& g, 
 
 // --------------------------------------------------------- principled --
 struct Weights {
-    float coat, metal, dielectric, diffuse, subsurface;
+    float sheen, coat, metal, dielectric, diffuse, subsurface;
 };
+
+// ---------------------------------------------------------------- sheen --
+// src/shaders/bsdf/lobes/sheen.slang:22-113. The lobe is a linearly
+// transformed cosine in a frame whose x axis is wo projected onto the surface.
+DEV float sheenLtc(float cosTheta, float roughness, uint32_t layer) {
+    const float last = (float)(kSheenLtcSize - 1u);
+    const float ux = sat(cosTheta) * last, uy = sat(roughness) * last;
+    const uint32_t x0 = min((uint32_t)ux, kSheenLtcSize - 1u), y0 = min((uint32_t)uy, kSheenLtcSize - 1u);
+    const uint32_t x1 = min(x0 + 1u, kSheenLtcSize - 1u), y1 = min(y0 + 1u, kSheenLtcSize - 1u);
+    const float tx = ux - floorf(ux), ty = uy - floorf(uy);
+    const float* t = kSheenLtc + layer * kSheenLtcSize * kSheenLtcSize;
+    const float e00 = t[y0 * kSheenLtcSize + x0], e10 = t[y0 * kSheenLtcSize + x1];
+    const float e01 = t[y1 * kSheenLtcSize + x0], e11 = t[y1 * kSheenLtcSize + x1];
+    return lerpf(lerpf(e00, e10, tx), lerpf(e01, e11, tx), ty);
+}
+
+struct SheenLtc {
+    Basis frame;
+    float a, b, albedo;
+};
+
+DEV SheenLtc makeSheenLtc(F3 wo, float sheenRoughness) {
+    SheenLtc p;
+    p.frame = makeBasis(F3{0.0f, 0.0f, 1.0f}, F4{wo.x, wo.y, wo.z, 1.0f});
+    p.a = p.b = p.albedo = 0.0f;
+    if (wo.z <= 0.0f) return p;
+    const float r = clampf(sheenRoughness, 1e-3f, 1.0f);
+    p.a = sheenLtc(wo.z, r, 0u);
+    p.b = sheenLtc(wo.z, r, 1u);
+    p.albedo = sheenLtc(wo.z, r, 2u);
+    return p;
+}
+
+// LTC density of a direction given in the sheen frame (also the sampling pdf)
+DEV float sheenDensity(F3 lw, const SheenLtc& p) {
+    const float z = fmaxf(lw.z, 0.0f);
+    if (z <= 0.0f || fabsf(p.a) < 1e-5f || p.albedo < 1e-5f) return 0.0f;
+    const float axbz = p.a * lw.x + p.b * lw.z, ay = p.a * lw.y;
+    const float len2 = axbz * axbz + ay * ay + lw.z * lw.z;
+    if (len2 <= 1e-6f) return 0.0f;
+    const float scale = p.a / len2;
+    return kInvPi * z * scale * scale;
+}
+
+DEV float sheenAlbedo(float cosTheta, float sheenRoughness) {
+    return sheenLtc(cosTheta, clampf(sheenRoughness, 1e-3f, 1.0f), 2u);
+}
+DEV float sheenLayerAtt(float weight, float cosTheta, float sheenRoughness) {
+    return sat(1.0f - weight * sheenAlbedo(cosTheta, sheenRoughness));
+}
+// src/shaders/bsdf/principled/attenuation.slang:25-28
+DEV float sheenAtt(F3 sheenColor, F3 w, float sheenRoughness) {
+    return sheenLayerAtt(maxc(sheenColor), w.z, sheenRoughness);
+}
+
+// evalSheen with a unit colour: value = albedo * density, pdf = density
+DEV Eval evalSheenLobe(float sheenRoughness, F3 wo, F3 wi) {
+    Eval e{f3(0.0f), 0.0f};
+    if (wo.z <= 0.0f || wi.z <= 0.0f) return e;
+    const SheenLtc p = makeSheenLtc(wo, sheenRoughness);
+    const float d = sheenDensity(toLocal(wi, p.frame), p);
+    e.value = f3(p.albedo * d);
+    e.pdf = d;
+    return e;
+}
 
 struct BState {
     BMat m;
@@ -392,7 +470,10 @@ DEV F3 dielectricAtt(const BMat& m, F3 w) {
     return sat(f3(1.0f) - albedo);
 }
 
-// src/shaders/bsdf/principled/weights.slang:4-62 (sheen weight is zero)
+// src/shaders/bsdf/principled/weights.slang:4-62. SH = false compiles the
+// sheen layer out (weight 0, attenuation 1) for scenes without sheen, which
+// keeps the shading kernel inside its register budget.
+template <bool SH = true>
 DEV Weights makeWeights(const BMat& m, F3 wo, uint32_t ff) {
     Weights w;
     F3 dc = diffuseColor(m);
@@ -403,14 +484,16 @@ DEV Weights makeWeights(const BMat& m, F3 wo, uint32_t ff) {
     float transW = m.transmission * fmaxf(lum(transmissionColor(m)), 1e-3f);
     float dielW = nonMetal * fmaxf(sat(lum(specularF0(m))) + transW, 0.0f);
     float total;
+    w.sheen = SH ? lum(m.sheenColor) : 0.0f;
     if (m.transmission > 0.0f && ff == 0u) {
+        w.sheen = 0.0f;
         w.coat = 0.0f;
         w.metal = m.metallic * baseW;
         w.dielectric = dielW;
         w.diffuse = 0.0f;
         w.subsurface = 0.0f;
     } else {
-        const float vs = 1.0f;  // sheenLayerAttenuation(0, ..)
+        const float vs = SH ? sheenAtt(m.sheenColor, wo, m.sheenRoughness) : 1.0f;
         float vc = coatAtt(m, wo);
         float da = lum(dielectricAtt(m, wo));
         w.coat = vs * m.clearcoat * 0.25f;
@@ -419,11 +502,12 @@ DEV Weights makeWeights(const BMat& m, F3 wo, uint32_t ff) {
         w.diffuse = vs * vc * da * baseScatter * (1.0f - m.subsurface) * diffW;
         w.subsurface = vs * vc * da * baseScatter * m.subsurface * diffW;
     }
-    total = 0.0f + w.coat + w.metal + w.dielectric + w.diffuse + w.subsurface;
+    total = w.sheen + w.coat + w.metal + w.dielectric + w.diffuse + w.subsurface;
     if (total <= 0.0f) {
-        w = Weights{0.0f, 0.0f, 0.0f, 1.0f, 0.0f};
+        w = Weights{0.0f, 0.0f, 0.0f, 0.0f, 1.0f, 0.0f};
         return w;
     }
+    w.sheen /= total;
     w.coat /= total;
     w.metal /= total;
     w.dielectric /= total;
@@ -432,6 +516,7 @@ DEV Weights makeWeights(const BMat& m, F3 wo, uint32_t ff) {
     return w;
 }
 
+template <bool SH = true>
 DEV BState makeState(const BMat& m, F3 wo, uint32_t ff, float l, uint32_t spectral) {
     BState s;
     s.m = m;
@@ -441,18 +526,31 @@ DEV BState makeState(const BMat& m, F3 wo, uint32_t ff, float l, uint32_t spectr
     s.spectral = spectral;
     s.g = makeGGX(m, wo);
     s.ccAlpha = ccAlpha(m);
-    s.w = makeWeights(m, wo, ff);
+    s.w = makeWeights<SH>(m, wo, ff);
     return s;
 }
 DEV bool interior(const BState& s) { return s.m.transmission > 0.0f && s.ff == 0u; }
 
 // src/shaders/bsdf/principled/eval_rgb.slang:4-140 (RGB and single wavelength)
+template <bool SH = true>
 DEV Eval evalScalar(const Spec& sp, const BState& s, F3 wi) {
     Eval e{f3(0.0f), 0.0f};
     const BMat& m = s.m;
     if (wi.z > 0.0f) {
         bool inner = interior(s);
-        F3 coatV = f3(0.0f), metalV = f3(0.0f), dielV = f3(0.0f), subV = f3(0.0f);
+        F3 coatV = f3(0.0f), metalV = f3(0.0f), dielV = f3(0.0f), subV = f3(0.0f), sheenV = f3(0.0f);
+        F3 sheenColor = f3(0.0f);
+        if constexpr (SH) {
+            if (s.w.sheen > 0.0f || !inner) {
+                sheenColor = m.sheenColor;
+                if (s.spectral) sheenColor = f3(spec1(sp, sat(sheenColor), s.l));
+            }
+            if (s.w.sheen > 0.0f) {
+                Eval sh = evalSheenLobe(m.sheenRoughness, s.wo, wi);
+                sheenV = sheenColor * sh.value.x;
+                e.pdf += s.w.sheen * sh.pdf;
+            }
+        }
         if (s.w.coat > 0.0f) {
             Eval c = evalClearcoat(m.clearcoat, s.wo, wi, s.ccAlpha);
             coatV = c.value;
@@ -508,13 +606,16 @@ DEV Eval evalScalar(const Spec& sp, const BState& s, F3 wi) {
             e.value = metalV * m.metallic + dielV * nonMetal;
             return e;
         }
+        float vs = SH ? sheenAtt(sheenColor, s.wo, m.sheenRoughness) : 1.0f;
         float vc = coatAtt(m, s.wo);
         F3 vd = dielectricAtt(m, s.wo);
         F3 base = coatV + (metalV * m.metallic + (dielV + vd * subV) * nonMetal) * (vc * coatAtt(m, wi));
-        e.value = f3(0.0f) + base * 1.0f;
+        e.value = sheenV + base * vs;
         return e;
     }
-    float ca = interior(s) ? 1.0f : 1.0f * coatAtt(m, s.wo);
+    // src/shaders/bsdf/principled/attenuation.slang:57-66: sheen and coat of the
+    // material's own (RGB) sheen colour over the transmitted lobe
+    float ca = interior(s) ? 1.0f : (SH ? sheenAtt(m.sheenColor, s.wo, m.sheenRoughness) : 1.0f) * coatAtt(m, s.wo);
     F3 tc = transmissionColor(m) * ((1.0f - m.metallic) * m.transmission);
     if (s.spectral) tc = f3(spec1(sp, sat(tc), s.l));
     float pdf;
@@ -532,13 +633,22 @@ DEV Eval evalScalar(const Spec& sp, const BState& s, F3 wi) {
 }
 
 // src/shaders/bsdf/principled/eval_spectral.slang:4-115 (four hero lanes)
+template <bool SH = true>
 DEV F4 evalHero(const Spec& sp, const BState& s, F3 wi, F4 l, F4& tp) {
     const BMat& m = s.m;
     tp = f4(0.0f);
     if (wi.z > 0.0f) {
         bool inner = interior(s);
         float coatV = 0.0f;
-        F4 metalV = f4(0.0f), dielV = f4(0.0f), subV = f4(0.0f);
+        F4 metalV = f4(0.0f), dielV = f4(0.0f), subV = f4(0.0f), sheenV = f4(0.0f), sheenColor = f4(0.0f);
+        if constexpr (SH) {
+            if (s.w.sheen > 0.0f || !inner) sheenColor = spec4(sp, sat(m.sheenColor), l);
+            if (s.w.sheen > 0.0f) {
+                Eval sh = evalSheenLobe(m.sheenRoughness, s.wo, wi);
+                sheenV = sheenColor * sh.value.x;
+                tp = tp + f4(s.w.sheen * sh.pdf);
+            }
+        }
         if (s.w.coat > 0.0f) {
             Eval c = evalClearcoat(m.clearcoat, s.wo, wi, s.ccAlpha);
             coatV = c.value.x;
@@ -597,13 +707,14 @@ DEV F4 evalHero(const Spec& sp, const BState& s, F3 wi, F4 l, F4& tp) {
         float nonMetal = 1.0f - m.metallic;
         if (inner) return metalV * m.metallic + dielV * nonMetal;
         float vc = coatAtt(m, s.wo);
-        F4 vs = sat(f4(1.0f) + f4(0.0f) * 0.0f);
+        F4 vs = f4(1.0f);
+        if constexpr (SH) vs = sat(f4(1.0f) - sheenColor * sheenAlbedo(s.wo.z, m.sheenRoughness));
         F4 vd = spec4(sp, sat(dielectricAtt(m, s.wo)), l);
         float ra = vc * coatAtt(m, wi);
         F4 base = f4(coatV) + (metalV * m.metallic + (dielV + vd * subV) * nonMetal) * ra;
-        return f4(0.0f) + vs * base;
+        return sheenV + vs * base;
     }
-    float ca = interior(s) ? 1.0f : 1.0f * coatAtt(m, s.wo);
+    float ca = interior(s) ? 1.0f : (SH ? sheenAtt(m.sheenColor, s.wo, m.sheenRoughness) : 1.0f) * coatAtt(m, s.wo);
     F4 tc = spec4(sp, sat(transmissionColor(m)), l) * ((1.0f - m.metallic) * m.transmission);
     F4 v;
     v.x = evalTransLane(m, s.wo, wi, s.ff, s.g, ca, tc.x, l.x, 1u, tp.x);
@@ -615,12 +726,24 @@ DEV F4 evalHero(const Spec& sp, const BState& s, F3 wi, F4 l, F4& tp) {
 }
 
 // src/shaders/bsdf/principled/sample.slang:4-51 with the lobe samplers
+template <bool SH = true>
 DEV bool sampleLobe(const BState& s, Rng& rng, F3& wi, uint32_t& isT) {
     wi = f3(0.0f);
     isT = 0u;
     const BMat& m = s.m;
     float sel = rnd(rng);
-    sel -= 0.0f;  // sheen weight
+    if (SH && sel < s.w.sheen) {
+        // sheen.slang:99-113: a cosine-distributed disk sample through the LTC transform
+        const SheenLtc p = makeSheenLtc(s.wo, m.sheenRoughness);
+        if (fabsf(p.a) < 1e-5f || p.albedo < 1e-5f) return false;
+        const float r = sqrtf(rnd(rng));
+        const float phi = 2.0f * kPi * rnd(rng);
+        const float dx = r * cosf(phi), dy = r * sinf(phi);
+        const float dz = sqrtf(fmaxf(1.0f - (dx * dx + dy * dy), 0.0f));
+        wi = toWorld(normalize(F3{dx - dz * p.b, dy, dz * p.a}), p.frame);
+        return wi.z > 0.0f;
+    }
+    sel -= s.w.sheen;
     if (sel < s.w.coat) {
         float u1 = rnd(rng);
         float u2 = rnd(rng);

@@ -39,6 +39,7 @@ DEV F3 operator/(F3 a, float s) { return F3{a.x / s, a.y / s, a.z / s}; }
 DEV F3 operator-(F3 a) { return F3{-a.x, -a.y, -a.z}; }
 DEV F4 operator+(F4 a, F4 b) { return F4{a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w}; }
 DEV F4 operator*(F4 a, F4 b) { return F4{a.x * b.x, a.y * b.y, a.z * b.z, a.w * b.w}; }
+DEV F4 operator-(F4 a, F4 b) { return F4{a.x - b.x, a.y - b.y, a.z - b.z, a.w - b.w}; }
 DEV F4 operator*(F4 a, float s) { return F4{a.x * s, a.y * s, a.z * s, a.w * s}; }
 DEV F4 operator/(F4 a, float s) { return F4{a.x / s, a.y / s, a.z / s, a.w / s}; }
 

@@ -90,6 +90,7 @@ struct mrt_context {
     mrt::DeviceArray<float4> texels;  // material texture pool (linear RGBA)
     mrt::DeviceArray<uint4> texInfo;  // per texture: first texel, width, height
     bool texturesEnabled = false;     // some material references a texture
+    bool sheenEnabled = false;        // some material has a sheen layer (selects the sheen shading variants)
     mrt::DeviceArray<uint32_t> meshTriBase;
     mrt::DeviceArray<mrt::TriRecord> tris;
     mrt::DeviceArray<mrt::BvhNode> nodes;

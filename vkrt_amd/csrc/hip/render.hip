@@ -644,8 +644,10 @@ __global__ __launch_bounds__(kBlock) void kNrcTargets(const float4* record, cons
 // bit. What keeps the hero variant inside the budget: the rgb2spec resolution is
 // a compile-time constant, medium coefficients and the scalar-wavelength state
 // are read only where used, and shadow rays are queued before BSDF sampling.
-template <int MODE, bool NRC = false, bool TEX = false>
-__global__ __launch_bounds__(kBlock, 3) void kShade(FrameParams fp, SceneView sv, PathView pv, QueueView in,
+// Materials with sheen take SHEEN variants at two waves per SIMD: the sheen
+// layer does not fit the three-wave budget of the hero kernel.
+template <int MODE, bool NRC = false, bool TEX = false, bool SHEEN = false>
+__global__ __launch_bounds__(kBlock, SHEEN ? 2 : 3) void kShade(FrameParams fp, SceneView sv, PathView pv, QueueView in,
                                                  const float4* hits, QueueView out, ShadowView sh, ShadowView she,
                                                  uint32_t* counters, int inSlot, int outSlot, NrcView nv,
                                                  const uint32_t* __restrict__ order) {
@@ -752,7 +754,7 @@ __global__ __launch_bounds__(kBlock, 3) void kShade(FrameParams fp, SceneView sv
         BMat mat = TEX ? loadSurfaceMaterial(sv, sf) : loadMaterial(sv.materials, sf.materialIndex, sf.vertexColor);
         F3 wo = toLocal(-d, sf.basis);
         const float stateLambda = MODE == kModeRgb ? 0.0f : (MODE == kModeSingle ? lam.x : (heroActive ? lam.x : misc.z));
-        BState st = makeState(mat, wo, sf.frontFace, stateLambda, MODE == kModeRgb ? 0u : 1u);
+        BState st = makeState<SHEEN>(mat, wo, sf.frontFace, stateLambda, MODE == kModeRgb ? 0u : 1u);
         const bool curNee = !refractive;
         // neural cache: training paths record every vertex; image paths stop at the query vertex
         const bool nrcTrain = NRC && q >= fp.nrcRenderPaths;
@@ -775,7 +777,7 @@ __global__ __launch_bounds__(kBlock, 3) void kShade(FrameParams fp, SceneView sv
                 F3 wl = toLocal(ls.wi, sf.basis);
                 if (!(isRefractive(mat) && wl.z <= 0.0f)) {
                     if (MODE == kModeRgb) {
-                        Eval e = evalScalar(fp.spec, st, wl);
+                        Eval e = evalScalar<SHEEN>(fp.spec, st, wl);
                         if (e.pdf > 0.0f) {
                             float mis = powerHeuristic(ls.pdfSolidAngle, e.pdf);
                             F3 tr = transmittance3(mediumSigma(), absorbing, ls.shadowDistance);
@@ -785,14 +787,14 @@ __global__ __launch_bounds__(kBlock, 3) void kShade(FrameParams fp, SceneView sv
                         }
                     } else if (heroActive) {
                         F4 btp;
-                        F4 fc = evalHero(fp.spec, st, wl, lam, btp) * fabsf(wl.z);
+                        F4 fc = evalHero<SHEEN>(fp.spec, st, wl, lam, btp) * fabsf(wl.z);
                         float mis = spectralMis(tpp * ls.pdfSolidAngle, tpp * btp);
                         if (mis > 0.0f) {
                             F4 tr = transmittance4(mediumSigma(), absorbing, ls.shadowDistance);
                             nee = tr * mis * fc * spec4(fp.spec, ls.emission, lam) / ls.pdfSolidAngle;
                         }
                     } else {
-                        Eval e = evalScalar(fp.spec, st, wl);
+                        Eval e = evalScalar<SHEEN>(fp.spec, st, wl);
                         if (e.pdf > 0.0f) {
                             float sv1 = e.value.x * fabsf(wl.z) * spec1(fp.spec, ls.emission, stateLambda);
                             float mis = powerHeuristic(ls.pdfSolidAngle, e.pdf);
@@ -814,7 +816,7 @@ __global__ __launch_bounds__(kBlock, 3) void kShade(FrameParams fp, SceneView sv
                     F3 le = environmentRadiance(sv, sd, es.wi);
                     F4 ne = f4(0.0f);
                     if (MODE == kModeRgb) {
-                        Eval e = evalScalar(fp.spec, st, wl);
+                        Eval e = evalScalar<SHEEN>(fp.spec, st, wl);
                         if (e.pdf > 0.0f) {
                             float mis = powerHeuristic(es.pdf, e.pdf);
                             F3 r = (e.value * fabsf(wl.z)) * mis * le / es.pdf;
@@ -822,11 +824,11 @@ __global__ __launch_bounds__(kBlock, 3) void kShade(FrameParams fp, SceneView sv
                         }
                     } else if (heroActive) {
                         F4 btp;
-                        F4 fc = evalHero(fp.spec, st, wl, lam, btp) * fabsf(wl.z);
+                        F4 fc = evalHero<SHEEN>(fp.spec, st, wl, lam, btp) * fabsf(wl.z);
                         float mis = spectralMis(tpp * es.pdf, tpp * btp);
                         if (mis > 0.0f) ne = fc * mis * spec4(fp.spec, le, lam) / es.pdf;
                     } else {
-                        Eval e = evalScalar(fp.spec, st, wl);
+                        Eval e = evalScalar<SHEEN>(fp.spec, st, wl);
                         if (e.pdf > 0.0f) {
                             float sv1 = e.value.x * fabsf(wl.z) * spec1(fp.spec, le, stateLambda);
                             ne.x = powerHeuristic(es.pdf, e.pdf) * sv1 / es.pdf;
@@ -955,12 +957,12 @@ __global__ __launch_bounds__(kBlock, 3) void kShade(FrameParams fp, SceneView sv
         if (wo.z <= 0.0f) break;
         F3 wl;
         uint32_t isT;
-        if (!sampleLobe(st, rng, wl, isT)) break;
+        if (!sampleLobe<SHEEN>(st, rng, wl, isT)) break;
         F3 wi = toWorld(wl, sf.basis);
         uint32_t newFlags = flags & ~(kFPrevNee | kFPendingCollapse | kFPrevEnvNee);
         if (heroActive) {
             F4 btp;
-            F4 val = evalHero(fp.spec, st, wl, lam, btp);
+            F4 val = evalHero<SHEEN>(fp.spec, st, wl, lam, btp);
             float sp = btp.x;
             if (!(sp > 0.0f)) break;
             F4 weight = val * fabsf(wl.z) / sp;
@@ -988,7 +990,7 @@ __global__ __launch_bounds__(kBlock, 3) void kShade(FrameParams fp, SceneView sv
                 }
             }
         } else {
-            Eval e = evalScalar(fp.spec, st, wl);
+            Eval e = evalScalar<SHEEN>(fp.spec, st, wl);
             if (e.pdf <= 0.0f) break;
             F3 weight = e.value * fabsf(wl.z) / e.pdf;
             if (!(e.pdf > 0.0f && anyPos(weight))) break;
@@ -1778,6 +1780,26 @@ static void launchExtend(mrt_context* ctx, bool countTraversal, const SceneView&
                            stack, salt, pathBase);
 }
 
+// kShade variant for the scene: RGB with the neural cache, textured, sheen.
+template <int MODE, bool SHEEN>
+static void launchShade(hipStream_t s, bool tex, const NrcRun* nrc, const FrameParams& fp, const SceneView& sv,
+                        const PathView& pv, QueueView in, const float4* hits, QueueView out, ShadowView shv,
+                        ShadowView shev, uint32_t* cnt, int cur, int nxt, const uint32_t* order) {
+    const dim3 grid(blocksFor(fp.count)), block(kBlock);
+    if (MODE == kModeRgb && nrc && tex)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(kShade<kModeRgb, true, true, SHEEN>), grid, block, 0, s, fp, sv, pv, in,
+                           hits, out, shv, shev, cnt, cur, nxt, nrc->view, order);
+    else if (MODE == kModeRgb && nrc)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(kShade<kModeRgb, true, false, SHEEN>), grid, block, 0, s, fp, sv, pv, in,
+                           hits, out, shv, shev, cnt, cur, nxt, nrc->view, order);
+    else if (tex)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(kShade<MODE, false, true, SHEEN>), grid, block, 0, s, fp, sv, pv, in, hits,
+                           out, shv, shev, cnt, cur, nxt, NrcView{}, order);
+    else
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(kShade<MODE, false, false, SHEEN>), grid, block, 0, s, fp, sv, pv, in,
+                           hits, out, shv, shev, cnt, cur, nxt, NrcView{}, order);
+}
+
 template <int MODE>
 static bool launchChunk(mrt_context* ctx, FrameParams fp, StageEvents& ev, bool countTraversal,
                         const NrcRun* nrc = nullptr, bool keepSampleRad = false, bool sortShading = false) {
@@ -1830,19 +1852,12 @@ static bool launchChunk(mrt_context* ctx, FrameParams fp, StageEvents& ev, bool 
         // textured materials take a separate variant so that untextured scenes
         // keep the plain kernel's register budget
         const bool tex = sv.texels != nullptr;
-        if (MODE == kModeRgb && nrc && tex)
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(kShade<kModeRgb, true, true>), dim3(blocksFor(fp.count)), dim3(kBlock),
-                               0, s, fp, sv, pv, qv[cur], b.hits.ptr, qv[nxt], shv, shev, cnt, cur, nxt, nrc->view,
-                               order);
-        else if (MODE == kModeRgb && nrc)
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(kShade<kModeRgb, true>), dim3(blocksFor(fp.count)), dim3(kBlock), 0, s,
-                               fp, sv, pv, qv[cur], b.hits.ptr, qv[nxt], shv, shev, cnt, cur, nxt, nrc->view, order);
-        else if (tex)
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(kShade<MODE, false, true>), dim3(blocksFor(fp.count)), dim3(kBlock), 0,
-                               s, fp, sv, pv, qv[cur], b.hits.ptr, qv[nxt], shv, shev, cnt, cur, nxt, NrcView{}, order);
+        if (ctx->sheenEnabled)
+            launchShade<MODE, true>(s, tex, nrc, fp, sv, pv, qv[cur], b.hits.ptr, qv[nxt], shv, shev, cnt, cur, nxt,
+                                    order);
         else
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(kShade<MODE, false>), dim3(blocksFor(fp.count)), dim3(kBlock), 0, s, fp,
-                               sv, pv, qv[cur], b.hits.ptr, qv[nxt], shv, shev, cnt, cur, nxt, NrcView{}, order);
+            launchShade<MODE, false>(s, tex, nrc, fp, sv, pv, qv[cur], b.hits.ptr, qv[nxt], shv, shev, cnt, cur, nxt,
+                                     order);
         if (timed) ev.end(s);
         if (fp.neeEnabled || fp.envNee) {  // otherwise no shadow record can exist
             hipLaunchKernelGGL(kPrepShadow, dim3(1), dim3(1), 0, s, cnt);

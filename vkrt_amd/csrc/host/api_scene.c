@@ -40,12 +40,6 @@ static Material sanitizeMaterial(const VKRT* vkrt, Material m) {
     m.specularTint = finiteClamp(m.specularTint, 0.0f, 0.0f, 1.0f);
     m.anisotropic = finiteClamp(m.anisotropic, 0.0f, 0.0f, 1.0f);
     m.sheenTintWeight[3] = finiteClamp(m.sheenTintWeight[3], 0.0f, 0.0f, 1.0f);
-    /* The sheen lobe needs the reference's LTC fit table, which this build does
-     * not carry; a sheen layer is dropped (weight 0) rather than approximated. */
-    if (m.sheenTintWeight[3] > 0.0f) {
-        VKRT_LOG_ERROR("sheen is not supported; sheen weight set to 0");
-        m.sheenTintWeight[3] = 0.0f;
-    }
     m.clearcoat = finiteClamp(m.clearcoat, 0.0f, 0.0f, 1.0f);
     m.clearcoatGloss = finiteClamp(m.clearcoatGloss, 0.0f, 0.0f, 1.0f);
     m.ior = finiteClamp(m.ior, 1.0f, 1.0f, 4.0f);

@@ -109,6 +109,25 @@ def cornell_box() -> Scene:
     return s
 
 
+def sheen_cornell() -> Scene:
+    """The Cornell box with microfiber sheen on both boxes
+    (src/shaders/bsdf/lobes/sheen.slang): a velvet-like tall box (white tint,
+    full weight, sheen roughness 0.3 over a dark red base) and a short box with
+    a tinted half-weight sheen (roughness 0.8) over a clearcoated blue base."""
+    s = cornell_box()
+    velvet = default_material(baseColor=(0.35, 0.05, 0.05), roughness=0.9, metallic=0.0, specular=0.3)
+    velvet.sheenTintWeight[:] = (1.0, 1.0, 1.0, 1.0)
+    velvet.sheenRoughness = 0.3
+    satin = default_material(baseColor=(0.1, 0.2, 0.6), roughness=0.4, metallic=0.0, specular=0.5, clearcoat=0.4,
+                             clearcoatGloss=0.7)
+    satin.sheenTintWeight[:] = (0.9, 0.8, 0.5, 0.5)
+    satin.sheenRoughness = 0.8
+    s.materials += [velvet, satin]
+    s.meshes[4].material = len(s.materials) - 2  # tall box
+    s.meshes[3].material = len(s.materials) - 1  # short box
+    return s
+
+
 def alpha_cornell() -> Scene:
     """The Cornell box with three surfaces that run the alpha test
     (src/shaders/rt/alpha_test.slang): a horizontal BLEND sheet of opacity 0.5
