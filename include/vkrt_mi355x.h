@@ -103,6 +103,16 @@ VKRT_Result VKRT_renderBackward(VKRT* vkrt, const float* dLossdImage, VKRT_Gradi
  * reads of the sorted order cost more than the divergence they remove. */
 VKRT_Result VKRT_setMaterialSorting(VKRT* vkrt, uint8_t enabled);
 
+/* Denoiser feature images. The reference always writes its albedo and normal
+ * outputs for the denoiser (src/shaders/integrator/path/loop.slang:83-111,
+ * writeback.slang:87-114); here they cost a pass per depth, so they are off
+ * until turned on (the denoiser debug views turn the pass on by themselves).
+ * Turning them on restarts accumulation. The images are RGBA32F of the render
+ * extent: the running mean of the first non-specular-follow surface's albedo
+ * and shading normal, with the accumulated feature weight in .w. */
+VKRT_Result VKRT_setDenoiserFeatures(VKRT* vkrt, uint8_t enabled);
+VKRT_Result VKRT_readDenoiserFeatures(const VKRT* vkrt, float* outAlbedo, float* outNormal, size_t floatCount);
+
 /* Neural radiance cache (an addition; the reference has no neural cache).
  * RGB render mode only. When enabled, VKRT_draw shades the first
  * prefixBounces bounces of every path as usual and then replaces the rest of

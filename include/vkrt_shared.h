@@ -64,6 +64,17 @@ typedef vkrt_f4 vkrt_m4[4]; /* column-major: m[column][row], as cglm mat4 */
 #define VKRT_DEBUG_MODE_BOUNCE_COUNT 3u
 #define VKRT_DEBUG_MODE_NEE_ONLY 4u
 #define VKRT_DEBUG_MODE_BSDF_ONLY 5u
+#define VKRT_DEBUG_MODE_SELECTION_MASK 6u
+#define VKRT_DEBUG_MODE_BASE_COLOR_MAP 7u
+#define VKRT_DEBUG_MODE_METALLIC_MAP 8u
+#define VKRT_DEBUG_MODE_ROUGHNESS_MAP 9u
+#define VKRT_DEBUG_MODE_NORMAL_MAP 10u
+#define VKRT_DEBUG_MODE_EMISSIVE_MAP 11u
+#define VKRT_DEBUG_MODE_DENOISER_ALBEDO 12u
+#define VKRT_DEBUG_MODE_DENOISER_NORMAL 13u
+#define VKRT_DEBUG_MODE_DENOISER_FEATURE_VALIDITY 14u
+#define VKRT_DEBUG_MODE_DENOISER_FEATURE_DEPTH 15u
+#define VKRT_DEBUG_MODE_DENOISER_FOLLOW_SPECULAR 16u
 #define VKRT_DEBUG_MODE_COUNT 17u
 
 #define VKRT_INVALID_INDEX 0xFFFFFFFFu

@@ -46,6 +46,9 @@ def lib():
         g = lambda n: getattr(L, n + suffix)  # noqa: E731
         g("oracle_render").restype = C.c_int
         g("oracle_render").argtypes = [P, P, C.c_uint32, C.c_uint32, rp, u16p, C.c_uint32, C.c_uint32, C.c_int, u64p]
+        g("oracle_render_ex").restype = C.c_int
+        g("oracle_render_ex").argtypes = [P, P, C.c_uint32, C.c_uint32, rp, rp, rp, u16p, C.c_uint32, C.c_uint32,
+                                          C.c_int, u64p]
         g("oracle_primary_hits").restype = C.c_int
         g("oracle_primary_hits").argtypes = [P, P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, u32p, u32p, C.c_int]
         g("oracle_trace_rays").restype = C.c_int
@@ -130,16 +133,23 @@ class OracleScene:
         except Exception:
             pass
 
-    def render(self, width, height, accum=None, precision=32, rows=None, threads=None, output=False):
+    def render(self, width, height, accum=None, precision=32, rows=None, threads=None, output=False, features=None):
+        """One frame into accum (H x W x 4, .w = samples). features, if given, is a
+        pair of H x W x 4 albedo and normal images (.w = feature weight) that the
+        frame blends into, as the reference's albedo / normal outputs."""
         dt, sfx = _dtype(precision)
         if accum is None:
             accum = np.zeros((height, width, 4), dtype=dt)
         assert accum.dtype == dt and accum.shape == (height, width, 4)
+        albedo, normal = features if features is not None else (None, None)
+        for img in (albedo, normal):
+            assert img is None or (img.dtype == dt and img.shape == (height, width, 4))
         out = np.zeros((height, width, 4), dtype=np.uint16) if output else None
         r0, r1 = rows if rows is not None else (0, height)
         bounces = C.c_uint64()
-        rc = getattr(lib(), "oracle_render" + sfx)(
-            self.addr, self.bvh, width, height, ptr(accum), ptr(out) if out is not None else None, r0, r1,
+        rc = getattr(lib(), "oracle_render_ex" + sfx)(
+            self.addr, self.bvh, width, height, ptr(accum), ptr(albedo) if albedo is not None else None,
+            ptr(normal) if normal is not None else None, ptr(out) if out is not None else None, r0, r1,
             threads or default_threads(), C.byref(bounces))
         assert rc == 0
         return (accum, out, bounces.value) if output else (accum, bounces.value)

@@ -183,6 +183,6 @@ def test_alpha_same_seed_parity(mode):
     print(f"relMSE {err:.3e}, pixels within 1e-4: {close:.5f}, bounces gpu {st.extensionRays} oracle {bounces}, "
           f"shadow rays {st.shadowRays}")
     assert st.extensionRays == bounces
-    assert err < 1e-4
-    assert close > 0.99
+    assert err < 1e-12  # measured 1.4e-14 .. 3.5e-14
+    assert close > 0.9999
     r.close()

@@ -75,5 +75,5 @@ def test_stress_same_seed_band(stress):
           f"GPU bounces (whole frame) {st.extensionRays}", flush=True)
     assert np.all(gpu[..., 3] == 16)
     assert np.isfinite(gpu).all()
-    assert err < 1e-4
-    assert close > 0.99
+    assert err < 1e-10  # measured 0
+    assert close > 0.9999

@@ -43,6 +43,23 @@ MATERIAL_TEXTURE_SLOT_BASE_COLOR = 0
 MATERIAL_TEXTURE_SLOT_METALLIC_ROUGHNESS = 1
 MATERIAL_TEXTURE_SLOT_NORMAL = 2
 MATERIAL_TEXTURE_SLOT_EMISSIVE = 3
+VKRT_DEBUG_MODE_NONE = 0
+VKRT_DEBUG_MODE_NORMALS = 1
+VKRT_DEBUG_MODE_DEPTH = 2
+VKRT_DEBUG_MODE_BOUNCE_COUNT = 3
+VKRT_DEBUG_MODE_NEE_ONLY = 4
+VKRT_DEBUG_MODE_BSDF_ONLY = 5
+VKRT_DEBUG_MODE_SELECTION_MASK = 6
+VKRT_DEBUG_MODE_BASE_COLOR_MAP = 7
+VKRT_DEBUG_MODE_METALLIC_MAP = 8
+VKRT_DEBUG_MODE_ROUGHNESS_MAP = 9
+VKRT_DEBUG_MODE_NORMAL_MAP = 10
+VKRT_DEBUG_MODE_EMISSIVE_MAP = 11
+VKRT_DEBUG_MODE_DENOISER_ALBEDO = 12
+VKRT_DEBUG_MODE_DENOISER_NORMAL = 13
+VKRT_DEBUG_MODE_DENOISER_FEATURE_VALIDITY = 14
+VKRT_DEBUG_MODE_DENOISER_FEATURE_DEPTH = 15
+VKRT_DEBUG_MODE_DENOISER_FOLLOW_SPECULAR = 16
 RENDER_PHASE_INACTIVE = 0
 RENDER_PHASE_SAMPLING = 1
 RENDER_PHASE_DENOISING = 2
@@ -610,6 +627,8 @@ def library() -> C.CDLL:
         "VKRT_getTextureSnapshot": (C.c_int, [P, u32, C.POINTER(TextureSnapshot)]),
         "VKRT_addTextureFromFile": (C.c_int, [P, C.c_char_p, C.c_char_p, u32, up]),
         "VKRT_setMaterialTexture": (C.c_int, [P, u32, u32, u32]),
+        "VKRT_setDenoiserFeatures": (C.c_int, [P, C.c_uint8]),
+        "VKRT_readDenoiserFeatures": (C.c_int, [P, fp, fp, sz]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -1020,6 +1039,20 @@ class Renderer:
 
     def synchronize(self):
         _ok(self.lib.VKRT_deviceSynchronize(self.h), "VKRT_deviceSynchronize")
+
+    def set_denoiser_features(self, enabled: bool):
+        _ok(self.lib.VKRT_setDenoiserFeatures(self.h, 1 if enabled else 0), "VKRT_setDenoiserFeatures")
+
+    def denoiser_features(self) -> tuple[np.ndarray, np.ndarray]:
+        """(albedo, normal) feature images, H x W x 4 float32 (.w = feature weight)."""
+        albedo = np.zeros((self.height, self.width, 4), dtype=np.float32)
+        normal = np.zeros_like(albedo)
+        _ok(self.lib.VKRT_readDenoiserFeatures(self.h, fptr(albedo), fptr(normal), albedo.size),
+            "VKRT_readDenoiserFeatures")
+        return albedo, normal
+
+    def set_debug_mode(self, mode: int):
+        _ok(self.lib.VKRT_setDebugMode(self.h, mode), "VKRT_setDebugMode")
 
     def set_material_sorting(self, enabled: bool):
         _ok(self.lib.VKRT_setMaterialSorting(self.h, 1 if enabled else 0), "VKRT_setMaterialSorting")

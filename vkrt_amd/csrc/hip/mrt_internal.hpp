@@ -56,6 +56,11 @@ struct PathBuffers {
     DeviceArray<uint8_t> sortKey;                         // material class of every queued ray
     DeviceArray<uint32_t> sortPerm, sortCount, sortOffset;  // shading order, per-block counts, their scan
     DeviceArray<uint8_t> sortScratch;                     // rocPRIM scan temporary storage
+    // denoiser features per path (only while feature images or debug views are on):
+    // state bit 0 = resolved, bit 1 = followed a specular hit at depth 0;
+    // albedo.w = resolve depth + 1
+    DeviceArray<uint32_t> featState;
+    DeviceArray<float4> featAlbedo, featNormal;
     size_t capacity = 0;
 };
 
@@ -117,6 +122,13 @@ struct mrt_context {
     mrt::DeviceArray<float4> accum;
     mrt::DeviceArray<ushort4> output;
     mrt::DeviceArray<float4> frameSum;
+    // denoiser feature images (accumulation extent, .w = feature weight) and the
+    // per-pixel frame records behind them and the debug views
+    mrt::DeviceArray<float4> albedoImg, normalImg;
+    mrt::DeviceArray<float4> frameFeat;  // 3 per pixel: albedo sum + weight, normal sum + depth sum, follow count
+    mrt::DeviceArray<float4> debug0;     // per pixel: primary-surface debug value of sample 0 (w = 1 once set)
+    mrt::DeviceArray<uint32_t> bounce0;  // per pixel: bounce count of sample 0
+    bool featureImagesValid = false;
     uint32_t width = 0, height = 0;
 
     // adjoint (render_backward): loss gradient image and parameter gradients

@@ -1169,6 +1169,275 @@ __global__ __launch_bounds__(kBlock) void kAccumulate(SceneData sd, uint32_t wid
     output[idx] = make_ushort4(o[0], o[1], o[2], 65535);
 }
 
+// ---------------------------------------------------- features, debug --
+// Denoiser feature images (src/shaders/integrator/path/loop.slang:83-111,
+// bsdf/principled/denoiser.slang) and the debug views (debug.slang,
+// utility/debug.slang, writeback.slang:25-58). They run as passes of their
+// own, only while feature images or a debug view are asked for, so the
+// shading kernels keep their register budget.
+struct FeatureView {
+    uint32_t* state;  // per path: bit 0 resolved, bit 1 followed a specular hit at depth 0
+    float4* albedo;   // per path: resolved albedo, w = resolve depth + 1
+    float4* normal;   // per path: resolved shading normal
+    float4* debug0;   // per pixel: primary-surface debug value of sample 0, w = 1 once set
+    uint32_t* bounce0;  // per pixel: bounce count of sample 0
+};
+
+DEV bool isPrimarySurfaceDebug(uint32_t mode) {
+    return mode == VKRT_DEBUG_MODE_NORMALS || mode == VKRT_DEBUG_MODE_DEPTH ||
+           (mode >= VKRT_DEBUG_MODE_BASE_COLOR_MAP && mode <= VKRT_DEBUG_MODE_EMISSIVE_MAP);
+}
+
+// denoiser.slang:7-22
+DEV bool denoiserFollowsSpecular(const BMat& m, uint32_t frontFace) {
+    const float eps = 1e-3f;
+    const float diffuseW = maxc(diffuseColor(m)) * (1.0f - m.transmission);
+    const float subsurfaceW = diffuseW * m.subsurface;
+    const float sheenW = maxc(m.sheenColor) * (1.0f - m.transmission);
+    const float coatW = frontFace != 0u ? m.clearcoat : 0.0f;
+    if (diffuseW > eps || subsurfaceW > eps || sheenW > eps || coatW > eps) return false;
+    if (m.transmission > eps && dielectricIdentity(m)) return true;
+    if (m.roughness > 0.02f) return false;
+    return m.transmission > eps || m.metallic >= 1.0f - eps || maxc(specularF0(m)) > eps;
+}
+
+// denoiser.slang:24-46
+DEV F3 denoiserAlbedo(const BMat& m) {
+    const float nonMetal = 1.0f - m.metallic;
+    const F3 dc = diffuseColor(m), f0 = specularF0(m);
+    const F3 metalF0 = hasConductor(m) ? metalFresnel(m, 1.0f) : sat(m.baseColor);
+    const float base = nonMetal * (1.0f - m.transmission);
+    const float diffuseW = base * (1.0f - m.subsurface), subsurfaceW = base * m.subsurface;
+    const float specularW = base * sat(lum(specularF0(m)));
+    const float transmissionW = nonMetal * m.transmission, metallicW = m.metallic;
+    const float sheenW = maxc(m.sheenColor) * base;
+    const float total = diffuseW + subsurfaceW + specularW + transmissionW + metallicW + sheenW;
+    if (total <= 0.0f) return sat(dc);
+    const F3 a = dc * diffuseW + dc * subsurfaceW + (f0 * specularW + f3(transmissionW)) +
+                 (metalF0 * metallicW + m.sheenColor * sheenW);
+    return sat(a / total);
+}
+
+// debug.slang:30-66, utility/debug.slang:13-34
+template <bool TEX>
+DEV F3 primarySurfaceDebugValue(const SceneView& sv, uint32_t mode, const Surface& sf, float t) {
+    if (mode == VKRT_DEBUG_MODE_NORMALS) return sf.basis.n * 0.5f + f3(0.5f);
+    if (mode == VKRT_DEBUG_MODE_DEPTH) return f3(1.0f / (1.0f + t));
+    const uint32_t slot = mode == VKRT_DEBUG_MODE_BASE_COLOR_MAP  ? VKRT_MATERIAL_TEXTURE_SLOT_BASE_COLOR
+                          : mode == VKRT_DEBUG_MODE_NORMAL_MAP   ? VKRT_MATERIAL_TEXTURE_SLOT_NORMAL
+                          : mode == VKRT_DEBUG_MODE_EMISSIVE_MAP ? VKRT_MATERIAL_TEXTURE_SLOT_EMISSIVE
+                                                                 : VKRT_MATERIAL_TEXTURE_SLOT_METALLIC_ROUGHNESS;
+    const float fb = slot == VKRT_MATERIAL_TEXTURE_SLOT_NORMAL ? 0.5f : 1.0f;
+    F4 v = F4{fb, fb, 1.0f, 1.0f};
+    if constexpr (TEX) v = sampleMaterialSlot(sv, sv.materials[sf.materialIndex], slot, sf.tc, v);
+    if (mode == VKRT_DEBUG_MODE_METALLIC_MAP) return f3(v.z);
+    if (mode == VKRT_DEBUG_MODE_ROUGHNESS_MAP) return f3(v.y);
+    return F3{v.x, v.y, v.z};
+}
+
+__global__ void kFeatInit(uint32_t* state, uint32_t count) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < count) state[i] = 0u;
+}
+
+// One pass per depth over the hits kShade is about to shade: the path's
+// state up to the medium test repeats kShade's, then sample 0 records the
+// primary-surface debug value, every path resolves its features at its first
+// vertex that is not a specular hit to follow, and sample 0 counts bounces.
+template <int MODE, bool TEX>
+__global__ __launch_bounds__(kBlock) void kFeatures(FrameParams fp, SceneView sv, PathView pv, QueueView in,
+                                                    const float4* hits, const uint32_t* counters, int inSlot,
+                                                    FeatureView fv) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= counters[inSlot]) return;
+    const float4 oq = in.o[i], dq = in.d[i], hh = hits[i];
+    const uint32_t q = __float_as_uint(oq.w);
+    if (fp.nrc && q >= fp.nrcRenderPaths) return;  // training paths of the neural cache
+    const Hit h{hh.x, __float_as_uint(hh.y), hh.z, hh.w};
+    if (h.tri == kNoHit) return;
+    const bool first = fp.leanStart != 0u && fp.depth == 0u;
+    const uint32_t flags = first ? (MODE == kModeHero ? kFHeroActive : 0u) : pv.flags[q];
+    const bool heroActive = MODE == kModeHero && (flags & kFHeroActive) != 0u;
+    F4 thr = first ? f4(1.0f) : ld4(pv.thr[q]);
+    F4 misc = heroActive ? f4(0.0f) : (first ? F4{0.0f, 1.0f, 0.0f, 0.0f} : ld4(pv.misc[q]));
+    if (flags & kFAbsorption) {
+        const F4 sigma = ld4(pv.sigma[q]);
+        if (MODE == kModeRgb) {
+            F3 t = transmittance3(sigma, true, h.t);
+            thr = F4{thr.x * t.x, thr.y * t.y, thr.z * t.z, thr.w};
+        } else if (MODE == kModeSingle) {
+            thr.x *= transmittance3(sigma, true, h.t).x;
+        } else if (heroActive) {
+            thr = thr * transmittance4(sigma, true, h.t);
+        } else {
+            misc.y *= transmittance3(sigma, true, h.t).x;
+        }
+    }
+    if (MODE == kModeRgb ? !anyPos(F3{thr.x, thr.y, thr.z})
+                         : (MODE == kModeSingle ? !(thr.x > 0.0f) : (heroActive ? !anyPos(thr) : !(misc.y > 0.0f))))
+        return;
+    const F3 o = F3{oq.x, oq.y, oq.z}, d = F3{dq.x, dq.y, dq.z};
+    const Surface sf = reconstruct<TEX>(sv, o, d, h);
+    const uint64_t p = fp.pathBegin + q;
+    const bool sample0 = p < fp.numPixels;
+    const uint32_t dm = fp.sd.debugMode;
+    if (sample0 && fp.depth == 0u && isPrimarySurfaceDebug(dm)) {
+        const F3 v = primarySurfaceDebugValue<TEX>(sv, dm, sf, h.t);
+        fv.debug0[p] = make_float4(v.x, v.y, v.z, 1.0f);
+        return;
+    }
+    const BMat mat = TEX ? loadSurfaceMaterial(sv, sf) : loadMaterial(sv.materials, sf.materialIndex, sf.vertexColor);
+    const uint32_t st = fv.state[q];
+    uint32_t next = st;
+    const bool follow = denoiserFollowsSpecular(mat, sf.frontFace);
+    if (fp.depth == 0u && follow) next |= 2u;
+    if (!(st & 1u) && !follow) {
+        const F3 a = denoiserAlbedo(mat);
+        fv.albedo[q] = make_float4(a.x, a.y, a.z, (float)(fp.depth + 1u));
+        fv.normal[q] = make_float4(sf.basis.n.x, sf.basis.n.y, sf.basis.n.z, 0.0f);
+        next |= 1u;
+    }
+    if (next != st) fv.state[q] = next;
+    if (sample0) fv.bounce0[p] = fp.depth + 1u;
+}
+
+// frameFeat[3k..3k+2] += the features of every sample of pixel k in the chunk
+__global__ __launch_bounds__(kBlock) void kFeatReduce(uint32_t numPixels, uint64_t pathBegin, uint32_t count,
+                                                      FeatureView fv, float4* frameFeat) {
+    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= numPixels) return;
+    const uint64_t end = pathBegin + count;
+    F4 a = ld4(frameFeat[3u * k]), n = ld4(frameFeat[3u * k + 1u]), f = ld4(frameFeat[3u * k + 2u]);
+    for (uint64_t s = pathBegin / numPixels;; s++) {
+        const uint64_t p = s * numPixels + k;
+        if (p >= end) break;
+        if (p < pathBegin) continue;
+        const uint32_t q = (uint32_t)(p - pathBegin), st = fv.state[q];
+        if (st & 1u) {
+            const F4 pa = ld4(fv.albedo[q]), pn = ld4(fv.normal[q]);
+            a = F4{a.x + pa.x, a.y + pa.y, a.z + pa.z, a.w + 1.0f};
+            n = F4{n.x + pn.x, n.y + pn.y, n.z + pn.z, n.w + pa.w};
+        }
+        if (st & 2u) f.x += 1.0f;
+    }
+    frameFeat[3u * k] = st4(a);
+    frameFeat[3u * k + 1u] = st4(n);
+    frameFeat[3u * k + 2u] = st4(f);
+}
+
+// writeback.slang:68-72
+DEV F3 blendAccumulated(F3 prev, float prevW, F3 cur, float curW) {
+    const float tw = prevW + curW;
+    if (tw <= 0.0f) return f3(0.0f);
+    return (prev * prevW + cur * curW) / tw;
+}
+
+DEV ushort4 encodeDisplay16(F3 c) {
+    c = sat(c);
+    const float e[3] = {srgbEncode(c.x), srgbEncode(c.y), srgbEncode(c.z)};
+    unsigned short o[3];
+    for (int j = 0; j < 3; j++) o[j] = (unsigned short)floorf(sat(e[j]) * 65535.0f + 0.5f);
+    return make_ushort4(o[0], o[1], o[2], 65535);
+}
+
+// kAccumulate with the feature images and the debug views
+// (src/shaders/entry/path/raygen_*.slang:17-34, writeback.slang:9-123)
+__global__ __launch_bounds__(kBlock) void kAccumulateFeatures(SceneData sd, uint32_t width, uint32_t height,
+                                                              uint32_t vw, uint32_t vh, const float4* frameSum,
+                                                              const float4* frameFeat, FeatureView fv, float4* accum,
+                                                              float4* albedoImg, float4* normalImg, ushort4* output,
+                                                              uint32_t spectralOutput, uint32_t haveFeatures) {
+    const uint32_t idx = blockIdx.x * kBlock + threadIdx.x;
+    if (idx >= width * height) return;
+    const uint32_t x = idx % width, y = idx / width;
+    const bool inside = vw > 0 && vh > 0 && x >= sd.viewportRect[0] && y >= sd.viewportRect[1] &&
+                        x < sd.viewportRect[0] + vw && y < sd.viewportRect[1] + vh;
+    if (!inside) {
+        accum[idx] = albedoImg[idx] = normalImg[idx] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        output[idx] = make_ushort4(0, 0, 0, 0);
+        return;
+    }
+    const uint32_t k = (y - sd.viewportRect[1]) * vw + (x - sd.viewportRect[0]);
+    const uint32_t spp = sd.samplesPerPixel > 1u ? sd.samplesPerPixel : 1u;
+    const uint32_t dm = sd.debugMode;
+    bool early = false;
+    F3 rad = f3(0.0f), albedo = f3(0.0f), normal = f3(0.0f);
+    float weight = 0.0f;
+    if (dm == VKRT_DEBUG_MODE_SELECTION_MASK) {
+        rad = f3(0.05f);  // debug.slang:10-19 with no mesh selected
+        early = true;
+    } else if (isPrimarySurfaceDebug(dm) && fv.debug0[k].w > 0.0f) {
+        const float4 v = fv.debug0[k];
+        rad = F3{v.x, v.y, v.z};
+        early = true;
+    } else if (dm == VKRT_DEBUG_MODE_BOUNCE_COUNT) {
+        const float maxD = sd.rrMaxDepth > 1u ? (float)sd.rrMaxDepth : 1.0f;
+        const float t = clampf((float)fv.bounce0[k] / maxD, 0.0f, 1.0f);
+        rad = F3{t, 1.0f - t, 0.0f};
+        early = true;
+    }
+    if (!early) {
+        const F4 fs = ld4(frameSum[k]);
+        rad = F3{fs.x, fs.y, fs.z} / (float)spp;
+        float depthSum = 0.0f, follow = 0.0f;
+        if (haveFeatures) {
+            const F4 a = ld4(frameFeat[3u * k]), n = ld4(frameFeat[3u * k + 1u]);
+            follow = frameFeat[3u * k + 2u].x;
+            weight = a.w;
+            depthSum = n.w;
+            if (weight > 0.0f) {
+                albedo = F3{a.x, a.y, a.z} / weight;
+                normal = F3{n.x, n.y, n.z} / weight;
+            }
+        }
+        switch (dm) {
+            case VKRT_DEBUG_MODE_DENOISER_ALBEDO: rad = albedo; early = true; break;
+            case VKRT_DEBUG_MODE_DENOISER_NORMAL:
+                rad = weight > 0.0f ? normal * 0.5f + f3(0.5f) : f3(0.0f);
+                early = true;
+                break;
+            case VKRT_DEBUG_MODE_DENOISER_FEATURE_VALIDITY: rad = f3(sat(weight / (float)spp)); early = true; break;
+            case VKRT_DEBUG_MODE_DENOISER_FEATURE_DEPTH: {
+                float dd = 0.0f;
+                if (weight > 0.0f) {
+                    const float maxD = sd.rrMaxDepth > 1u ? (float)(sd.rrMaxDepth - 1u) : 1.0f;
+                    dd = sat((depthSum / weight - 1.0f) / maxD);
+                }
+                rad = f3(dd);
+                early = true;
+                break;
+            }
+            case VKRT_DEBUG_MODE_DENOISER_FOLLOW_SPECULAR: {
+                const float t = sat(follow / (float)spp);
+                rad = F3{lerpf(0.05f, 1.0f, t), lerpf(0.05f, 0.6f, t), lerpf(0.05f, 0.0f, t)};
+                early = true;
+                break;
+            }
+            default: break;
+        }
+    }
+    if (early) {  // writeback.slang:59-66: the debug value, no sample count, no features
+        accum[idx] = make_float4(rad.x, rad.y, rad.z, 0.0f);
+        albedoImg[idx] = normalImg[idx] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        output[idx] = encodeDisplay16(rad);
+        return;
+    }
+    const float4 prev = accum[idx];
+    const float pw = (float)(uint32_t)(prev.w + 0.5f), cw = (float)spp, tw = pw + cw;
+    const F3 acc = blendAccumulated(F3{prev.x, prev.y, prev.z}, pw, rad, cw);
+    accum[idx] = make_float4(acc.x, acc.y, acc.z, tw);
+    const float4 pa = albedoImg[idx], pn = normalImg[idx];
+    const F3 a = blendAccumulated(F3{pa.x, pa.y, pa.z}, pa.w, albedo, weight);
+    const F3 n = blendAccumulated(F3{pn.x, pn.y, pn.z}, pn.w, normal, weight);
+    albedoImg[idx] = make_float4(a.x, a.y, a.z, pa.w + weight);
+    normalImg[idx] = make_float4(n.x, n.y, n.z, pn.w + weight);
+    F3 c = spectralOutput ? xyzToSrgb(acc) : acc;
+    c = maxs(c * sd.exposure, 0.0f);
+    if (VKRT_SCENE_TONE_MAPPING_MODE_FROM_RENDER_SETTINGS(sd.packedRenderSettings) == VKRT_TONE_MAPPING_MODE_ACES)
+        c = (c * (c * 2.51f + f3(0.03f))) / (c * (c * 2.43f + f3(0.59f)) + f3(0.14f));
+    output[idx] = encodeDisplay16(c);
+}
+
 __global__ void kFill4(float4* p, size_t n, float4 v) {
     size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
     if (i < n) p[i] = v;
@@ -1802,7 +2071,8 @@ static void launchShade(hipStream_t s, bool tex, const NrcRun* nrc, const FrameP
 
 template <int MODE>
 static bool launchChunk(mrt_context* ctx, FrameParams fp, StageEvents& ev, bool countTraversal,
-                        const NrcRun* nrc = nullptr, bool keepSampleRad = false, bool sortShading = false) {
+                        const NrcRun* nrc = nullptr, bool keepSampleRad = false, bool sortShading = false,
+                        const FeatureView* fv = nullptr) {
     hipStream_t s = ctx->stream;
     PathBuffers& b = ctx->paths;
     PathView pv = pathView(ctx);
@@ -1823,6 +2093,7 @@ static bool launchChunk(mrt_context* ctx, FrameParams fp, StageEvents& ev, bool 
         hipLaunchKernelGGL(HIP_KERNEL_NAME(kGenerate<MODE, true>), dim3(blocksFor(fp.count)), dim3(kBlock), 0, s, fp, pv,
                            qv[0]);
     hipLaunchKernelGGL(kSetCount, dim3(1), dim3(1), 0, s, cnt, kCntQueue0, fp.count);
+    if (fv) hipLaunchKernelGGL(kFeatInit, dim3(blocksFor(fp.count)), dim3(kBlock), 0, s, fv->state, fp.count);
     int cur = 0;
     uint32_t maxDepth = fp.sd.rrMaxDepth;
     for (uint32_t depth = 0; depth < maxDepth; depth++) {
@@ -1852,6 +2123,14 @@ static bool launchChunk(mrt_context* ctx, FrameParams fp, StageEvents& ev, bool 
         // textured materials take a separate variant so that untextured scenes
         // keep the plain kernel's register budget
         const bool tex = sv.texels != nullptr;
+        if (fv) {
+            if (tex)
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(kFeatures<MODE, true>), dim3(blocksFor(fp.count)), dim3(kBlock), 0,
+                                   s, fp, sv, pv, qv[cur], b.hits.ptr, cnt, cur, *fv);
+            else
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(kFeatures<MODE, false>), dim3(blocksFor(fp.count)), dim3(kBlock),
+                                   0, s, fp, sv, pv, qv[cur], b.hits.ptr, cnt, cur, *fv);
+        }
         if (ctx->sheenEnabled)
             launchShade<MODE, true>(s, tex, nrc, fp, sv, pv, qv[cur], b.hits.ptr, qv[nxt], shv, shev, cnt, cur, nxt,
                                     order);
@@ -1902,6 +2181,9 @@ static bool launchChunk(mrt_context* ctx, FrameParams fp, StageEvents& ev, bool 
         hipLaunchKernelGGL(kFinalizeReduce<MODE>, dim3(blocksFor(fp.numPixels)), dim3(kBlock), 0, s, fp.numPixels,
                            fp.pathBegin, imagePaths, pv, ctx->frameSum.ptr);
     }
+    if (fv)
+        hipLaunchKernelGGL(kFeatReduce, dim3(blocksFor(fp.numPixels)), dim3(kBlock), 0, s, fp.numPixels, fp.pathBegin,
+                           imagePaths, *fv, ctx->frameFeat.ptr);
     return check(hipGetLastError(), "kernel launch");
 }
 
@@ -1930,7 +2212,29 @@ extern "C" int mrt_clear_accumulation(mrt_context* ctx) {
     size_t n = (size_t)ctx->width * ctx->height;
     hipLaunchKernelGGL(kFill4, dim3(blocksFor(n)), dim3(kBlock), 0, ctx->stream, ctx->accum.ptr, n,
                        make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+    // the feature images restart with the accumulation (reallocated on the next feature frame if smaller)
+    if (ctx->albedoImg.ptr && ctx->albedoImg.capacity >= n && ctx->normalImg.ptr && ctx->normalImg.capacity >= n) {
+        hipLaunchKernelGGL(kFill4, dim3(blocksFor(n)), dim3(kBlock), 0, ctx->stream, ctx->albedoImg.ptr, n,
+                           make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+        hipLaunchKernelGGL(kFill4, dim3(blocksFor(n)), dim3(kBlock), 0, ctx->stream, ctx->normalImg.ptr, n,
+                           make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+    }
+    ctx->featureImagesValid = false;
     return check(hipGetLastError(), "clear accumulation") ? 0 : -2;
+}
+
+extern "C" int mrt_read_features(mrt_context* ctx, float* albedo, float* normal, size_t floatCount) {
+    if (!ctx || !albedo || !normal) return -1;
+    const size_t n = (size_t)ctx->width * ctx->height;
+    if (!ctx->featureImagesValid || floatCount < n * 4u) {
+        setError("no feature images (enable them before rendering) or buffer too small");
+        return -1;
+    }
+    if (!check(hipSetDevice(ctx->device), "hipSetDevice") || !check(hipStreamSynchronize(ctx->stream), "sync") ||
+        !check(hipMemcpy(albedo, ctx->albedoImg.ptr, n * sizeof(float4), hipMemcpyDeviceToHost), "read albedo") ||
+        !check(hipMemcpy(normal, ctx->normalImg.ptr, n * sizeof(float4), hipMemcpyDeviceToHost), "read normal"))
+        return -2;
+    return 0;
 }
 
 // Validation, buffers and per-frame parameters shared by the forward and the
@@ -2088,21 +2392,59 @@ extern "C" int mrt_render_frame(mrt_context* ctx, const SceneData* scene, const 
         nrcRun.batch = prm->nrcBatch;
         nrcRun.learningRate = prm->nrcLearningRate;
     }
+    // feature images and debug views: a feature pass per depth, then the
+    // accumulation kernel that writes them (off for the plain render)
+    const uint32_t dm = scene->debugMode;
+    const bool debugView = dm != VKRT_DEBUG_MODE_NONE && dm != VKRT_DEBUG_MODE_NEE_ONLY &&
+                           dm != VKRT_DEBUG_MODE_BSDF_ONLY;
+    const bool featurePass = prm->features != 0u || (debugView && dm != VKRT_DEBUG_MODE_SELECTION_MASK);
+    const size_t n = (size_t)prm->width * prm->height;
+    FeatureView fview{};
+    if (featurePass || debugView) {
+        PathBuffers& b = ctx->paths;
+        const size_t px = std::max<size_t>(fp.numPixels, 1);
+        if (!b.featState.reserve(chunk) || !b.featAlbedo.reserve(chunk) || !b.featNormal.reserve(chunk) ||
+            !ctx->frameFeat.reserve(3 * px) || !ctx->debug0.reserve(px) || !ctx->bounce0.reserve(px))
+            return -3;
+        if (!ctx->albedoImg.ptr || ctx->albedoImg.capacity < n || !ctx->normalImg.ptr || ctx->normalImg.capacity < n) {
+            if (!ctx->albedoImg.reserve(n) || !ctx->normalImg.reserve(n)) return -3;
+            hipLaunchKernelGGL(kFill4, dim3(blocksFor(n)), dim3(kBlock), 0, s, ctx->albedoImg.ptr, n,
+                               make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+            hipLaunchKernelGGL(kFill4, dim3(blocksFor(n)), dim3(kBlock), 0, s, ctx->normalImg.ptr, n,
+                               make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+        }
+        hipLaunchKernelGGL(kFill4, dim3(blocksFor(3 * px)), dim3(kBlock), 0, s, ctx->frameFeat.ptr, 3 * px,
+                           make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+        hipLaunchKernelGGL(kFill4, dim3(blocksFor(px)), dim3(kBlock), 0, s, ctx->debug0.ptr, px,
+                           make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+        hipLaunchKernelGGL(kFeatInit, dim3(blocksFor(px)), dim3(kBlock), 0, s, ctx->bounce0.ptr, (uint32_t)px);
+        fview = FeatureView{b.featState.ptr, b.featAlbedo.ptr, b.featNormal.ptr, ctx->debug0.ptr, ctx->bounce0.ptr};
+    }
+    const FeatureView* fv = featurePass ? &fview : nullptr;
     for (uint64_t begin = 0; begin < totalPaths; begin += chunk) {
         fp.pathBegin = begin;
         fp.count = (uint32_t)std::min<uint64_t>(chunk, totalPaths - begin);
         if (fp.nrc) fp.count = (uint32_t)chunk;  // image paths and training paths in one chunk
         const bool sorted = fs.sortShading;
         bool ok = mode == kModeRgb
-                      ? launchChunk<kModeRgb>(ctx, fp, ev, countTraversal, fp.nrc ? &nrcRun : nullptr, false, sorted)
-                  : mode == kModeHero ? launchChunk<kModeHero>(ctx, fp, ev, countTraversal, nullptr, false, sorted)
-                                      : launchChunk<kModeSingle>(ctx, fp, ev, countTraversal, nullptr, false, sorted);
+                      ? launchChunk<kModeRgb>(ctx, fp, ev, countTraversal, fp.nrc ? &nrcRun : nullptr, false, sorted,
+                                              fv)
+                  : mode == kModeHero ? launchChunk<kModeHero>(ctx, fp, ev, countTraversal, nullptr, false, sorted, fv)
+                                      : launchChunk<kModeSingle>(ctx, fp, ev, countTraversal, nullptr, false, sorted,
+                                                                 fv);
         if (!ok) return -4;
         chunks++;
     }
-    size_t n = (size_t)prm->width * prm->height;
-    hipLaunchKernelGGL(kAccumulate, dim3(blocksFor(n)), dim3(kBlock), 0, s, fp.sd, prm->width, prm->height, vw, vh,
-                       ctx->frameSum.ptr, ctx->accum.ptr, ctx->output.ptr, renderMode == VKRT_RENDER_MODE_SPECTRAL ? 1u : 0u);
+    const uint32_t spectralOut = renderMode == VKRT_RENDER_MODE_SPECTRAL ? 1u : 0u;
+    if (featurePass || debugView) {
+        hipLaunchKernelGGL(kAccumulateFeatures, dim3(blocksFor(n)), dim3(kBlock), 0, s, fp.sd, prm->width, prm->height,
+                           vw, vh, ctx->frameSum.ptr, ctx->frameFeat.ptr, fview, ctx->accum.ptr, ctx->albedoImg.ptr,
+                           ctx->normalImg.ptr, ctx->output.ptr, spectralOut, featurePass ? 1u : 0u);
+        ctx->featureImagesValid = true;
+    } else {
+        hipLaunchKernelGGL(kAccumulate, dim3(blocksFor(n)), dim3(kBlock), 0, s, fp.sd, prm->width, prm->height, vw,
+                           vh, ctx->frameSum.ptr, ctx->accum.ptr, ctx->output.ptr, spectralOut);
+    }
     hipEventRecord(ctx->evEnd, s);
     if (!check(hipEventSynchronize(ctx->evEnd), "frame")) return -4;
     uint32_t hostCounters[64];

@@ -228,6 +228,7 @@ static void fillRenderParams(const VKRT* vkrt, mrt_render_params* rp) {
     rp->hasTransmissive = vkrt->packed.hasTransmissive;
     rp->traversalStatistics = vkrt->traversalStatistics;
     rp->materialSort = vkrt->materialSort;
+    rp->features = vkrt->denoiserFeatures;
     if (vkrt->nrc.enabled && vkrt->settings.renderMode == VKRT_RENDER_MODE_RGB) {
         rp->nrcEnabled = 1u;
         rp->nrcPrefix = vkrt->nrc.prefixBounces;
@@ -442,6 +443,24 @@ VKRT_Result VKRT_setMaterialSorting(VKRT* vkrt, uint8_t enabled) {
     if (!vkrt) return VKRT_ERROR_INVALID_ARGUMENT;
     vkrt->materialSort = enabled ? 1u : 0u;
     return VKRT_SUCCESS;
+}
+
+/* The feature images restart with the accumulation when they are turned on. */
+VKRT_Result VKRT_setDenoiserFeatures(VKRT* vkrt, uint8_t enabled) {
+    if (!vkrt) return VKRT_ERROR_INVALID_ARGUMENT;
+    enabled = enabled ? 1u : 0u;
+    if (vkrt->denoiserFeatures == enabled) return VKRT_SUCCESS;
+    vkrt->denoiserFeatures = enabled;
+    vkrtResetSceneData(vkrt);
+    return VKRT_SUCCESS;
+}
+
+VKRT_Result VKRT_readDenoiserFeatures(const VKRT* vkrt, float* outAlbedo, float* outNormal, size_t floatCount) {
+    if (!vkrt || !outAlbedo || !outNormal) return VKRT_ERROR_INVALID_ARGUMENT;
+    if (!vkrt->device) return VKRT_ERROR_DEVICE_LOST;
+    if (floatCount != (size_t)vkrt->renderWidth * vkrt->renderHeight * 4u) return VKRT_ERROR_INVALID_ARGUMENT;
+    return mrt_read_features(vkrt->device, outAlbedo, outNormal, floatCount) == 0 ? VKRT_SUCCESS
+                                                                                    : VKRT_ERROR_OPERATION_FAILED;
 }
 
 VKRT_Result VKRT_setTraversalStatistics(VKRT* vkrt, uint8_t enabled) {

@@ -106,6 +106,7 @@ typedef struct mrt_render_params {
     float sceneMin[3]; /* scene box, for the cache's position input */
     float sceneMax[3];
     uint32_t materialSort; /* shade each depth's hits grouped by material class */
+    uint32_t features;     /* accumulate the denoiser albedo / normal feature images */
 } mrt_render_params;
 
 typedef struct mrt_render_stats {
@@ -136,6 +137,11 @@ int mrt_resize_accumulation(mrt_context* ctx, uint32_t width, uint32_t height);
 int mrt_clear_accumulation(mrt_context* ctx);
 int mrt_read_accumulation(mrt_context* ctx, float* out, size_t floatCount);
 int mrt_read_output(mrt_context* ctx, uint16_t* out, size_t count);
+/* Feature images of the accumulation extent, RGBA32F each: albedo and shading
+ * normal running means with the feature weight in .w
+ * (src/shaders/integrator/path/writeback.slang:87-114). Fails before the
+ * first frame that computed them. */
+int mrt_read_features(mrt_context* ctx, float* albedo, float* normal, size_t floatCount);
 
 /* Neural radiance cache network (nrc.hip). Inputs are 11 floats per sample in
  * [0, 1] (position 3, direction theta/pi phi/2pi, normal likewise, roughness,
