@@ -226,6 +226,22 @@ VKRT_Result VKRT_tracePrimaryHits(VKRT* vkrt, uint32_t sampleIndex, uint32_t* ou
  * 2 occluded by a transmissive material. */
 VKRT_Result VKRT_traceShadowRays(VKRT* vkrt, const float* rays, size_t count, uint32_t* outVisibility);
 
+/* VKRT_traceRays on device memory, allocating nothing. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). deviceRays, deviceMesh, devicePrimitive and the optional
+ * deviceT / deviceBarycentrics are device pointers on the context's GPU,
+ * laid out as in VKRT_traceRays; results are bit-identical to it. The
+ * traversal stacks and the overflow counter live in one fixed device array of
+ * the library (about 61 MB per GPU, held until the process ends) shared by
+ * every call on that GPU: calls that overlap on one GPU race. At most 2^30
+ * rays per call (VKRT_ERROR_INVALID_ARGUMENT beyond). The scene must already
+ * be on the device (VKRT_ERROR_OPERATION_FAILED when it changed since the
+ * last upload, as uploading allocates). Returns after the results are
+ * written. */
+VKRT_Result VKRT_traceRaysDeviceNoAlloc(VKRT* vkrt, const float* deviceRays, size_t count, uint32_t* deviceMesh,
+                                        uint32_t* devicePrimitive, float* deviceT, float* deviceBarycentrics);
+
 #ifdef __cplusplus
 }
 #endif

@@ -21,6 +21,8 @@ VKRT_INVALID_INDEX = 0xFFFFFFFF
 VKRT_ERROR_INVALID_ARGUMENT = -1
 VKRT_ERROR_OPERATION_FAILED = -2
 VKRT_ERROR_OUT_OF_MEMORY = -3
+VKRT_ERROR_DEVICE_LOST = -4
+VKRT_ERROR_INITIALIZATION_FAILED = -5
 RENDER_MODE_RGB = 0
 RENDER_MODE_SPECTRAL = 1
 SPECTRAL_SINGLE = 0
@@ -597,6 +599,7 @@ def library() -> C.CDLL:
         "VKRT_exportSceneBuffers": (C.c_int, [P, C.POINTER(SceneBuffers)]),
         "VKRT_traceRays": (C.c_int, [P, fp, sz, up, up, fp, fp]),
         "VKRT_traceShadowRays": (C.c_int, [P, fp, sz, up]),
+        "VKRT_traceRaysDeviceNoAlloc": (C.c_int, [P, P, sz, P, P, P, P]),
         "VKRT_tracePrimaryHits": (C.c_int, [P, u32, up, up]),
         "VKRT_setTraversalStatistics": (C.c_int, [P, C.c_uint8]),
         "VKRT_deviceSynchronize": (C.c_int, [P]),
@@ -1092,6 +1095,16 @@ class Renderer:
         _ok(self.lib.VKRT_traceRays(self.h, fptr(rays), n, uptr(mesh), uptr(prim), fptr(t), fptr(bary)),
             "VKRT_traceRays")
         return mesh, prim, t, bary
+
+    def trace_rays_device_noalloc(self, rays: int, count: int, mesh: int, prim: int, t: int = 0, bary: int = 0):
+        """VKRT_traceRaysDeviceNoAlloc on device addresses (integers, e.g. a
+        torch tensor's data_ptr() on the renderer's GPU): rays count x 8
+        float32, mesh / prim count uint32, optional t count float32 and bary
+        count x 2 float32 (0 leaves them out). Work queued on other streams
+        that writes the rays must be finished first; the results are written
+        when this returns."""
+        _ok(self.lib.VKRT_traceRaysDeviceNoAlloc(self.h, rays or None, count, mesh or None, prim or None, t or None,
+                                                 bary or None), "VKRT_traceRaysDeviceNoAlloc")
 
     def trace_shadow_rays(self, rays: np.ndarray):
         rays = np.ascontiguousarray(rays, dtype=np.float32)

@@ -59,6 +59,7 @@ extern "C" int mrt_create(int deviceIndex, mrt_context** out) {
         return -5;
     }
     for (auto& e : ctx->evStage) hipEventCreate(&e);
+    mrt::noAllocScratchInit(ctx);  // without it only the allocation-free query refuses to run
     *out = ctx;
     return 0;
 }

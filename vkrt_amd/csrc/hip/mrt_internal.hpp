@@ -73,6 +73,9 @@ bool nrcTrainSteps(mrt_context* ctx, const float4* in, uint32_t stride, const fl
                    uint32_t capacity, uint32_t steps, uint32_t batch, float lr);
 float nrcReadLoss(mrt_context* ctx);
 
+// allocation-free ray query (render.hip): resolves its scratch array's device address
+bool noAllocScratchInit(mrt_context* ctx);
+
 }  // namespace mrt
 
 struct mrt_context {
@@ -129,6 +132,7 @@ struct mrt_context {
     mrt::DeviceArray<float4> debug0;     // per pixel: primary-surface debug value of sample 0 (w = 1 once set)
     mrt::DeviceArray<uint32_t> bounce0;  // per pixel: bounce count of sample 0
     bool featureImagesValid = false;
+    uint32_t* noAllocScratch = nullptr;  // device address of the allocation-free query's fixed scratch array
     uint32_t width = 0, height = 0;
 
     // adjoint (render_backward): loss gradient image and parameter gradients

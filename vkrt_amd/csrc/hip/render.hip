@@ -2659,6 +2659,97 @@ static int runQuery(mrt_context* ctx, const float* rays, size_t count, uint32_t 
     return rc;
 }
 
+// ------------------------------------------------------ allocation-free --
+// The allocation-free ray query keeps everything beyond its inputs and
+// outputs in one fixed device array: a header whose word 0 counts dropped
+// traversal-stack pushes, then the global traversal stacks of a persistent
+// grid of kNoAllocBlocks blocks (kTraceBlock lanes x kGlobalStackWords words
+// each): 64 + 256 * 256 * 232 words, about 61 MB, held on every device the
+// library runs on until the process ends.
+constexpr uint32_t kNoAllocBlocks = 256;
+constexpr size_t kNoAllocHeaderWords = 64;
+constexpr size_t kNoAllocScratchWords =
+    kNoAllocHeaderWords + (size_t)kNoAllocBlocks * kTraceBlock * (size_t)kGlobalStackWords;
+static __device__ __align__(256) uint32_t scratch[kNoAllocScratchWords];
+
+// Device address of scratch on the current device; called once per context
+// at creation with its device current (the array's name is not a device
+// address in host code).
+bool mrt::noAllocScratchInit(mrt_context* ctx) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+        (void)hipGetLastError();
+        ctx->noAllocScratch = nullptr;
+        return false;
+    }
+    ctx->noAllocScratch = static_cast<uint32_t*>(p);
+    return true;
+}
+
+__global__ void kClearWords(uint32_t* p, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+// kTraceQuery's closest-hit search (same keys, same stack) writing the
+// per-mesh answer directly: mesh, primitive within the mesh, t, barycentrics.
+__global__ __launch_bounds__(kTraceBlock) void kTraceQueryDirect(SceneView sv, const float4* rays, uint32_t n,
+                                                                const uint32_t* triMesh, const uint32_t* meshTriBase,
+                                                                uint32_t* outMesh, uint32_t* outPrim, float* outT,
+                                                                float* outBary, uint32_t* scratchWords) {
+    __shared__ uint32_t lds[kLdsStackWords];
+    TraversalStack st;
+    st.lds = lds + threadIdx.x;
+    st.glob = scratchWords + kNoAllocHeaderWords + (size_t)(blockIdx.x * kTraceBlock + threadIdx.x) * kGlobalStackWords;
+    st.dropped = 0;
+    for (uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x; i < n; i += gridDim.x * kTraceBlock) {
+        const float4 a = rays[2 * (size_t)i], b = rays[2 * (size_t)i + 1];
+        const TraversalRay r = makeTraversalRay(xyz(a), xyz(b));
+        const Hit h = traceClosest(sv, r, a.w, b.w, st, alphaRayKey(alphaSalt(0u, 0ull, 0u, 3u), i));
+        const bool hit = h.tri != kNoHit && h.tri < sv.triCount;
+        const uint32_t mesh = hit ? triMesh[h.tri] : VKRT_INVALID_INDEX;
+        outMesh[i] = mesh;
+        outPrim[i] = hit ? h.tri - meshTriBase[mesh] : VKRT_INVALID_INDEX;
+        if (outT) outT[i] = hit ? h.t : 0.0f;
+        if (outBary) {
+            outBary[2 * (size_t)i] = hit ? h.u : 0.0f;
+            outBary[2 * (size_t)i + 1] = hit ? h.v : 0.0f;
+        }
+    }
+    if (st.dropped) atomicAdd(&scratchWords[0], st.dropped);
+}
+
+extern "C" int mrt_trace_rays_noalloc(mrt_context* ctx, const float* deviceRays, size_t count, uint32_t* deviceMesh,
+                                      uint32_t* devicePrim, float* deviceT, float* deviceBary) {
+    if (!ctx || !deviceRays || !deviceMesh || !devicePrim) return -1;
+    if (count > MRT_NOALLOC_MAX_RAYS) {
+        setError("allocation-free ray query: more rays than its maximum of 2^30 per call");
+        return -1;
+    }
+    if (!ctx->noAllocScratch) {
+        setError("allocation-free ray query: no device address for its scratch array");
+        return -2;
+    }
+    if (!check(hipSetDevice(ctx->device), "hipSetDevice")) return -2;
+    if (count == 0) return 0;
+    hipStream_t s = ctx->stream;
+    hipLaunchKernelGGL(kClearWords, dim3(1), dim3((uint32_t)kNoAllocHeaderWords), 0, s, ctx->noAllocScratch,
+                       (uint32_t)kNoAllocHeaderWords);
+    hipLaunchKernelGGL(kTraceQueryDirect, dim3(kNoAllocBlocks), dim3(kTraceBlock), 0, s, ctx->view(),
+                       reinterpret_cast<const float4*>(deviceRays), (uint32_t)count, ctx->triMesh.ptr,
+                       ctx->meshTriBase.ptr, deviceMesh, devicePrim, deviceT, deviceBary, ctx->noAllocScratch);
+    uint32_t dropped = 0;
+    if (!check(hipGetLastError(), "allocation-free ray query launch") ||
+        !check(hipStreamSynchronize(s), "allocation-free ray query") ||
+        !check(hipMemcpy(&dropped, ctx->noAllocScratch, sizeof(dropped), hipMemcpyDeviceToHost), "read counter"))
+        return -4;
+    if (dropped) {
+        setError("traversal stack overflow");
+        return -5;
+    }
+    return 0;
+}
+
 extern "C" int mrt_trace_rays(mrt_context* ctx, const float* rays, size_t count, uint32_t* outMesh, uint32_t* outPrim,
                               float* outT, float* outBary) {
     if (!ctx || !rays || !outMesh || !outPrim) return -1;

@@ -544,6 +544,28 @@ VKRT_Result VKRT_traceRays(VKRT* vkrt, const float* rays, size_t count, uint32_t
                                                                                                         : deviceError();
 }
 
+VKRT_Result VKRT_traceRaysDeviceNoAlloc(VKRT* vkrt, const float* deviceRays, size_t count, uint32_t* deviceMesh,
+                                        uint32_t* devicePrimitive, float* deviceT, float* deviceBarycentrics) {
+    if (!vkrt || !deviceRays || !deviceMesh || !devicePrimitive) return VKRT_ERROR_INVALID_ARGUMENT;
+    if (!vkrt->device) return VKRT_ERROR_DEVICE_LOST;
+    if (count > MRT_NOALLOC_MAX_RAYS) {
+        VKRT_LOG_ERROR("VKRT_traceRaysDeviceNoAlloc: %zu rays exceed the maximum of 2^30 per call", count);
+        return VKRT_ERROR_INVALID_ARGUMENT;
+    }
+    /* uploading a changed scene allocates: the scene must already be on the device */
+    if (vkrt->geometryUploaded != vkrt->geometryRevision || vkrt->materialUploaded != vkrt->materialRevision ||
+        vkrt->environmentUploaded != vkrt->environmentRevision || !vkrt->packed.meshInfos) {
+        VKRT_LOG_ERROR("VKRT_traceRaysDeviceNoAlloc: the scene changed since its last upload; call VKRT_traceRays "
+                       "or draw a frame first");
+        return VKRT_ERROR_OPERATION_FAILED;
+    }
+    if (count == 0) return VKRT_SUCCESS;
+    return mrt_trace_rays_noalloc(vkrt->device, deviceRays, count, deviceMesh, devicePrimitive, deviceT,
+                                  deviceBarycentrics) == 0
+               ? VKRT_SUCCESS
+               : deviceError();
+}
+
 VKRT_Result VKRT_traceShadowRays(VKRT* vkrt, const float* rays, size_t count, uint32_t* outVisibility) {
     if (!rays || !outVisibility) return VKRT_ERROR_INVALID_ARGUMENT;
     VKRT_Result r = requireTraceable(vkrt);

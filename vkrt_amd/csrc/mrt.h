@@ -181,6 +181,14 @@ int mrt_trace_shadow_rays(mrt_context* ctx, const float* rays, size_t count, uin
 int mrt_primary_hits(mrt_context* ctx, const SceneData* scene, const mrt_render_params* params, uint32_t sampleIndex,
                      uint32_t* outMesh, uint32_t* outPrim);
 
+/* mrt_trace_rays on device memory with no allocation: rays and outputs are
+ * device pointers; the traversal stacks and the overflow counter live in a
+ * fixed device array of the library. Calls that overlap on one device race
+ * on that array. At most MRT_NOALLOC_MAX_RAYS rays per call. */
+#define MRT_NOALLOC_MAX_RAYS (1ull << 30)
+int mrt_trace_rays_noalloc(mrt_context* ctx, const float* deviceRays, size_t count, uint32_t* deviceMesh,
+                           uint32_t* devicePrim, float* deviceT, float* deviceBary);
+
 #ifdef __cplusplus
 }
 #endif
