@@ -66,6 +66,42 @@ def test_packed_normals_tangents_colours_round_trip():
     r.close()
 
 
+REF_PACKING = os.path.join(os.path.dirname(GOLDEN), "..", "oracle", "_ref", "libref_packing.so")
+
+
+@pytest.mark.skipif(not os.path.exists(REF_PACKING), reason="oracle/_ref is built only where the reference is")
+def test_shader_vertices_equal_the_reference_packing_bit_for_bit():
+    """Every exported ShaderVertex equals the reference's own packShaderVertex
+    (src/core/utility/packing.c, compiled from its sources into oracle/_ref)
+    byte for byte: random, unnormalised, axis-aligned, tiny and zero normals
+    and tangents, both handedness signs and colours outside [0, 1]."""
+    ref = C.CDLL(REF_PACKING)
+    ref.packShaderVertex.restype = V.ShaderVertex
+    ref.packShaderVertex.argtypes = [C.POINTER(V.Vertex)]
+    rng = np.random.default_rng(5)
+    n = 4096
+    nrm = rng.normal(size=(n, 3)) * np.exp(rng.uniform(-7, 7, (n, 1)))
+    axes = np.concatenate([np.eye(3), -np.eye(3), [[0, 0, 0], [1e-12, 0, 0], [0, -0.0, 1e-9]]])
+    nrm[:len(axes)] = axes
+    nrm[len(axes):2 * len(axes)] = axes * 3.0 + 1e-7
+    verts = V.make_vertices(rng.normal(size=(n, 3)).astype(np.float32), nrm.astype(np.float32),
+                            rng.uniform(-0.5, 1.5, (n, 4)).astype(np.float32),
+                            rng.uniform(-3, 3, (n, 2)).astype(np.float32), rng.uniform(-3, 3, (n, 2)).astype(np.float32))
+    tan = rng.normal(size=(n, 3)) * np.exp(rng.uniform(-5, 5, (n, 1)))
+    tan[:len(axes)] = axes[::-1]
+    verts[:, 8:11] = tan
+    verts[:, 11] = rng.choice([1.0, -1.0, 0.5, -0.0, 0.0], n)
+    r, sv = exported_vertices(verts)
+    try:
+        vin = (V.Vertex * n).from_buffer_copy(np.ascontiguousarray(verts).tobytes())
+        # 44 bytes of fields; the last 4 of the 48-byte record are padding
+        diff = [i for i in range(n) if bytes(sv[i])[:44] != bytes(ref.packShaderVertex(C.byref(vin[i])))[:44]]
+        assert len(sv) == n
+        assert not diff, f"{len(diff)} vertices differ, first {diff[:5]}"
+    finally:
+        r.close()
+
+
 def transform_matrix(pos, rot, scale):
     out = (C.c_float * 16)()
     V.library().VKRT_buildMeshTransformMatrix((C.c_float * 3)(*pos), (C.c_float * 3)(*rot),
@@ -219,3 +255,53 @@ def test_differentiable_cornell_and_parameter_upload():
     assert r.scene_settings().environmentTextureIndex == 0
     A.forget(r)
     r.close()
+
+
+_INIT_PROBE = (
+    "import ctypes as C, vkrt_amd as V\n"
+    "lib = V.library(); h = C.c_void_p(); lib.VKRT_create(C.byref(h))\n"
+    "info = V.CreateInfo(); lib.VKRT_defaultCreateInfo(C.byref(info))\n"
+    "print('rc', lib.VKRT_initWithCreateInfo(h, C.byref(info)), flush=True)\n")
+
+
+def _init_with_table(path):
+    import subprocess
+    import sys
+    env = dict(os.environ, VKRT_RGB2SPEC_PATH=str(path), HIP_VISIBLE_DEVICES="")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    p = subprocess.run([sys.executable, "-c", _INIT_PROBE], cwd=root, env=env, capture_output=True, text=True,
+                       timeout=120)
+    rc = int(p.stdout.split("rc")[-1])
+    return rc, p.stderr
+
+
+def test_missing_or_malformed_rgb2spec_table_fails_initialisation(tmp_path):
+    """The table header check of src/core/scene/rgb2spec.c:17-59: "SPEC"
+    magic, res >= 2 and a payload of exactly res + 3 * res^3 * 3 floats.
+    Each malformed file (and a missing one) fails VKRT_initWithCreateInfo
+    before any device work; a well-formed res-2 table passes the check."""
+    def table(res, magic=b"SPEC", extra=0):
+        n = res + 3 * res ** 3 * 3 + extra
+        return magic + np.uint32(res).tobytes() + np.linspace(0, 1, max(n, 0), dtype=np.float32).tobytes()
+
+    cases = {
+        "bad_magic": table(2, magic=b"SPCE"),
+        "res_one": table(1),
+        "short": table(2, extra=-1),
+        "long": table(2, extra=1),
+        "channels_missing": table(2)[:8 + 4 * (2 + 2 ** 3 * 3)],
+        "header_only": b"SPEC\x02\x00\x00",
+        "empty": b"",
+    }
+    for name, data in cases.items():
+        path = tmp_path / f"{name}.coeff"
+        path.write_bytes(data)
+        rc, err = _init_with_table(path)
+        assert rc == V.VKRT_ERROR_INITIALIZATION_FAILED, name
+        assert "malformed" in err or (name == "empty" and "not found" in err), (name, err)
+    rc, err = _init_with_table(tmp_path / "absent.coeff")
+    assert rc == V.VKRT_ERROR_INITIALIZATION_FAILED and "not found" in err
+    good = tmp_path / "res2.coeff"
+    good.write_bytes(table(2))
+    rc, err = _init_with_table(good)
+    assert "malformed" not in err and "not found" not in err, err
