@@ -66,6 +66,25 @@ WORKLOADS = {
 }
 
 
+def traversal_digest() -> str:
+    """SHA-256 of the code the closest-hit kernel is built from (the traversal
+    and common device headers and kExtend's definition), recorded with the
+    committed traffic counters so that a changed kernel does not report them."""
+    import hashlib
+
+    h = hashlib.sha256()
+    hip = os.path.join(ROOT, "vkrt_amd", "csrc", "hip")
+    for name in ("dev_traverse.h", "dev_common.h"):
+        with open(os.path.join(hip, name), "rb") as f:
+            h.update(f.read())
+    with open(os.path.join(hip, "render.hip")) as f:
+        text = f.read()
+    start = text.index("// Closest-hit traversal, persistent with lane refill")
+    end = text.index("\n}\n", text.index("void kExtend(", start)) + 3
+    h.update(text[start:end].encode())
+    return h.hexdigest()
+
+
 def log(msg: str):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -452,7 +471,7 @@ def main():
     r.synchronize()
     t0 = time.perf_counter()
     bounces = shadow = paths = 0
-    ext_ms = 0.0
+    ext_ms = shade_ms = ext0_ms = shade0_ms = 0.0
     ext_launches = 0
     ext_rays = 0
     for i in range(args.steps):
@@ -462,6 +481,9 @@ def main():
         shadow += s.shadowRays
         paths += s.paths
         ext_ms += s.extendMs
+        shade_ms += s.shadeMs
+        ext0_ms += s.extendMsDepth0
+        shade0_ms += s.shadeMsDepth0
         ext_launches += s.extendLaunches
     r.synchronize()
     elapsed = time.perf_counter() - t0
@@ -497,7 +519,10 @@ def main():
     if args.workload == "spectral" and os.path.exists(pmc_path):
         with open(pmc_path) as f:
             pmc = json.load(f)
-        traffic = round(pmc["kExtend"]["hbm_bytes_per_ray"] * ext_rays / max(1, ext_launches))
+        if pmc.get("kernel_sources_sha256") == traversal_digest():
+            traffic = round(pmc["kExtend"]["hbm_bytes_per_ray"] * ext_rays / max(1, ext_launches))
+        else:
+            log("the committed traffic counters were measured on another kExtend build: traffic not reported")
     roofline = {
         "bound": "hbm",
         "achieved": round(achieved, 1),
@@ -523,6 +548,19 @@ def main():
         "unit": "GB/s",
         "extend_share_of_time": round(ext_ms * 1e-3 / elapsed, 3),
     }
+    # Every path traces one primary ray (depth 0); on this view most of them see
+    # only the sky. The split shows what the deeper, incoherent bounces cost.
+    depth_split = {
+        "depth0": {"bounces_per_step": paths // max(1, args.steps),
+                   "extend_ms_per_step": round(ext0_ms / max(1, args.steps), 3),
+                   "shade_ms_per_step": round(shade0_ms / max(1, args.steps), 3)},
+        "depth_ge1": {"bounces_per_step": (bounces - paths) // max(1, args.steps),
+                      "extend_ms_per_step": round((ext_ms - ext0_ms) / max(1, args.steps), 3),
+                      "shade_ms_per_step": round((shade_ms - shade0_ms) / max(1, args.steps), 3),
+                      "gbounces_per_s_of_kernel_time": round((bounces - paths) / max(1e-9, (ext_ms - ext0_ms + shade_ms
+                                                                                          - shade0_ms) * 1e-3) / 1e9, 4)},
+    }
+    log(f"depth 0: {depth_split['depth0']}; depth >= 1: {depth_split['depth_ge1']}")
 
     # Variant: the same workload with environment NEE (an addition of this
     # build; the reference gathers the environment only on BSDF misses).
@@ -585,6 +623,7 @@ def main():
         },
         "roofline": roofline,
         "pipeline": pipeline,
+        "depth_split": depth_split,
         "bvh": bvh,
         "variants": variants,
         "cpu_baseline": cpu,

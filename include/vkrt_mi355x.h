@@ -59,6 +59,8 @@ typedef struct VKRT_TraceStats {
     uint64_t nodeVisits;    /* closest-hit traversal only, when statistics are enabled */
     uint64_t triangleTests; /* closest-hit traversal only, when statistics are enabled */
     float backwardMs;       /* device time of the last VKRT_renderBackward (primal + replay) */
+    float extendMsDepth0;   /* the part of extendMs spent on the primary rays (depth 0) */
+    float shadeMsDepth0;    /* the part of shadeMs spent on the primary hits and misses (depth 0) */
 } VKRT_TraceStats;
 VKRT_Result VKRT_getTraceStats(const VKRT* vkrt, VKRT_TraceStats* outStats);
 

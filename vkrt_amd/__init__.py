@@ -330,6 +330,8 @@ class TraceStats(C.Structure):
         ("nodeVisits", C.c_uint64),
         ("triangleTests", C.c_uint64),
         ("backwardMs", C.c_float),
+        ("extendMsDepth0", C.c_float),
+        ("shadeMsDepth0", C.c_float),
     ]
 
 

@@ -2003,11 +2003,14 @@ static PathView pathView(mrt_context* ctx) {
 struct StageEvents {
     std::vector<hipEvent_t> pool;
     std::vector<int> stage;  // per event pair: 0 extend, 1 shade, 2 shadow, 3 cache inference, 4 cache training
+    std::vector<uint32_t> depth;  // per event pair: the path depth it belongs to
+    uint32_t depthNow = 0;
     size_t used = 0;
     bool begin(hipStream_t s, int which) {
         if (used + 2 > pool.size()) return false;
         hipEventRecord(pool[used], s);
         stage.push_back(which);
+        depth.push_back(depthNow);
         return true;
     }
     void end(hipStream_t s) {
@@ -2099,6 +2102,7 @@ static bool launchChunk(mrt_context* ctx, FrameParams fp, StageEvents& ev, bool 
     for (uint32_t depth = 0; depth < maxDepth; depth++) {
         int nxt = 1 - cur;
         fp.depth = depth;
+        ev.depthNow = depth;
         hipLaunchKernelGGL(kPrepDepth, dim3(1), dim3(64), 0, s, cnt, cur, nxt);
         bool timed = ev.begin(s, 0);
         const uint32_t pathBase = (uint32_t)fp.pathBegin;
@@ -2473,6 +2477,8 @@ extern "C" int mrt_render_frame(mrt_context* ctx, const SceneData* scene, const 
             int which = ev.stage[e / 2];
             stageMs[which] += t;
             stageN[which]++;
+            if (ev.depth[e / 2] == 0u && which == 0) stats->extendMsDepth0 += t;
+            if (ev.depth[e / 2] == 0u && which == 1) stats->shadeMsDepth0 += t;
         }
         stats->extendMs = stageMs[0];
         stats->shadeMs = stageMs[1];

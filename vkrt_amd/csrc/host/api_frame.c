@@ -265,6 +265,8 @@ VKRT_Result VKRT_trace(VKRT* vkrt) {
     vkrt->stats.extendMs = rs.extendMs;
     vkrt->stats.shadowMs = rs.shadowMs;
     vkrt->stats.shadeMs = rs.shadeMs;
+    vkrt->stats.extendMsDepth0 = rs.extendMsDepth0;
+    vkrt->stats.shadeMsDepth0 = rs.shadeMsDepth0;
     vkrt->stats.extendLaunches = rs.extendLaunches;
     vkrt->stats.shadeLaunches = rs.shadeLaunches;
     vkrt->stats.shadowLaunches = rs.shadowLaunches;

@@ -127,6 +127,8 @@ typedef struct mrt_render_stats {
     float nrcLoss;
     float nrcInferMs;
     float nrcTrainMs;
+    float extendMsDepth0;  /* the part of extendMs / shadeMs spent on the paths' first rays */
+    float shadeMsDepth0;
 } mrt_render_stats;
 
 /* One frame: samplesPerPixel paths for every viewport pixel, accumulated into
