@@ -116,7 +116,12 @@ VKRT_Result VKRT_setDenoiserFeatures(VKRT* vkrt, uint8_t enabled);
 VKRT_Result VKRT_readDenoiserFeatures(const VKRT* vkrt, float* outAlbedo, float* outNormal, size_t floatCount);
 
 /* Neural radiance cache (an addition; the reference has no neural cache).
- * RGB render mode only. When enabled, VKRT_draw shades the first
+ * RGB render mode and spectral mode with hero wavelength sampling; in the
+ * latter the cache's three outputs are CIE XYZ radiance, which an image path
+ * uplifts to its four wavelengths through the rgb2spec table (a path whose
+ * wavelengths collapsed at a dispersive surface is traced on without the
+ * cache). Single-wavelength spectral mode ignores the cache. When enabled,
+ * VKRT_draw shades the first
  * prefixBounces bounces of every path as usual and then replaces the rest of
  * the path with the cache's prediction of the radiance scattered at the next
  * surface (emission there is still added); trainingPaths extra paths per frame

@@ -138,12 +138,13 @@ def test_training_fits_a_smooth_function(renderer):
     assert mse < 0.4 * var
 
 
-def cornell(size, spp, depth, nee=True, seed=1):
+def cornell(size, spp, depth, nee=True, seed=1, mode=V.RENDER_MODE_RGB):
     from vkrt_amd import scenes
 
     r = V.Renderer(size, size)
     scenes.load(r, scenes.cornell_box())
-    r.configure(spp=spp, max_depth=depth, min_depth=4, mode=V.RENDER_MODE_RGB, rng=V.RNG_PCG32, seed=seed, nee=nee)
+    r.configure(spp=spp, max_depth=depth, min_depth=4, mode=mode, spectral=V.SPECTRAL_HERO, rng=V.RNG_PCG32,
+                seed=seed, nee=nee)
     r.start(size, size, 0)
     return r
 
@@ -227,4 +228,96 @@ def test_online_training_improves_the_image():
         errs[trained] = relmse(img, ref)
         r.close()
     print(f"relMSE of 64 frames: untrained cache {errs[False]:.4e}, trained cache {errs[True]:.4e}")
+    assert errs[True] < 0.5 * errs[False]
+
+
+# ------------------------------------------------- hero spectral mode --
+HERO = V.RENDER_MODE_SPECTRAL
+
+
+def test_zero_cache_equals_the_truncated_tracer_in_hero_mode():
+    """As in RGB: with a cache that predicts 0 and NEE off, hero spectral image
+    paths gather exactly the emission of their first prefix+1 vertices."""
+    size, spp = 48, 8
+    a = cornell(size, spp, 8, nee=False, mode=HERO)
+    a.nrc_set_weights(np.zeros(PARAMS, np.float32))
+    a.set_neural_cache(True, prefixBounces=2, trainingPaths=4096, train=0)
+    a.start(size, size, 0)
+    a.draw()
+    img = a.accumulation()
+    st = a.neural_cache_stats()
+    a.close()
+    b = cornell(size, spp, 3, nee=False, mode=HERO)
+    b.draw()
+    ref = b.accumulation()
+    b.close()
+    print(f"queries {st.queries}, training records {st.trainingRecords}")
+    assert st.queries > 0.5 * size * size * spp and st.trainingRecords > 4096
+    assert np.array_equal(img, ref)
+
+
+def test_hero_cache_output_is_uplifted_and_added_with_the_path_throughput():
+    """A cache that predicts the constant XYZ (c, c, c): each query adds its four
+    lanes' throughput times a quarter of the rgb2spec uplift of that colour's
+    linear sRGB. For c = 1 and c = 2 the brightest channel exceeds 1, where the
+    uplift scales linearly, so the two images differ from the zero-cache image
+    by amounts in the ratio 1 : 2; the difference is grey-ish, as an
+    equal-energy XYZ is."""
+    size, spp = 32, 8
+    w = np.zeros(PARAMS, np.float32)
+    w[:OFF_HIDDEN].reshape(64, 192)[0, 172] = 1.0
+    for k in range(4):
+        w[OFF_HIDDEN + k * 4096:OFF_HIDDEN + (k + 1) * 4096].reshape(64, 64)[0, 0] = 1.0
+    imgs = []
+    for c in (0.0, 1.0, 2.0):
+        w[OFF_OUT:OFF_OUT + 192].reshape(3, 64)[:, 0] = c
+        r = cornell(size, spp, 8, nee=True, mode=HERO)
+        r.nrc_set_weights(w)
+        r.set_neural_cache(True, trainingPaths=1024, train=0)
+        r.start(size, size, 0)
+        r.draw()
+        imgs.append(r.accumulation()[..., :3].astype(np.float64))
+        r.close()
+    d1, d2 = imgs[1] - imgs[0], imgs[2] - imgs[0]
+    assert d1.sum() > 0
+    assert np.allclose(d2, 2 * d1, rtol=1e-4, atol=1e-6)
+    m = d1.reshape(-1, 3).sum(0)
+    print(f"XYZ added by the unit cache: {m / m[1]}")
+    assert np.all(m > 0) and np.all(np.abs(m / m[1] - 1.0) < 0.5)
+
+
+def test_online_training_improves_the_image_in_hero_mode():
+    """Hero spectral: the cache trained online for 60 frames on XYZ targets,
+    then frozen, against the same cache untrained; 64 accumulated frames of each
+    against a 4096-spp plain spectral image (XYZ)."""
+    size = 64
+
+    def relmse(x, ref):
+        return float(np.mean((x - ref) ** 2 / (ref * ref + 1e-2)))
+
+    ref_r = cornell(size, 4096, 8, mode=HERO)
+    ref_r.draw()
+    ref = ref_r.accumulation()[..., :3].astype(np.float64)
+    ref_r.close()
+    errs = {}
+    for trained in (False, True):
+        r = cornell(size, 1, 8, mode=HERO)
+        r.nrc_reset(1)
+        if trained:
+            r.set_neural_cache(True, trainingPaths=16384, trainingSteps=4, batchSize=4096, learningRate=1e-2)
+            for _ in range(60):
+                r.start(size, size, 0)
+                r.draw()
+            st = r.neural_cache_stats()
+            print(f"last loss {st.lastLoss:.3f}, queries {st.queries}, records {st.trainingRecords}")
+            assert st.trainingRecords > 16384
+        r.set_neural_cache(True, trainingPaths=16384, train=0)
+        r.start(size, size, 0)
+        for _ in range(64):
+            r.draw()
+        img = r.accumulation()[..., :3].astype(np.float64)
+        assert np.isfinite(img).all()
+        errs[trained] = relmse(img, ref)
+        r.close()
+    print(f"hero relMSE of 64 frames: untrained cache {errs[False]:.4e}, trained cache {errs[True]:.4e}")
     assert errs[True] < 0.5 * errs[False]

@@ -67,8 +67,11 @@ struct PathBuffers {
 // neural radiance cache (nrc.hip)
 bool nrcEnsure(mrt_context* ctx, uint32_t seed);
 void nrcRelease(mrt_context* ctx);
+struct Spec;
+// spec: null for RGB queries; for hero spectral queries the rgb2spec table the
+// cached XYZ is uplifted with
 bool nrcInferQueries(mrt_context* ctx, const float4* queries, uint32_t stride, const uint32_t* count,
-                     uint32_t capacity, float4* rad);
+                     uint32_t capacity, float4* rad, const Spec* spec);
 bool nrcTrainSteps(mrt_context* ctx, const float4* in, uint32_t stride, const float4* target, const uint32_t* count,
                    uint32_t capacity, uint32_t steps, uint32_t batch, float lr);
 float nrcReadLoss(mrt_context* ctx);

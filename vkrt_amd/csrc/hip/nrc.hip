@@ -23,6 +23,7 @@
 #include <algorithm>
 #include <cmath>
 
+#include "dev_shading.h"
 #include "mrt_internal.hpp"
 
 namespace mrt {
@@ -190,9 +191,13 @@ __device__ inline float4v nrcForward(const _Float16* w, const half8 x[6], half8 
 // index bits when rad != null), in[3] the throughput when rad != null. Writes
 // raw outputs to out (when non-null) and/or adds throughput * max(output, 0) to
 // rad[path].
+// rad != null: every query adds throughput x prediction to its path's radiance.
+// spectral: the prediction is CIE XYZ; it is converted to linear sRGB, clipped
+// at 0 and uplifted with the rgb2spec table at the query's four hero
+// wavelengths (query word 4); word 3 holds the per-lane weights.
 __global__ __launch_bounds__(kNrcBlock) void kNrcInfer(const _Float16* __restrict__ wfwd, const float4* __restrict__ in,
                                                       uint32_t stride, const uint32_t* count, uint32_t nFixed,
-                                                      float4* out, float4* rad) {
+                                                      float4* out, float4* rad, Spec spec, uint32_t spectral) {
     __shared__ __align__(16) _Float16 w[kNrcParams];
     for (int k = threadIdx.x; k < kNrcParams / 8; k += kNrcBlock)
         reinterpret_cast<half8*>(w)[k] = reinterpret_cast<const half8*>(wfwd)[k];
@@ -216,9 +221,18 @@ __global__ __launch_bounds__(kNrcBlock) void kNrcInfer(const _Float16* __restric
                 const uint32_t path = __float_as_uint(rec[2].w);
                 const float4 b = rec[3];
                 float4 acc = rad[path];
-                acc.x += b.x * fmaxf(o[0], 0.0f);
-                acc.y += b.y * fmaxf(o[1], 0.0f);
-                acc.z += b.z * fmaxf(o[2], 0.0f);
+                if (spectral) {
+                    const F3 c = xyzToSrgb(F3{fmaxf(o[0], 0.0f), fmaxf(o[1], 0.0f), fmaxf(o[2], 0.0f)});
+                    const F4 sp = spec4(spec, F3{fmaxf(c.x, 0.0f), fmaxf(c.y, 0.0f), fmaxf(c.z, 0.0f)}, ld4(rec[4]));
+                    acc.x += b.x * sp.x;
+                    acc.y += b.y * sp.y;
+                    acc.z += b.z * sp.z;
+                    acc.w += b.w * sp.w;
+                } else {
+                    acc.x += b.x * fmaxf(o[0], 0.0f);
+                    acc.y += b.y * fmaxf(o[1], 0.0f);
+                    acc.z += b.z * fmaxf(o[2], 0.0f);
+                }
                 rad[path] = acc;
             }
         }
@@ -557,10 +571,10 @@ void nrcRelease(mrt_context* ctx) {
 // rad[path] += beta * cache(query) for *count queries (device counter)
 // rad[path] += throughput * cache(query) for min(*count, capacity) queries of `stride` float4
 bool nrcInferQueries(mrt_context* ctx, const float4* queries, uint32_t stride, const uint32_t* count,
-                     uint32_t capacity, float4* rad) {
+                     uint32_t capacity, float4* rad, const Spec* spec) {
     NrcState* st = nrcState(ctx);
     hipLaunchKernelGGL(kNrcInfer, dim3(st->inferGrid), dim3(kNrcBlock), 0, ctx->stream, st->wfwd.ptr, queries, stride,
-                       count, capacity, nullptr, rad);
+                       count, capacity, nullptr, rad, spec ? *spec : Spec{}, spec ? 1u : 0u);
     return check(hipGetLastError(), "nrc inference");
 }
 
@@ -640,7 +654,7 @@ extern "C" int mrt_nrc_infer(mrt_context* ctx, const float* inputs, uint32_t n, 
     }
     if (!st->scratchIn.upload(in.data(), in.size(), ctx->stream) || !st->scratchOut.reserve(n)) return -3;
     hipLaunchKernelGGL(kNrcInfer, dim3(st->inferGrid), dim3(kNrcBlock), 0, ctx->stream, st->wfwd.ptr,
-                       st->scratchIn.ptr, 3u, nullptr, n, st->scratchOut.ptr, nullptr);
+                       st->scratchIn.ptr, 3u, nullptr, n, st->scratchOut.ptr, nullptr, Spec{}, 0u);
     if (!check(hipGetLastError(), "nrc inference")) return -3;
     std::vector<float4> out(n);
     if (!check(hipMemcpyAsync(out.data(), st->scratchOut.ptr, n * sizeof(float4), hipMemcpyDeviceToHost, ctx->stream),

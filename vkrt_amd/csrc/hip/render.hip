@@ -74,8 +74,9 @@ struct FrameParams {
     uint32_t traceShadowAlways;
     uint32_t envNee;  // environment importance sampling for NEE
     Spec spec;
-    // neural radiance cache (RGB): paths [0, nrcRenderPaths) of the chunk render the
-    // image and query the cache after nrcPrefix bounces; the rest are training paths
+    // neural radiance cache (RGB, and hero spectral with the cache in CIE XYZ): paths
+    // [0, nrcRenderPaths) of the chunk render the image and query the cache after
+    // nrcPrefix bounces; the rest are training paths
     uint32_t nrc, nrcPrefix, nrcRenderPaths;
     float nrcBoxMin[3], nrcBoxInv[3];
     // kGenerate wrote only the primary rays; kShade builds each path's state at depth 0
@@ -94,7 +95,7 @@ struct NrcView {
     float4* trainTarget;
     uint32_t queryCapacity, recordCapacity;
 };
-constexpr int kNrcQueryWords = 4, kNrcRecordWords = 6;
+constexpr int kNrcQueryWords = 5, kNrcRecordWords = 7;
 
 // What one chunk does with the neural cache after its depth loop.
 struct NrcRun {
@@ -617,16 +618,34 @@ DEV void nrcInputs(const FrameParams& fp, const Surface& sf, const BMat& mat, F3
 
 // Training targets of the cache: the radiance the rest of the path carried back
 // to each recorded vertex, divided by the throughput that arrived there, less
-// the vertex's own emission. Records whose throughput is too small to divide
-// by are marked with w = -1.
+// the vertex's own emission. In RGB mode that is the target. In hero spectral
+// mode the same difference is taken per wavelength lane and the cache learns
+// its CIE XYZ (the hero estimator's own conversion of the four lanes), so every
+// target is a non-negative, unbiased estimate; records of paths whose hero
+// wavelengths collapsed later no longer hold per-lane radiance and are dropped.
+// Records whose throughput is too small to divide by are marked with w = -1.
 __global__ __launch_bounds__(kBlock) void kNrcTargets(const float4* record, const uint32_t* count, uint32_t capacity,
-                                                     const float4* rad, float4* target) {
+                                                     const float4* rad, const uint32_t* flags, uint32_t hero,
+                                                     float4* target) {
     const uint32_t r = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t n = min(*count, capacity);
     if (r >= n) return;
     const float4* rec = record + (size_t)r * kNrcRecordWords;
     const uint32_t path = __float_as_uint(rec[2].w);
     const float4 L = rad[path], b = rec[3], R = rec[4], E = rec[5];
+    if (hero) {
+        const float lc[4] = {L.x, L.y, L.z, L.w}, bc[4] = {b.x, b.y, b.z, b.w}, rc[4] = {R.x, R.y, R.z, R.w},
+                    ec[4] = {E.x, E.y, E.z, E.w};
+        float sl[4];
+        bool ok = (flags[path] & kFHeroActive) != 0u;
+        for (int c = 0; c < 4; c++) {
+            ok = ok && bc[c] > 1e-4f;
+            sl[c] = ok ? fmaxf((lc[c] - rc[c]) / bc[c] - ec[c], 0.0f) : 0.0f;
+        }
+        const F3 xyz = ok ? toXYZ4(F4{sl[0], sl[1], sl[2], sl[3]}, ld4(rec[6]), f4(470.0f)) : f3(0.0f);
+        target[r] = make_float4(fminf(xyz.x, 1000.0f), fminf(xyz.y, 1000.0f), fminf(xyz.z, 1000.0f), ok ? 0.0f : -1.0f);
+        return;
+    }
     const float lc[3] = {L.x, L.y, L.z}, bc[3] = {b.x, b.y, b.z}, rc[3] = {R.x, R.y, R.z}, ec[3] = {E.x, E.y, E.z};
     float t[3];
     bool ok = true;
@@ -757,8 +776,11 @@ __global__ __launch_bounds__(kBlock, SHEEN ? 2 : 3) void kShade(FrameParams fp, 
         BState st = makeState<SHEEN>(mat, wo, sf.frontFace, stateLambda, MODE == kModeRgb ? 0u : 1u);
         const bool curNee = !refractive;
         // neural cache: training paths record every vertex; image paths stop at the query vertex
+        // (hero spectral: only while the four wavelengths are alive; a path whose
+        // wavelengths collapsed is traced on without the cache)
+        const bool nrcLanes = MODE != kModeHero || heroActive;
         const bool nrcTrain = NRC && q >= fp.nrcRenderPaths;
-        const bool nrcQuery = NRC && !nrcTrain && fp.depth == fp.nrcPrefix;
+        const bool nrcQuery = NRC && !nrcTrain && fp.depth == fp.nrcPrefix && nrcLanes;
         Rng rng = first ? ps.rng : loadRng(pv.rng[q], fp.rngPolicy);
         F4 tpp = heroActive ? (first ? f4(1.0f) : ld4(pv.tpp[q])) : f4(0.0f);
 
@@ -876,14 +898,22 @@ __global__ __launch_bounds__(kBlock, SHEEN ? 2 : 3) void kShade(FrameParams fp, 
             }
         }
 
-        if (NRC && nrcTrain && wo.z > 0.0f) {
+        if (NRC && nrcTrain && nrcLanes && wo.z > 0.0f) {
             const uint32_t slot = waveAppend(true, &counters[kCntNrcRecord]);
             if (slot < nv.recordCapacity) {
                 float4* r = nv.record + (size_t)slot * kNrcRecordWords;
                 nrcInputs(fp, sf, mat, d, q, r);
-                r[3] = make_float4(thr.x, thr.y, thr.z, 0.0f);
-                r[4] = st4(radIn());
-                r[5] = hasEm ? make_float4(em.x, em.y, em.z, 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                if (MODE == kModeRgb) {
+                    r[3] = make_float4(thr.x, thr.y, thr.z, 0.0f);
+                    r[4] = st4(radIn());
+                    r[5] = hasEm ? make_float4(em.x, em.y, em.z, 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    r[6] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                } else {  // hero: four wavelength lanes and the wavelengths
+                    r[3] = st4(thr);
+                    r[4] = st4(radIn());
+                    r[5] = hasEm ? st4(em) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    r[6] = st4(lam);
+                }
             }
         }
 
@@ -944,7 +974,16 @@ __global__ __launch_bounds__(kBlock, SHEEN ? 2 : 3) void kShade(FrameParams fp, 
                 if (slot < nv.queryCapacity) {
                     float4* r = nv.query + (size_t)slot * kNrcQueryWords;
                     nrcInputs(fp, sf, mat, d, q, r);
-                    r[3] = make_float4(thr.x, thr.y, thr.z, 0.0f);
+                    if (MODE == kModeRgb) {
+                        r[3] = make_float4(thr.x, thr.y, thr.z, 0.0f);
+                        r[4] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    } else {
+                        // hero: the cached XYZ radiance enters like the radiance of a
+                        // source seen by all four lanes, each lane one quarter of the
+                        // estimator's four-wavelength average
+                        r[3] = st4(thr * 0.25f);
+                        r[4] = st4(lam);
+                    }
                 }
             }
             break;
@@ -2058,12 +2097,13 @@ static void launchShade(hipStream_t s, bool tex, const NrcRun* nrc, const FrameP
                         const PathView& pv, QueueView in, const float4* hits, QueueView out, ShadowView shv,
                         ShadowView shev, uint32_t* cnt, int cur, int nxt, const uint32_t* order) {
     const dim3 grid(blocksFor(fp.count)), block(kBlock);
-    if (MODE == kModeRgb && nrc && tex)
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(kShade<kModeRgb, true, true, SHEEN>), grid, block, 0, s, fp, sv, pv, in,
-                           hits, out, shv, shev, cnt, cur, nxt, nrc->view, order);
-    else if (MODE == kModeRgb && nrc)
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(kShade<kModeRgb, true, false, SHEEN>), grid, block, 0, s, fp, sv, pv, in,
-                           hits, out, shv, shev, cnt, cur, nxt, nrc->view, order);
+    constexpr int NM = MODE == kModeSingle ? kModeRgb : MODE;  // the cache runs in RGB and hero modes
+    if (MODE != kModeSingle && nrc && tex)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(kShade<NM, true, true, SHEEN>), grid, block, 0, s, fp, sv, pv, in, hits,
+                           out, shv, shev, cnt, cur, nxt, nrc->view, order);
+    else if (MODE != kModeSingle && nrc)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(kShade<NM, true, false, SHEEN>), grid, block, 0, s, fp, sv, pv, in, hits,
+                           out, shv, shev, cnt, cur, nxt, nrc->view, order);
     else if (tex)
         hipLaunchKernelGGL(HIP_KERNEL_NAME(kShade<MODE, false, true, SHEEN>), grid, block, 0, s, fp, sv, pv, in, hits,
                            out, shv, shev, cnt, cur, nxt, NrcView{}, order);
@@ -2158,18 +2198,18 @@ static bool launchChunk(mrt_context* ctx, FrameParams fp, StageEvents& ev, bool 
         cur = nxt;
     }
     uint32_t imagePaths = fp.count;
-    if (MODE == kModeRgb && nrc) {
+    if (MODE != kModeSingle && nrc) {
         // the cache's estimate of what image paths would have gathered after their query vertex
         imagePaths = fp.nrcRenderPaths;
         bool timed = ev.begin(s, 3);
         if (!nrcInferQueries(ctx, nrc->view.query, kNrcQueryWords, cnt + kCntNrcQuery, nrc->view.queryCapacity,
-                             pv.rad))
+                             pv.rad, MODE == kModeHero ? &fp.spec : nullptr))
             return false;
         if (timed) ev.end(s);
         timed = ev.begin(s, 4);
         hipLaunchKernelGGL(kNrcTargets, dim3(blocksFor(std::max(1u, nrc->view.recordCapacity))), dim3(kBlock), 0, s,
-                           nrc->view.record, cnt + kCntNrcRecord, nrc->view.recordCapacity, pv.rad,
-                           nrc->view.trainTarget);
+                           nrc->view.record, cnt + kCntNrcRecord, nrc->view.recordCapacity, pv.rad, pv.flags,
+                           MODE == kModeHero ? 1u : 0u, nrc->view.trainTarget);
         if (nrc->train &&
             !nrcTrainSteps(ctx, nrc->view.record, kNrcRecordWords, nrc->view.trainTarget, cnt + kCntNrcRecord,
                            nrc->view.recordCapacity, nrc->steps, nrc->batch, nrc->learningRate))
@@ -2289,7 +2329,7 @@ static int setupFrame(mrt_context* ctx, const SceneData* scene, const mrt_render
     uint64_t totalPaths = numPixels * spp;
     uint64_t chunk = std::min<uint64_t>(totalPaths, kMaxChunk);
     if (numPixels > 0 && numPixels <= kMaxChunk) chunk = std::min<uint64_t>(totalPaths, (kMaxChunk / numPixels) * numPixels);
-    const bool nrc = prm->nrcEnabled != 0u && mode == kModeRgb && numPixels > 0;
+    const bool nrc = prm->nrcEnabled != 0u && (mode == kModeRgb || mode == kModeHero) && numPixels > 0;
     if (nrc) {
         // one chunk: the image paths followed by the cache's training paths
         if (totalPaths + prm->nrcTrainingPaths > kMaxChunk) {
@@ -2433,7 +2473,8 @@ extern "C" int mrt_render_frame(mrt_context* ctx, const SceneData* scene, const 
         bool ok = mode == kModeRgb
                       ? launchChunk<kModeRgb>(ctx, fp, ev, countTraversal, fp.nrc ? &nrcRun : nullptr, false, sorted,
                                               fv)
-                  : mode == kModeHero ? launchChunk<kModeHero>(ctx, fp, ev, countTraversal, nullptr, false, sorted, fv)
+                  : mode == kModeHero ? launchChunk<kModeHero>(ctx, fp, ev, countTraversal, fp.nrc ? &nrcRun : nullptr,
+                                                               false, sorted, fv)
                                       : launchChunk<kModeSingle>(ctx, fp, ev, countTraversal, nullptr, false, sorted,
                                                                  fv);
         if (!ok) return -4;

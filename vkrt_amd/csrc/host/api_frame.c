@@ -229,7 +229,11 @@ static void fillRenderParams(const VKRT* vkrt, mrt_render_params* rp) {
     rp->traversalStatistics = vkrt->traversalStatistics;
     rp->materialSort = vkrt->materialSort;
     rp->features = vkrt->denoiserFeatures;
-    if (vkrt->nrc.enabled && vkrt->settings.renderMode == VKRT_RENDER_MODE_RGB) {
+    /* the cache runs in RGB and in hero spectral mode (in CIE XYZ there) */
+    const int cacheMode = vkrt->settings.renderMode == VKRT_RENDER_MODE_RGB ||
+                          (vkrt->settings.renderMode == VKRT_RENDER_MODE_SPECTRAL &&
+                           vkrt->settings.spectralSamplingMode == VKRT_SPECTRAL_SAMPLING_MODE_HERO);
+    if (vkrt->nrc.enabled && cacheMode) {
         rp->nrcEnabled = 1u;
         rp->nrcPrefix = vkrt->nrc.prefixBounces;
         rp->nrcTrainingPaths = vkrt->nrc.trainingPaths;
