@@ -444,6 +444,26 @@ extern "C" int mrt_build_bvh(mrt_context* ctx, mrt_bvh_stats* stats) {
         m = next;
     }
 
+    // the root cluster's box, padded outward by 1e-5 of its size and coordinates
+    // so that rounding in the rays' slab test can never reject a ray that
+    // reaches a triangle
+    {
+        uint32_t root = 0;
+        Aabb rb{};
+        if (!check(hipMemcpyAsync(&root, clusters[cur].ptr, sizeof(root), hipMemcpyDeviceToHost, s), "root") ||
+            !check(hipStreamSynchronize(s), "root sync") ||
+            !check(hipMemcpy(&rb, boxes.ptr + root, sizeof(rb), hipMemcpyDeviceToHost), "root box")) {
+            cleanup();
+            return -4;
+        }
+        for (int a = 0; a < 3; a++) {
+            const float ext = rb.hi[a] - rb.lo[a];
+            const bool finite = std::isfinite(rb.lo[a]) && std::isfinite(rb.hi[a]) && ext >= 0.0f;
+            ctx->sceneBoxLo[a] = finite ? rb.lo[a] - 1e-5f * (fabsf(rb.lo[a]) + ext) - 1e-30f : -INFINITY;
+            ctx->sceneBoxHi[a] = finite ? rb.hi[a] + 1e-5f * (fabsf(rb.hi[a]) + ext) + 1e-30f : INFINITY;
+        }
+    }
+
     // 5: collapse into 8-wide nodes, writing triangle records in leaf order
     hipLaunchKernelGGL(kInitCollapse, dim3(1), dim3(1), 0, s, front[0].ptr, counts.ptr, clusters[cur].ptr);
     uint32_t depth = 0;

@@ -159,6 +159,7 @@ struct SceneView {
     const float4* texels;      // material texture pool; nullptr when no material has a texture
     const uint4* texInfo;      // per texture: first texel, width, height
     uint32_t texCount;
+    float boxLo[3], boxHi[3];  // the triangles' bounding box, padded outward (infinite before a build)
 };
 
 struct Hit {

@@ -380,6 +380,19 @@ DEV bool closestStep(const SceneView& s, const TraversalRay& r, float tmin, Clos
     return true;
 }
 
+// Whether a ray can reach the scene's (padded) bounding box within
+// [tmin, tmax]. Conservative: slab distances are computed with the ray's
+// reciprocal direction, a NaN slab (an axis-parallel ray on a box plane)
+// constrains nothing, and the far distance is widened.
+DEV bool reachesSceneBox(const SceneView& s, const TraversalRay& r, float tmin, float tmax) {
+    const float ax = (s.boxLo[0] - r.o.x) * r.inv.x, bx = (s.boxHi[0] - r.o.x) * r.inv.x;
+    const float ay = (s.boxLo[1] - r.o.y) * r.inv.y, by = (s.boxHi[1] - r.o.y) * r.inv.y;
+    const float az = (s.boxLo[2] - r.o.z) * r.inv.z, bz = (s.boxHi[2] - r.o.z) * r.inv.z;
+    const float tnear = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), tmin));
+    const float tfar = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), tmax));
+    return tnear <= tfar * 1.00001f || !(tfar == tfar);
+}
+
 // Closest hit in (tmin, tmax).
 DEV Hit traceClosest(const SceneView& s, const TraversalRay& r, float tmin, float tmax, TraversalStack& st,
                      uint32_t alphaKey, uint32_t* nodeVisits = nullptr, uint32_t* triTests = nullptr) {

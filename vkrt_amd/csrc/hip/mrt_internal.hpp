@@ -135,7 +135,11 @@ struct mrt_context {
     mrt::DeviceArray<float4> debug0;     // per pixel: primary-surface debug value of sample 0 (w = 1 once set)
     mrt::DeviceArray<uint32_t> bounce0;  // per pixel: bounce count of sample 0
     bool featureImagesValid = false;
-    uint32_t* noAllocScratch = nullptr;  // device address of the allocation-free query's fixed scratch array
+    uint32_t* noAllocScratch = nullptr;
+    // bounding box of the triangles of the last BVH build, padded outward
+    // (kExtend rejects rays that miss it before touching the tree)
+    float sceneBoxLo[3] = {-INFINITY, -INFINITY, -INFINITY};
+    float sceneBoxHi[3] = {INFINITY, INFINITY, INFINITY};  // device address of the allocation-free query's fixed scratch array
     uint32_t width = 0, height = 0;
 
     // adjoint (render_backward): loss gradient image and parameter gradients
@@ -180,6 +184,10 @@ struct mrt_context {
         v.texels = texturesEnabled ? texels.ptr : nullptr;
         v.texInfo = texturesEnabled ? texInfo.ptr : nullptr;
         v.texCount = texturesEnabled ? (uint32_t)texInfo.count : 0u;
+        for (int a = 0; a < 3; a++) {
+            v.boxLo[a] = sceneBoxLo[a];
+            v.boxHi[a] = sceneBoxHi[a];
+        }
         return v;
     }
 };

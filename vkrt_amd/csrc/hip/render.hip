@@ -506,9 +506,13 @@ __global__ __launch_bounds__(kTraceBlock) void kExtend(SceneView sv, const float
                 if (base < partSize && k < end) {
                     float4 a = ro[k], b = rd[k];
                     r = makeTraversalRay(xyz(a), xyz(b));
-                    closestBegin(cs, st, b.w);
-                    if (ALPHA) alphaKey = alphaRayKey(alphaSalt, pathBase + __float_as_uint(a.w));
-                    rayId = k;
+                    if (reachesSceneBox(sv, r, kRayTMin, b.w)) {
+                        closestBegin(cs, st, b.w);
+                        if (ALPHA) alphaKey = alphaRayKey(alphaSalt, pathBase + __float_as_uint(a.w));
+                        rayId = k;
+                    } else {  // misses every triangle: the lane is free again at once
+                        hits[k] = make_float4(b.w, __uint_as_float(kNoHit), 0.0f, 0.0f);
+                    }
                 }
             }
             if (base + idle >= partSize) {  // wave-uniform: this partition is used up
