@@ -128,12 +128,15 @@ __device__ uint32_t nodeExponent(float lo, float hi) {
     return (uint32_t)e;
 }
 // Lower and upper plane codes of a child interval [clo, chi] inside the node:
-// decode(qlo) <= clo and decode(qhi) >= chi exactly.
+// the exact planes origin + q * step (step a power of two, so q * step is
+// exact and the sum is exact in double precision) satisfy lo <= clo and
+// hi >= chi, with no rounding in between: the traversal folds the decode into
+// its slab distances and never rounds a plane on its own.
 __device__ void quantizePlanes(float origin, float step, float clo, float chi, uint32_t& qlo, uint32_t& qhi) {
     float a = floorf((clo - origin) / step), b = ceilf((chi - origin) / step);
     int l = (int)fminf(fmaxf(a, 0.0f), 255.0f), h = (int)fminf(fmaxf(b, 0.0f), 255.0f);
-    while (l > 0 && fmaf((float)l, step, origin) > clo) l--;
-    while (h < 255 && fmaf((float)h, step, origin) < chi) h++;
+    while (l > 0 && (double)l * (double)step + (double)origin > (double)clo) l--;
+    while (h < 255 && (double)h * (double)step + (double)origin < (double)chi) h++;
     qlo = (uint32_t)l;
     qhi = (uint32_t)h;
 }

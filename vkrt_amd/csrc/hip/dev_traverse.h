@@ -220,14 +220,18 @@ DEV Children boxes8(const BvhNode* node, const TraversalRay& r, float tmin, floa
     const uint32_t nz0 = r.negZ ? w3.z : w2.x, nz1 = r.negZ ? w3.w : w2.y;
     const uint32_t fz0 = r.negZ ? w2.x : w3.z, fz1 = r.negZ ? w2.y : w3.w;
     c.mask = 0;
+    // The plane code q decodes to origin + q * step; its slab distance
+    // (plane - o) / d is q * (step / d) + (origin / d - o / d), one fma per
+    // plane: step / d is exact (step is a power of two) and the decoded planes
+    // contain the children without rounding (see quantizePlanes).
+    const float ax = sx * r.inv.x, ay = sy * r.inv.y, az = sz * r.inv.z;
+    const float bx = fmaf(ox, r.inv.x, -r.oInv.x), by = fmaf(oy, r.inv.y, -r.oInv.y),
+                bz = fmaf(oz, r.inv.z, -r.oInv.z);
 #pragma unroll
     for (int i = 0; i < 8; i++) {
-        float bnx = fmaf(planeByte(nx0, nx1, i), sx, ox), bfx = fmaf(planeByte(fx0, fx1, i), sx, ox);
-        float bny = fmaf(planeByte(ny0, ny1, i), sy, oy), bfy = fmaf(planeByte(fy0, fy1, i), sy, oy);
-        float bnz = fmaf(planeByte(nz0, nz1, i), sz, oz), bfz = fmaf(planeByte(fz0, fz1, i), sz, oz);
-        float t0x = fmaf(bnx, r.inv.x, -r.oInv.x), t1x = fmaf(bfx, r.inv.x, -r.oInv.x);
-        float t0y = fmaf(bny, r.inv.y, -r.oInv.y), t1y = fmaf(bfy, r.inv.y, -r.oInv.y);
-        float t0z = fmaf(bnz, r.inv.z, -r.oInv.z), t1z = fmaf(bfz, r.inv.z, -r.oInv.z);
+        float t0x = fmaf(planeByte(nx0, nx1, i), ax, bx), t1x = fmaf(planeByte(fx0, fx1, i), ax, bx);
+        float t0y = fmaf(planeByte(ny0, ny1, i), ay, by), t1y = fmaf(planeByte(fy0, fy1, i), ay, by);
+        float t0z = fmaf(planeByte(nz0, nz1, i), az, bz), t1z = fmaf(planeByte(fz0, fz1, i), az, bz);
         float tnear = fmaxf(fmaxf(t0x, t0y), fmaxf(t0z, tmin));
         float tfar = fminf(fminf(t1x, t1y), fminf(t1z, tmax)) * 1.0000005f;
         bool hit = tnear <= tfar && (uint32_t)i < count;

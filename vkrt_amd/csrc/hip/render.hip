@@ -457,7 +457,7 @@ constexpr uint32_t kNoRay = 0xFFFFFFFFu;
 // ALPHA: some mesh runs the alpha test (the opaque variant keeps six waves per SIMD);
 // TEXA: some material has textures, so the test may read a texture's alpha.
 template <bool COUNT, bool ALPHA, bool TEXA = false>
-__global__ __launch_bounds__(kTraceBlock) void kExtend(SceneView sv, const float4* ro, const float4* rd, float4* hits,
+__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(TEXA ? 4 : 6))) void kExtend(SceneView sv, const float4* ro, const float4* rd, float4* hits,
                                                       uint32_t* counters, int countSlot, uint32_t* gstack,
                                                       uint32_t alphaSalt, uint32_t pathBase) {
     uint32_t nodeVisits = 0, triTests = 0;
