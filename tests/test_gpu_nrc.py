@@ -120,6 +120,29 @@ def test_gradient_matches_reference(renderer):
     assert np.all(g[OFF_OUT + 192:] == 0.0)
 
 
+def test_gradient_of_a_batch_larger_than_one_slice(renderer):
+    """66,770 samples: the training passes run over two slices of the batch
+    (65,536 and 1,234 samples) and the second adds to the first's partials."""
+    rng = np.random.default_rng(4)
+    renderer.nrc_set_weights(rng.normal(0, 0.15, PARAMS).astype(np.float32))
+    w = renderer.nrc_weights()
+    n = 65536 + 1234
+    v = random_inputs(n, 9)
+    t = rng.uniform(0, 2, (n, 3)).astype(np.float32)
+    g, loss = renderer.nrc_gradient(v, t)
+    ref, ref_loss = gradient(w, v, t)
+    assert loss == pytest.approx(ref_loss, rel=1e-2)
+    rel = np.linalg.norm(g - ref) / np.linalg.norm(ref)
+    print(f"relative gradient error {rel:.2e}")
+    assert rel < 2e-2
+    # the second slice alone: its gradient is the difference of the two
+    g1, _ = renderer.nrc_gradient(v[:65536], t[:65536])
+    ref2, _ = gradient(w, v[65536:], t[65536:])
+    rel2 = np.linalg.norm((g - g1) - ref2) / np.linalg.norm(ref2)
+    print(f"second slice: relative gradient error {rel2:.2e}")
+    assert rel2 < 5e-2
+
+
 def test_training_fits_a_smooth_function(renderer):
     renderer.nrc_reset(7)
     n = 1 << 15

@@ -46,12 +46,16 @@ constexpr int kNrcParams = kNrcOffOut + kNrcOutRows * kNrcWidth;  // 29696
 constexpr int kNrcBwdOut = 4 * kNrcWidth * kNrcWidth;  // 16384
 constexpr int kNrcBwdParams = kNrcBwdOut + kNrcWidth * kNrcOutRows;  // 17408
 constexpr int kNrcBlock = 256;
-// LDS copy of the gradient during training, rows padded by 4 floats so that the
-// four lane groups of a fragment (rows 4 apart) fall in different banks
-constexpr int kNrcPadIn = kNrcIn + 4, kNrcPadW = kNrcWidth + 4;
-constexpr int kNrcLdsHidden = kNrcWidth * kNrcPadIn;                          // 12544
-constexpr int kNrcLdsOut = kNrcLdsHidden + 4 * kNrcWidth * kNrcPadW;          // 29952
-constexpr int kNrcLdsGrad = kNrcLdsOut + kNrcOutRows * kNrcWidth;             // 30976
+// LDS copies of the fp16 weights, rows padded by 16 bytes so that the 16 rows a
+// fragment read touches start 4 banks apart (the dense rows of 384 and 128
+// bytes would put them all on the same few banks): forward W0 rows of 200
+// halfs, W1..W4 rows of 72, W5 dense; backward W1^T..W4^T rows of 72, W5^T rows of 20.
+constexpr int kNrcLdsStrideIn = kNrcIn + 8, kNrcLdsStrideW = kNrcWidth + 8, kNrcLdsStrideOut = kNrcOutRows + 4;
+constexpr int kNrcLdsOffHidden = kNrcWidth * kNrcLdsStrideIn;                      // 12800
+constexpr int kNrcLdsOffOut = kNrcLdsOffHidden + 4 * kNrcWidth * kNrcLdsStrideW;   // 31232
+constexpr int kNrcLdsFwd = kNrcLdsOffOut + kNrcOutRows * kNrcWidth;                // 32256
+constexpr int kNrcLdsBwdOut = 4 * kNrcWidth * kNrcLdsStrideW;                      // 18432
+constexpr int kNrcLdsBwd = kNrcLdsBwdOut + kNrcWidth * kNrcLdsStrideOut;           // 19712
 
 // hidden neuron held in K-slot s of a 64-wide hidden input (see the header)
 __host__ __device__ inline int nrcNeuronOfSlot(int s) {
@@ -142,7 +146,35 @@ __device__ inline half8 nrcPack(float4v lo, float4v hi) {
                  (_Float16)hi[0], (_Float16)hi[1], (_Float16)hi[2], (_Float16)hi[3]};
 }
 
-// Forward pass of 16 samples (one per lane%16). w: forward fp16 weights (LDS or global).
+// Copies of the forward (and backward) fp16 weights from their dense global
+// layout into the padded LDS layout, by the block's threads.
+__device__ inline void nrcLoadForward(_Float16* w, const _Float16* src, int nthreads) {
+    for (int k = threadIdx.x; k < kNrcWidth * (kNrcIn / 8); k += nthreads) {
+        const int r = k / (kNrcIn / 8), c = k % (kNrcIn / 8);
+        *reinterpret_cast<half8*>(w + r * kNrcLdsStrideIn + 8 * c) =
+            *reinterpret_cast<const half8*>(src + kNrcOffW0 + r * kNrcIn + 8 * c);
+    }
+    for (int k = threadIdx.x; k < 4 * kNrcWidth * (kNrcWidth / 8); k += nthreads) {
+        const int r = k / (kNrcWidth / 8), c = k % (kNrcWidth / 8);
+        *reinterpret_cast<half8*>(w + kNrcLdsOffHidden + r * kNrcLdsStrideW + 8 * c) =
+            *reinterpret_cast<const half8*>(src + kNrcOffHidden + r * kNrcWidth + 8 * c);
+    }
+    for (int k = threadIdx.x; k < kNrcOutRows * kNrcWidth / 8; k += nthreads)
+        reinterpret_cast<half8*>(w + kNrcLdsOffOut)[k] = reinterpret_cast<const half8*>(src + kNrcOffOut)[k];
+}
+__device__ inline void nrcLoadBackward(_Float16* w, const _Float16* src, int nthreads) {
+    for (int k = threadIdx.x; k < 4 * kNrcWidth * (kNrcWidth / 8); k += nthreads) {
+        const int r = k / (kNrcWidth / 8), c = k % (kNrcWidth / 8);
+        *reinterpret_cast<half8*>(w + r * kNrcLdsStrideW + 8 * c) = *reinterpret_cast<const half8*>(src + r * kNrcWidth + 8 * c);
+    }
+    for (int k = threadIdx.x; k < kNrcWidth * (kNrcOutRows / 4); k += nthreads) {
+        const int r = k / (kNrcOutRows / 4), c = k % (kNrcOutRows / 4);
+        *reinterpret_cast<half4*>(w + kNrcLdsBwdOut + r * kNrcLdsStrideOut + 4 * c) =
+            *reinterpret_cast<const half4*>(src + kNrcBwdOut + r * kNrcOutRows + 4 * c);
+    }
+}
+
+// Forward pass of 16 samples (one per lane%16). w: the padded LDS copy of the forward weights.
 // Saves the hidden activations as B fragments when h != nullptr. Returns the output D fragment.
 template <bool SAVE>
 __device__ inline float4v nrcForward(const _Float16* w, const half8 x[6], half8 (*h)[2]) {
@@ -151,7 +183,7 @@ __device__ inline float4v nrcForward(const _Float16* w, const half8 x[6], half8 
 #pragma unroll
     for (int ib = 0; ib < 4; ib++) {
         float4v acc = {0.0f, 0.0f, 0.0f, 0.0f};
-        const _Float16* wr = w + kNrcOffW0 + (ib * 16 + row) * kNrcIn + 8 * g;
+        const _Float16* wr = w + (ib * 16 + row) * kNrcLdsStrideIn + 8 * g;
 #pragma unroll
         for (int kb = 0; kb < 6; kb++) acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ldHalf8(wr + 32 * kb), x[kb], acc, 0, 0, 0);
         d[ib] = relu4(acc);
@@ -163,11 +195,11 @@ __device__ inline float4v nrcForward(const _Float16* w, const half8 x[6], half8 
     }
 #pragma unroll
     for (int layer = 1; layer < kNrcHidden; layer++) {
-        const _Float16* wl = w + kNrcOffHidden + (layer - 1) * kNrcWidth * kNrcWidth;
+        const _Float16* wl = w + kNrcLdsOffHidden + (layer - 1) * kNrcWidth * kNrcLdsStrideW;
 #pragma unroll
         for (int ib = 0; ib < 4; ib++) {
             float4v acc = {0.0f, 0.0f, 0.0f, 0.0f};
-            const _Float16* wr = wl + (ib * 16 + row) * kNrcWidth + 8 * g;
+            const _Float16* wr = wl + (ib * 16 + row) * kNrcLdsStrideW + 8 * g;
             acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ldHalf8(wr), in[0], acc, 0, 0, 0);
             acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ldHalf8(wr + 32), in[1], acc, 0, 0, 0);
             d[ib] = relu4(acc);
@@ -180,7 +212,7 @@ __device__ inline float4v nrcForward(const _Float16* w, const half8 x[6], half8 
         }
     }
     float4v out = {0.0f, 0.0f, 0.0f, 0.0f};
-    const _Float16* wo = w + kNrcOffOut + row * kNrcWidth + 8 * g;
+    const _Float16* wo = w + kNrcLdsOffOut + row * kNrcWidth + 8 * g;
     out = __builtin_amdgcn_mfma_f32_16x16x32_f16(ldHalf8(wo), in[0], out, 0, 0, 0);
     out = __builtin_amdgcn_mfma_f32_16x16x32_f16(ldHalf8(wo + 32), in[1], out, 0, 0, 0);
     return out;  // lanes 0..15 hold outputs 0..3 of sample l (rows 0..2 used)
@@ -198,9 +230,8 @@ __device__ inline float4v nrcForward(const _Float16* w, const half8 x[6], half8 
 __global__ __launch_bounds__(kNrcBlock) void kNrcInfer(const _Float16* __restrict__ wfwd, const float4* __restrict__ in,
                                                       uint32_t stride, const uint32_t* count, uint32_t nFixed,
                                                       float4* out, float4* rad, Spec spec, uint32_t spectral) {
-    __shared__ __align__(16) _Float16 w[kNrcParams];
-    for (int k = threadIdx.x; k < kNrcParams / 8; k += kNrcBlock)
-        reinterpret_cast<half8*>(w)[k] = reinterpret_cast<const half8*>(wfwd)[k];
+    __shared__ __align__(16) _Float16 w[kNrcLdsFwd];
+    nrcLoadForward(w, wfwd, kNrcBlock);
     __syncthreads();
     const uint32_t n = count ? min(*count, nFixed) : nFixed;
     const int l = threadIdx.x & 63, g = l >> 4;
@@ -239,53 +270,80 @@ __global__ __launch_bounds__(kNrcBlock) void kNrcInfer(const _Float16* __restric
     }
 }
 
-// 16x16 table of one wave in LDS, [row][16 samples]
-__device__ inline void nrcWaitLds() {
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
+// Training runs in two passes over a batch.
+//
+// kNrcBackprop: every wave takes 16 samples, runs the forward pass keeping the
+// activations, back-propagates the relative L2 loss sum_c (y_c - t_c)^2 /
+// (y_c^2 + 0.01) (y detached in the denominator) and writes the fp16 layer
+// inputs and deltas of its samples into feature-major tables (one row of `ld`
+// halfs per feature, samples along the row):
+//   rows   0..191  encoded input x          rows 512..831  deltas d1..d5 (64 each)
+//   rows 192..511  activations h1..h5       rows 832..835  output deltas (3 used)
+// kNrcWgrad: the weight gradients dW = sum over samples of delta x input^T are
+// then plain GEMMs with the batch as the K dimension. Each wave takes one
+// output strip of one layer over one chunk of kNrcChunk samples and writes its
+// part of that chunk's partial gradient (grad + chunk * params); the optimiser
+// step sums the chunks' partials.
+constexpr int kNrcRowX = 0, kNrcRowH = kNrcIn, kNrcRowD = kNrcRowH + kNrcHidden * kNrcWidth;
+constexpr int kNrcRowOut = kNrcRowD + kNrcHidden * kNrcWidth;  // 832
+constexpr int kNrcTabRows = kNrcRowOut + 4;                    // 836
+constexpr uint32_t kNrcChunk = 512;                            // samples per partial gradient
+constexpr uint32_t kNrcSlice = 65536;                          // samples per pair of passes
+constexpr int kNrcWgradTasks = 9;  // W0 in four 48-column strips, W1..W4, W5
+
+__device__ inline void nrcStoreRows(_Float16* tab, uint32_t ld, int row0, uint32_t s, float4v v) {
+#pragma unroll
+    for (int t = 0; t < 4; t++) tab[(size_t)(row0 + t) * ld + s] = (_Float16)v[t];
 }
 
-// One optimiser step's gradient: every wave takes blocks of 16 samples of the
-// batch, runs the forward pass keeping the activations, back-propagates the
-// relative L2 loss sum_c (y_c - t_c)^2 / (y_c^2 + 0.01) (y detached in the
-// denominator) and adds the weight gradients into an LDS copy of the gradient,
-// which each block writes out as its partial gradient (grad + block * params).
-__global__ __launch_bounds__(kNrcBlock) void kNrcTrain(const _Float16* __restrict__ wfwd,
-                                                      const _Float16* __restrict__ wbwd, const float4* __restrict__ in,
-                                                      uint32_t stride, const float4* __restrict__ target,
-                                                      const uint32_t* count, uint32_t nFixed, uint32_t batch,
-                                                      uint32_t batchOffset, uint32_t shuffle, float* grad,
-                                                      float* lossSum) {
-    const uint32_t n = count ? min(*count, nFixed) : nFixed;
-    extern __shared__ __align__(16) float nrcShared[];
-    float* gl = nrcShared;                                                    // kNrcLdsGrad floats
-    _Float16* tabBase = reinterpret_cast<_Float16*>(nrcShared + kNrcLdsGrad);  // per wave: 64x16 + 192x16 halfs
-    for (int k = threadIdx.x; k < kNrcLdsGrad; k += kNrcBlock) gl[k] = 0.0f;
+// Both weight copies sit in LDS (104 KB, one block of eight waves per CU), and
+// every wave takes one group of 16 samples.
+constexpr int kNrcTrainBlock = 512;
+constexpr size_t kNrcTrainLds = (size_t)(kNrcLdsFwd + kNrcLdsBwd) * sizeof(_Float16);
+
+__global__ __launch_bounds__(kNrcTrainBlock) void kNrcBackprop(const _Float16* __restrict__ wfwd,
+                                                              const _Float16* __restrict__ wbwd,
+                                                              const float4* __restrict__ in, uint32_t stride,
+                                                              const float4* __restrict__ target, const uint32_t* count,
+                                                              uint32_t nFixed, uint32_t batch, uint32_t sliceBegin,
+                                                              uint32_t batchOffset, uint32_t shuffle, _Float16* tab,
+                                                              uint32_t ld, float* lossSum) {
+    extern __shared__ __align__(16) _Float16 nrcW[];
+    nrcLoadForward(nrcW, wfwd, kNrcTrainBlock);
+    nrcLoadBackward(nrcW + kNrcLdsFwd, wbwd, kNrcTrainBlock);
     __syncthreads();
+    const uint32_t n = count ? min(*count, nFixed) : nFixed;
     const int l = threadIdx.x & 63, row = l & 15, g = l >> 4;
-    _Float16* td = tabBase + (threadIdx.x >> 6) * (kNrcWidth * 16 + kNrcIn * 16);  // deltas [64][16]
-    _Float16* tx = td + kNrcWidth * 16;                                            // inputs [192][16]
-    const uint32_t wave = blockIdx.x * (kNrcBlock / 64) + (threadIdx.x >> 6);
-    const uint32_t waves = gridDim.x * (kNrcBlock / 64);
+    const uint32_t group = blockIdx.x * (kNrcTrainBlock / 64) + (threadIdx.x >> 6);
     float lossAcc = 0.0f;
-    for (uint32_t base = wave * 16u; base < batch; base += waves * 16u) {
-        const uint32_t bi = base + (uint32_t)row;
+    {
+        const uint32_t s = group * 16u + (uint32_t)row;  // table column
+        const uint32_t bi = sliceBegin + s;
         const bool valid = bi < batch && n > 0;
         // a pseudo-random training record for this batch slot, or record bi itself
-        const uint32_t idx = !valid ? 0u
-                                    : (shuffle ? (uint32_t)(((uint64_t)(batchOffset + bi) * 2654435761ull) % n)
-                                               : bi % n);
+        const uint32_t idx =
+            !valid ? 0u : (shuffle ? (uint32_t)(((uint64_t)(batchOffset + bi) * 2654435761ull) % n) : bi % n);
         float v[11];
         nrcLoadRaw(in, stride, idx, valid, v);
         half8 x[6];
         nrcEncode(v, g, x);
+#pragma unroll
+        for (int kb = 0; kb < 6; kb++)
+#pragma unroll
+            for (int t = 0; t < 8; t++) tab[(size_t)(kNrcRowX + 32 * kb + 8 * g + t) * ld + s] = x[kb][t];
         half8 h[kNrcHidden][2];
         const int z = nrcOpaqueZero();
-        const _Float16* wf = wfwd + z;
-        const _Float16* wb = wbwd + z;
-        float4v o = nrcForward<true>(wf, x, h);
+        float4v o = nrcForward<true>(nrcW + z, x, h);
+#pragma unroll
+        for (int layer = 0; layer < kNrcHidden; layer++)
+#pragma unroll
+            for (int kb = 0; kb < 2; kb++)
+#pragma unroll
+                for (int t = 0; t < 8; t++)
+                    tab[(size_t)(kNrcRowH + layer * kNrcWidth + nrcNeuronOfSlot(32 * kb + 8 * g + t)) * ld + s] =
+                        h[layer][kb][t];
 
-        // loss gradient, D layout (rows 0..2 of lane group 0)
+        // loss gradient, D layout (rows 0..2 of lane group 0); invalid slots get zero deltas
         float4v dOut = {0.0f, 0.0f, 0.0f, 0.0f};
         const float4 tg = valid ? target[idx] : make_float4(0.0f, 0.0f, 0.0f, -1.0f);
         if (g == 0 && tg.w >= 0.0f) {
@@ -298,67 +356,30 @@ __global__ __launch_bounds__(kNrcBlock) void kNrcTrain(const _Float16* __restric
             }
         }
         const half4 dOutH = half4{(_Float16)dOut[0], (_Float16)dOut[1], (_Float16)dOut[2], (_Float16)dOut[3]};
+        if (g == 0) nrcStoreRows(tab, ld, kNrcRowOut, s, dOut);
 
-        // dW5 = dOut x h5^T over the 16 samples
-#pragma unroll
-        for (int t = 0; t < 4; t++) td[(4 * g + t) * 16 + row] = dOutH[t];
-#pragma unroll
-        for (int kb = 0; kb < 2; kb++)
-#pragma unroll
-            for (int t = 0; t < 8; t++) tx[nrcNeuronOfSlot(32 * kb + 8 * g + t) * 16 + row] = h[4][kb][t];
-        nrcWaitLds();
-        {
-            half4 a = ldHalf4(td + row * 16 + 4 * g);
-#pragma unroll
-            for (int kc = 0; kc < 4; kc++) {
-                half4 b = ldHalf4(tx + (kc * 16 + row) * 16 + 4 * g);
-                float4v acc = __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, float4v{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
-                if (g == 0)
-#pragma unroll
-                    for (int t = 0; t < 3; t++) atomicAdd(&gl[kNrcLdsOut + t * kNrcWidth + kc * 16 + row], acc[t]);
-            }
-        }
         // delta of h5: (W5^T dOut) masked by h5 > 0, D layout in natural neuron blocks
+        const _Float16* wb = nrcW + kNrcLdsFwd + z;
         float4v dl[4];
 #pragma unroll
         for (int mb = 0; mb < 4; mb++) {
-            half4 a = ldHalf4(wb + kNrcBwdOut + (mb * 16 + row) * kNrcOutRows + 4 * g);
+            half4 a = ldHalf4(wb + kNrcLdsBwdOut + (mb * 16 + row) * kNrcLdsStrideOut + 4 * g);
             float4v acc = __builtin_amdgcn_mfma_f32_16x16x16f16(a, dOutH, float4v{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
 #pragma unroll
             for (int t = 0; t < 4; t++) acc[t] = (float)h[4][mb >> 1][(mb & 1) * 4 + t] > 0.0f ? acc[t] : 0.0f;
             dl[mb] = acc;
         }
-        // hidden layers 4..1: dW_l = delta_{l+1} x h_l^T; delta_l = (W_l^T delta_{l+1}) * (h_l > 0)
+        // delta_l = (W_l^T delta_{l+1}) * (h_l > 0), layers 4..1
 #pragma unroll
         for (int layer = kNrcHidden - 1; layer >= 1; layer--) {
-            nrcWaitLds();
 #pragma unroll
             for (int mb = 0; mb < 4; mb++)
-#pragma unroll
-                for (int t = 0; t < 4; t++) td[(mb * 16 + 4 * g + t) * 16 + row] = (_Float16)dl[mb][t];
-#pragma unroll
-            for (int kb = 0; kb < 2; kb++)
-#pragma unroll
-                for (int t = 0; t < 8; t++)
-                    tx[nrcNeuronOfSlot(32 * kb + 8 * g + t) * 16 + row] = h[layer - 1][kb][t];
-            nrcWaitLds();
-            float* gw = gl + kNrcLdsHidden + (layer - 1) * kNrcWidth * kNrcPadW;
-#pragma unroll
-            for (int ib = 0; ib < 4; ib++) {
-                half4 a = ldHalf4(td + (ib * 16 + row) * 16 + 4 * g);
-#pragma unroll
-                for (int kc = 0; kc < 4; kc++) {
-                    half4 b = ldHalf4(tx + (kc * 16 + row) * 16 + 4 * g);
-                    float4v acc = __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, float4v{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
-#pragma unroll
-                    for (int t = 0; t < 4; t++) atomicAdd(&gw[(ib * 16 + 4 * g + t) * kNrcPadW + kc * 16 + row], acc[t]);
-                }
-            }
+                nrcStoreRows(tab, ld, kNrcRowD + layer * kNrcWidth + mb * 16 + 4 * g, s, dl[mb]);
             const half8 bd[2] = {nrcPack(dl[0], dl[1]), nrcPack(dl[2], dl[3])};
-            const _Float16* wbl = wb + (layer - 1) * kNrcWidth * kNrcWidth;
+            const _Float16* wbl = wb + (layer - 1) * kNrcWidth * kNrcLdsStrideW;
 #pragma unroll
             for (int mb = 0; mb < 4; mb++) {
-                const _Float16* wr = wbl + (mb * 16 + row) * kNrcWidth + 8 * g;
+                const _Float16* wr = wbl + (mb * 16 + row) * kNrcLdsStrideW + 8 * g;
                 float4v acc = {0.0f, 0.0f, 0.0f, 0.0f};
                 acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ldHalf8(wr), bd[0], acc, 0, 0, 0);
                 acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ldHalf8(wr + 32), bd[1], acc, 0, 0, 0);
@@ -368,50 +389,76 @@ __global__ __launch_bounds__(kNrcBlock) void kNrcTrain(const _Float16* __restric
                 dl[mb] = acc;
             }
         }
-        // dW0 = delta_1 x x^T
-        nrcWaitLds();
 #pragma unroll
-        for (int mb = 0; mb < 4; mb++)
-#pragma unroll
-            for (int t = 0; t < 4; t++) td[(mb * 16 + 4 * g + t) * 16 + row] = (_Float16)dl[mb][t];
-#pragma unroll
-        for (int kb = 0; kb < 6; kb++)
-#pragma unroll
-            for (int t = 0; t < 8; t++) tx[(32 * kb + 8 * g + t) * 16 + row] = x[kb][t];
-        nrcWaitLds();
-#pragma unroll
-        for (int ib = 0; ib < 4; ib++) {
-            half4 a = ldHalf4(td + (ib * 16 + row) * 16 + 4 * g);
-            for (int kc = 0; kc < kNrcIn / 16; kc++) {
-                half4 b = ldHalf4(tx + (kc * 16 + row) * 16 + 4 * g);
-                float4v acc = __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, float4v{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
-#pragma unroll
-                for (int t = 0; t < 4; t++) atomicAdd(&gl[(ib * 16 + 4 * g + t) * kNrcPadIn + kc * 16 + row], acc[t]);
-            }
-        }
-        nrcWaitLds();
-    }
-    __syncthreads();
-    // this block's partial gradient, summed over blocks by the optimiser step
-    float* part = grad + (size_t)blockIdx.x * kNrcParams;
-    for (int k = threadIdx.x; k < kNrcParams; k += kNrcBlock) {
-        int src;
-        if (k < kNrcOffHidden) {
-            src = (k / kNrcIn) * kNrcPadIn + k % kNrcIn;
-        } else if (k < kNrcOffOut) {
-            const int r = k - kNrcOffHidden, layer = r / (kNrcWidth * kNrcWidth), e = r % (kNrcWidth * kNrcWidth);
-            src = kNrcLdsHidden + layer * kNrcWidth * kNrcPadW + (e / kNrcWidth) * kNrcPadW + e % kNrcWidth;
-        } else {
-            src = kNrcLdsOut + (k - kNrcOffOut);
-        }
-        part[k] = gl[src];
+        for (int mb = 0; mb < 4; mb++) nrcStoreRows(tab, ld, kNrcRowD + mb * 16 + 4 * g, s, dl[mb]);
     }
     // one loss sum per wave
     for (int off = 32; off > 0; off >>= 1) lossAcc += __shfl_down(lossAcc, off);
     if (l == 0 && lossAcc != 0.0f) atomicAdd(lossSum, lossAcc);
 }
 
-// Sum over the training blocks' partial gradients of parameter p, with 8
+// One strip of a weight gradient over one chunk: MB x NB tiles of 16 x 16,
+// A = delta rows arow0.. (output neurons), B = input rows brow0.. (input
+// features), K = the chunk's samples in steps of 32. ROWS3: the output layer,
+// whose delta table has three live rows (the A rows of lanes 3..15 are zero).
+template <int MB, int NB, bool ROWS3>
+__device__ inline void nrcWgradStrip(const _Float16* __restrict__ tab, uint32_t ld, uint32_t k0, int arow0, int brow0,
+                                     float* __restrict__ dst, int cols, int col0, bool accumulate) {
+    const int l = threadIdx.x & 63, row = l & 15, g = l >> 4;
+    float4v acc[MB][NB];
+#pragma unroll
+    for (int i = 0; i < MB; i++)
+#pragma unroll
+        for (int j = 0; j < NB; j++) acc[i][j] = float4v{0.0f, 0.0f, 0.0f, 0.0f};
+    const bool liveA = !ROWS3 || row < 3;
+    const _Float16* pa = tab + (size_t)(arow0 + (liveA ? row : 0)) * ld + k0 + 8 * g;
+    const _Float16* pb = tab + (size_t)(brow0 + row) * ld + k0 + 8 * g;
+#pragma unroll 2
+    for (uint32_t k = 0; k < kNrcChunk; k += 32) {
+        half8 a[MB], b[NB];
+#pragma unroll
+        for (int i = 0; i < MB; i++) a[i] = liveA ? ldHalf8(pa + (size_t)(16 * i) * ld + k) : half8{0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int j = 0; j < NB; j++) b[j] = ldHalf8(pb + (size_t)(16 * j) * ld + k);
+#pragma unroll
+        for (int i = 0; i < MB; i++)
+#pragma unroll
+            for (int j = 0; j < NB; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+    // D[4g + t][row]: weight row 16i + 4g + t, column col0 + 16j + row
+#pragma unroll
+    for (int i = 0; i < MB; i++)
+#pragma unroll
+        for (int j = 0; j < NB; j++)
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                float* p = dst + (size_t)(16 * i + 4 * g + t) * cols + col0 + 16 * j + row;
+                *p = accumulate ? *p + acc[i][j][t] : acc[i][j][t];
+            }
+}
+
+__global__ __launch_bounds__(kNrcBlock) void kNrcWgrad(const _Float16* __restrict__ tab, uint32_t ld, uint32_t chunks,
+                                                      float* grad, uint32_t accumulate) {
+    const uint32_t wave = blockIdx.x * (kNrcBlock / 64) + (threadIdx.x >> 6);
+    const uint32_t chunk = wave / kNrcWgradTasks, task = wave % kNrcWgradTasks;
+    if (chunk >= chunks) return;
+    float* part = grad + (size_t)chunk * kNrcParams;
+    const uint32_t k0 = chunk * kNrcChunk;
+    if (task < 4) {  // W0 [64][192], columns 48 task .. 48 task + 47
+        nrcWgradStrip<4, 3, false>(tab, ld, k0, kNrcRowD, kNrcRowX + 48 * (int)task, part + kNrcOffW0, kNrcIn,
+                                   48 * (int)task, accumulate != 0u);
+    } else if (task < 8) {  // W_l [64][64] = d_{l+1} x h_l^T, l = 1..4
+        const int layer = (int)task - 3;
+        nrcWgradStrip<4, 4, false>(tab, ld, k0, kNrcRowD + layer * kNrcWidth, kNrcRowH + (layer - 1) * kNrcWidth,
+                                   part + kNrcOffHidden + (layer - 1) * kNrcWidth * kNrcWidth, kNrcWidth, 0,
+                                   accumulate != 0u);
+    } else {  // W5 [16][64] = dOut x h5^T (rows 3..15 zero)
+        nrcWgradStrip<1, 4, true>(tab, ld, k0, kNrcRowOut, kNrcRowH + (kNrcHidden - 1) * kNrcWidth,
+                                  part + kNrcOffOut, kNrcWidth, 0, accumulate != 0u);
+    }
+}
+
+// Sum over the chunks' partial gradients of parameter p, with 8
 // independent loads in flight per step (a chain of single loads would wait on
 // each one in turn).
 __device__ inline float nrcSumParts(const float* grad, uint32_t parts, int p) {
@@ -425,7 +472,7 @@ __device__ inline float nrcSumParts(const float* grad, uint32_t parts, int p) {
     return ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
 }
 
-// Sum of the training blocks' partial gradients.
+// Sum of the chunks' partial gradients.
 __global__ __launch_bounds__(kNrcBlock) void kNrcSumParts(const float* grad, uint32_t parts, float* out) {
     const int p = blockIdx.x * kNrcBlock + threadIdx.x;
     if (p >= kNrcParams) return;
@@ -483,16 +530,15 @@ using namespace mrt;
 
 // ----------------------------------------------------------------- host ----
 struct NrcState {
-    DeviceArray<float> master, m1, m2, grad, gradSum, loss;  // grad: one partial per training block
+    DeviceArray<float> master, m1, m2, grad, gradSum, loss;  // grad: one partial per chunk of a batch
     DeviceArray<_Float16> wfwd, wbwd;
+    DeviceArray<_Float16> tab;  // layer inputs and deltas of one slice of a batch
     DeviceArray<float4> scratchIn, scratchTarget, scratchOut;
     uint32_t step = 0;
-    uint32_t inferGrid = 0, trainGrid = 0;
+    uint32_t inferGrid = 0;
 };
 
 namespace mrt {
-size_t nrcTrainLds() { return (size_t)kNrcLdsGrad * 4 + (size_t)(kNrcBlock / 64) * (kNrcWidth * 16 + kNrcIn * 16) * 2; }
-
 static NrcState* nrcState(mrt_context* ctx) { return reinterpret_cast<NrcState*>(ctx->nrc); }
 
 // Uniform initialisation with bound sqrt(6 / fan_in) (He et al. for ReLU
@@ -530,11 +576,9 @@ bool nrcEnsure(mrt_context* ctx, uint32_t seed) {
     if (ctx->nrc) return true;
     NrcState* st = new NrcState();
     ctx->nrc = st;
-    // one training block per CU (the LDS gradient copy fills most of a CU's LDS)
     st->inferGrid = (uint32_t)std::max(1, ctx->cuCount) * 2u;
-    st->trainGrid = (uint32_t)std::max(1, ctx->cuCount);
     bool ok = st->master.reserve(kNrcParams) && st->m1.reserve(kNrcParams) && st->m2.reserve(kNrcParams) &&
-              st->grad.reserve((size_t)st->trainGrid * kNrcParams) && st->loss.reserve(1) &&
+              st->loss.reserve(1) &&
               st->wfwd.reserve(kNrcParams) && st->wbwd.reserve(kNrcBwdParams) && st->gradSum.reserve(kNrcParams);
     if (!ok) return false;
     hipMemsetAsync(st->m1.ptr, 0, kNrcParams * sizeof(float), ctx->stream);
@@ -543,9 +587,9 @@ bool nrcEnsure(mrt_context* ctx, uint32_t seed) {
     std::vector<float> w;
     nrcInitWeights(w, seed);
     if (!nrcUploadMaster(ctx, w.data())) return false;
-    if (!check(hipFuncSetAttribute(reinterpret_cast<const void*>(kNrcTrain),
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)nrcTrainLds()),
-               "nrc train LDS"))
+    if (!check(hipFuncSetAttribute(reinterpret_cast<const void*>(kNrcBackprop),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)kNrcTrainLds),
+               "nrc training LDS"))
         return false;
     return check(hipStreamSynchronize(ctx->stream), "nrc init");
 }
@@ -558,6 +602,7 @@ void nrcRelease(mrt_context* ctx) {
     st->m2.release();
     st->grad.release();
     st->gradSum.release();
+    st->tab.release();
     st->loss.release();
     st->wfwd.release();
     st->wbwd.release();
@@ -578,6 +623,31 @@ bool nrcInferQueries(mrt_context* ctx, const float4* queries, uint32_t stride, c
     return check(hipGetLastError(), "nrc inference");
 }
 
+// The two training passes over a batch of `batch` slots (slot b takes record
+// hash(batchOffset + b) % n when shuffle is set, else b % n, of the n =
+// min(*count, capacity) records), in slices of kNrcSlice slots. Leaves the
+// gradient as partials in st->grad and returns their number (0 on failure).
+static uint32_t nrcGradientPasses(mrt_context* ctx, const float4* in, uint32_t stride, const float4* target,
+                                  const uint32_t* count, uint32_t capacity, uint32_t batch, uint32_t batchOffset,
+                                  uint32_t shuffle) {
+    NrcState* st = nrcState(ctx);
+    const uint32_t parts = (std::min(batch, kNrcSlice) + kNrcChunk - 1) / kNrcChunk;
+    if (!st->tab.reserve((size_t)kNrcTabRows * parts * kNrcChunk) || !st->grad.reserve((size_t)parts * kNrcParams))
+        return 0;
+    for (uint64_t b = 0; b < batch; b += kNrcSlice) {
+        const uint32_t len = (uint32_t)std::min<uint64_t>(kNrcSlice, batch - b);
+        const uint32_t chunks = (len + kNrcChunk - 1) / kNrcChunk, ld = chunks * kNrcChunk;
+        hipLaunchKernelGGL(kNrcBackprop, dim3(ld / (kNrcTrainBlock / 4)), dim3(kNrcTrainBlock), kNrcTrainLds, ctx->stream, st->wfwd.ptr,
+                           st->wbwd.ptr,
+                           in, stride, target, count, capacity, batch, (uint32_t)b, batchOffset, shuffle, st->tab.ptr,
+                           ld, st->loss.ptr);
+        // later slices add to the first slice's partials
+        hipLaunchKernelGGL(kNrcWgrad, dim3((chunks * kNrcWgradTasks + 3) / 4), dim3(kNrcBlock), 0, ctx->stream,
+                           st->tab.ptr, ld, chunks, st->grad.ptr, b > 0 ? 1u : 0u);
+    }
+    return check(hipGetLastError(), "nrc gradient passes") ? parts : 0u;
+}
+
 // `steps` Adam steps of `batch` records drawn from the min(*count, capacity)
 // training records (count may be null: capacity records)
 bool nrcTrainSteps(mrt_context* ctx, const float4* in, uint32_t stride, const float4* target, const uint32_t* count,
@@ -586,14 +656,13 @@ bool nrcTrainSteps(mrt_context* ctx, const float4* in, uint32_t stride, const fl
     if (capacity == 0 || steps == 0 || batch == 0) return true;
     hipMemsetAsync(st->loss.ptr, 0, sizeof(float), ctx->stream);
     for (uint32_t s = 0; s < steps; s++) {
-        hipLaunchKernelGGL(kNrcTrain, dim3(st->trainGrid), dim3(kNrcBlock), nrcTrainLds(), ctx->stream, st->wfwd.ptr,
-                           st->wbwd.ptr, in, stride, target, count, capacity, batch, st->step * batch, 1u,
-                           st->grad.ptr, st->loss.ptr);
+        const uint32_t parts = nrcGradientPasses(ctx, in, stride, target, count, capacity, batch, st->step * batch, 1u);
+        if (parts == 0) return false;
         st->step++;
         const float b1 = 0.9f, b2 = 0.99f;
         const float c1 = 1.0f - powf(b1, (float)st->step), c2 = 1.0f - powf(b2, (float)st->step);
         hipLaunchKernelGGL(kNrcAdam, dim3((kNrcParams + kNrcBlock - 1) / kNrcBlock), dim3(kNrcBlock), 0, ctx->stream,
-                           st->master.ptr, st->m1.ptr, st->m2.ptr, st->grad.ptr, st->trainGrid, st->wfwd.ptr,
+                           st->master.ptr, st->m1.ptr, st->m2.ptr, st->grad.ptr, parts, st->wfwd.ptr,
                            st->wbwd.ptr,
                            1.0f / (float)batch, lr, b1, b2, c1, c2, 1e-8f);
     }
@@ -711,11 +780,10 @@ extern "C" int mrt_nrc_gradient(mrt_context* ctx, const float* inputs, const flo
         !st->scratchTarget.upload(tg.data(), tg.size(), ctx->stream))
         return -3;
     hipMemsetAsync(st->loss.ptr, 0, sizeof(float), ctx->stream);
-    hipLaunchKernelGGL(kNrcTrain, dim3(st->trainGrid), dim3(kNrcBlock), nrcTrainLds(), ctx->stream, st->wfwd.ptr,
-                       st->wbwd.ptr, st->scratchIn.ptr, 3u, st->scratchTarget.ptr, nullptr, n, n, 0u, 0u,
-                       st->grad.ptr, st->loss.ptr);
+    const uint32_t parts = nrcGradientPasses(ctx, st->scratchIn.ptr, 3u, st->scratchTarget.ptr, nullptr, n, n, 0u, 0u);
+    if (parts == 0) return -3;
     hipLaunchKernelGGL(kNrcSumParts, dim3((kNrcParams + kNrcBlock - 1) / kNrcBlock), dim3(kNrcBlock), 0, ctx->stream,
-                       st->grad.ptr, st->trainGrid, st->gradSum.ptr);
+                       st->grad.ptr, parts, st->gradSum.ptr);
     if (!check(hipGetLastError(), "nrc gradient")) return -3;
     if (!check(hipMemcpyAsync(grad, st->gradSum.ptr, kNrcParams * sizeof(float), hipMemcpyDeviceToHost, ctx->stream),
                "nrc gradient D2H"))
