@@ -158,23 +158,29 @@ def stress_variant(steps: int = 4, warmup: int = 2):
     build_ms = st.bvhBuildMs
     r.synchronize()
     t0 = time.perf_counter()
-    bounces = ext_ms = launches = 0
+    bounces = ext_rays = ext_ms = launches = tail_rays = tail_ms = 0
+    tail_depth = 0
     for _ in range(steps):
         r.draw()
         s = r.stats()
         bounces += s.extensionRays
+        ext_rays += s.raysBeforeTail  # the rays kExtend traced (the fused tail traced the rest)
         ext_ms += s.extendMs
         launches += s.extendLaunches
+        tail_rays += s.tailRays
+        tail_ms += s.tailMs
+        tail_depth = s.tailStartDepth
     r.synchronize()
     elapsed = time.perf_counter() - t0
     r.set_traversal_statistics(True)
     r.draw()
     s = r.stats()
     r.set_traversal_statistics(False)
-    nodes = s.nodeVisits / max(1, s.extensionRays)
-    tris = s.triangleTests / max(1, s.extensionRays)
+    # per kExtend ray: the depths before the one the fused tail starts at
+    nodes = s.nodeVisitsBeforeTail / max(1, s.raysBeforeTail)
+    tris = s.triangleTestsBeforeTail / max(1, s.raysBeforeTail)
     per_ray = B_EXTEND_IO + nodes * NODE_BYTES + tris * TRI_BYTES
-    achieved = bounces * per_ray / (ext_ms * 1e-3) / 1e9 if ext_ms > 0 else 0.0
+    achieved = ext_rays * per_ray / (ext_ms * 1e-3) / 1e9 if ext_ms > 0 else 0.0
     out = {
         "workload": WORKLOADS["stress"]["name"],
         "triangles": scene.triangle_count,
@@ -188,7 +194,9 @@ def stress_variant(steps: int = 4, warmup: int = 2):
         "kExtend": {"achieved": round(achieved, 1), "frac": round(achieved / HBM_PEAK_GBPS, 4), "unit": "GB/s",
                     "avg_launch_ms": round(ext_ms / max(1, launches), 4), "launches": launches,
                     "bytes_per_ray": round(per_ray, 1), "nodes_per_ray": round(nodes, 3),
-                    "tris_per_ray": round(tris, 3)},
+                    "tris_per_ray": round(tris, 3), "rays_per_step": ext_rays // steps},
+        "tail": {"start_depth": tail_depth, "rays_per_step": tail_rays // steps,
+                 "ms_per_step": round(tail_ms / steps, 3)},
     }
     log(f"stress: {scene.triangle_count} triangles, BVH build {build_ms:.1f} ms, {out['value']:.3f} Gbounces/s, "
         f"kExtend {achieved:.0f} GB/s ({achieved / HBM_PEAK_GBPS:.3f} of peak), {nodes:.2f} nodes/ray, "
@@ -518,7 +526,9 @@ def main():
     bounces = shadow = paths = 0
     ext_ms = shade_ms = ext0_ms = shade0_ms = 0.0
     ext_launches = 0
-    ext_rays = 0
+    ext_rays = tail_rays = 0
+    tail_ms = 0.0
+    tail_depth = 0
     for i in range(args.steps):
         r.draw()
         s = r.stats()
@@ -530,9 +540,12 @@ def main():
         ext0_ms += s.extendMsDepth0
         shade0_ms += s.shadeMsDepth0
         ext_launches += s.extendLaunches
+        ext_rays += s.raysBeforeTail  # the rays kExtend traced (the fused tail traced the rest)
+        tail_rays += s.tailRays
+        tail_ms += s.tailMs
+        tail_depth = s.tailStartDepth
     r.synchronize()
     elapsed = time.perf_counter() - t0
-    ext_rays = bounces
     value = bounces / elapsed / 1e9
     ms_per_step = elapsed * 1e3 / args.steps
     log(f"timed {args.steps} frames: {elapsed:.3f} s, {bounces} bounces, {shadow} shadow rays -> {value:.3f} Gbounces/s")
@@ -548,10 +561,13 @@ def main():
     r.draw()
     s = r.stats()
     r.set_traversal_statistics(False)
-    nodes_per_ray = s.nodeVisits / max(1, s.extensionRays)
-    tris_per_ray = s.triangleTests / max(1, s.extensionRays)
+    # every ray (the whole-pipeline model) and kExtend's rays (the depths before the fused tail)
+    b_trav_all = (s.nodeVisits * NODE_BYTES + s.triangleTests * TRI_BYTES) / max(1, s.extensionRays)
+    nodes_per_ray = s.nodeVisitsBeforeTail / max(1, s.raysBeforeTail)
+    tris_per_ray = s.triangleTestsBeforeTail / max(1, s.raysBeforeTail)
     b_trav = nodes_per_ray * NODE_BYTES + tris_per_ray * TRI_BYTES
-    log(f"traversal: {nodes_per_ray:.2f} nodes/ray, {tris_per_ray:.2f} triangles/ray -> {b_trav:.0f} B/ray")
+    log(f"traversal (kExtend rays): {nodes_per_ray:.2f} nodes/ray, {tris_per_ray:.2f} triangles/ray -> {b_trav:.0f} B/ray;"
+        f" all rays {b_trav_all:.0f} B/ray")
 
     # Dominant kernel: the closest-hit traversal (kExtend), timed with HIP events on its stream.
     ext_bytes = ext_rays * (B_EXTEND_IO + b_trav)
@@ -577,6 +593,7 @@ def main():
         "traffic": traffic,
         "kernel": "kExtend<false, false, false> (closest-hit traversal, 8-wide quantised BVH)",
         "launches": ext_launches,
+        "rays_per_launch": ext_rays // max(1, ext_launches),
         "avg_launch_ms": round(ext_avg_ms, 4),
         "bytes_per_ray": round(B_EXTEND_IO + b_trav, 1),
         "nodes_per_ray": round(nodes_per_ray, 3),
@@ -584,7 +601,7 @@ def main():
     }
     # Whole-pipeline algorithmic bytes per bounce over wall time.
     bounces_per_px = bounces / (WIDTH * HEIGHT * args.steps)
-    b_bounce = (B_RAY + B_HIT + B_STATE + b_trav + B_NEE * shadow / max(1, bounces)
+    b_bounce = (B_RAY + B_HIT + B_STATE + b_trav_all + B_NEE * shadow / max(1, bounces)
                 + B_ACCUM_PER_PIXEL / max(bounces_per_px, 1e-9))
     pipeline = {
         "bytes_per_bounce": round(b_bounce, 1),
@@ -602,8 +619,12 @@ def main():
         "depth_ge1": {"bounces_per_step": (bounces - paths) // max(1, args.steps),
                       "extend_ms_per_step": round((ext_ms - ext0_ms) / max(1, args.steps), 3),
                       "shade_ms_per_step": round((shade_ms - shade0_ms) / max(1, args.steps), 3),
+                      "tail_ms_per_step": round(tail_ms / max(1, args.steps), 3),
                       "gbounces_per_s_of_kernel_time": round((bounces - paths) / max(1e-9, (ext_ms - ext0_ms + shade_ms
-                                                                                          - shade0_ms) * 1e-3) / 1e9, 4)},
+                                                                                          - shade0_ms + tail_ms) * 1e-3)
+                                                             / 1e9, 4)},
+        "tail": {"start_depth": tail_depth, "rays_per_step": tail_rays // max(1, args.steps),
+                 "ms_per_step": round(tail_ms / max(1, args.steps), 3)},
     }
     log(f"depth 0: {depth_split['depth0']}; depth >= 1: {depth_split['depth_ge1']}")
 

@@ -61,6 +61,13 @@ typedef struct VKRT_TraceStats {
     float backwardMs;       /* device time of the last VKRT_renderBackward (primal + replay) */
     float extendMsDepth0;   /* the part of extendMs spent on the primary rays (depth 0) */
     float shadeMsDepth0;    /* the part of shadeMs spent on the primary hits and misses (depth 0) */
+    uint32_t tailStartDepth; /* depth from which the fused tail carried the paths (0: none) */
+    uint32_t tailLaunches;
+    float tailMs;            /* fused tail kernel time; not part of extendMs or shadeMs */
+    uint64_t tailRays;       /* extension rays the fused tail traced (part of extensionRays) */
+    uint64_t raysBeforeTail; /* extension rays traced before tailStartDepth (all of them without a tail) */
+    uint64_t nodeVisitsBeforeTail;    /* nodeVisits and triangleTests of those rays, with statistics */
+    uint64_t triangleTestsBeforeTail; /* enabled (a statistics frame runs no tail but records them) */
 } VKRT_TraceStats;
 VKRT_Result VKRT_getTraceStats(const VKRT* vkrt, VKRT_TraceStats* outStats);
 
@@ -104,6 +111,18 @@ VKRT_Result VKRT_renderBackward(VKRT* vkrt, const float* dLossdImage, VKRT_Gradi
  * on the 1080p spectral workload the partition and the indirect, strided
  * reads of the sorted order cost more than the divergence they remove. */
 VKRT_Result VKRT_setMaterialSorting(VKRT* vkrt, uint8_t enabled);
+
+/* Fused tail (an addition): once the queues of a frame have become short, one
+ * launch carries every remaining path to its end, tracing and shading vertex
+ * after vertex per lane, instead of a traversal and a shading launch per depth
+ * that each wait for their slowest ray. It takes over at the first depth whose
+ * queue held at most maxRays rays in the previous frame of the same size and
+ * depth, in frames of one chunk whose shading needs none of shadow rays, the
+ * neural cache, feature images, material sorting, the alpha test, material
+ * textures or sheen; other frames keep one launch pair per depth. Results are
+ * unchanged (every path keeps its random stream and arithmetic). 0 turns it
+ * off; default 2^21 rays. */
+VKRT_Result VKRT_setFusedTail(VKRT* vkrt, uint32_t maxRays);
 
 /* Denoiser feature images. The reference always writes its albedo and normal
  * outputs for the denoiser (src/shaders/integrator/path/loop.slang:83-111,

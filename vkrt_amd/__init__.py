@@ -332,6 +332,13 @@ class TraceStats(C.Structure):
         ("backwardMs", C.c_float),
         ("extendMsDepth0", C.c_float),
         ("shadeMsDepth0", C.c_float),
+        ("tailStartDepth", C.c_uint32),
+        ("tailLaunches", C.c_uint32),
+        ("tailMs", C.c_float),
+        ("tailRays", C.c_uint64),
+        ("raysBeforeTail", C.c_uint64),
+        ("nodeVisitsBeforeTail", C.c_uint64),
+        ("triangleTestsBeforeTail", C.c_uint64),
     ]
 
 
@@ -572,6 +579,7 @@ def library() -> C.CDLL:
         "VKRT_addTextureFromPixels": (C.c_int, [P, C.POINTER(TextureUpload), up]),
         "VKRT_removeTexture": (C.c_int, [P, u32]),
         "VKRT_setMaterialSorting": (C.c_int, [P, C.c_uint8]),
+        "VKRT_setFusedTail": (C.c_int, [P, C.c_uint32]),
         "VKRT_defaultNeuralCacheSettings": (None, [C.POINTER(NeuralCacheSettings)]),
         "VKRT_setNeuralCache": (C.c_int, [P, C.POINTER(NeuralCacheSettings)]),
         "VKRT_getNeuralCacheStats": (C.c_int, [P, C.POINTER(NeuralCacheStats)]),
@@ -1061,6 +1069,9 @@ class Renderer:
 
     def set_material_sorting(self, enabled: bool):
         _ok(self.lib.VKRT_setMaterialSorting(self.h, 1 if enabled else 0), "VKRT_setMaterialSorting")
+
+    def set_fused_tail(self, max_rays: int):
+        _ok(self.lib.VKRT_setFusedTail(self.h, max_rays), "VKRT_setFusedTail")
 
     def set_traversal_statistics(self, enabled: bool):
         _ok(self.lib.VKRT_setTraversalStatistics(self.h, 1 if enabled else 0), "VKRT_setTraversalStatistics")

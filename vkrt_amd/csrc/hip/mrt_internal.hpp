@@ -151,6 +151,12 @@ struct mrt_context {
     mrt::DeviceArray<uint32_t> counters;
     mrt::DeviceArray<uint32_t> traversalStack;
     uint32_t traceGrid = 0;
+    uint32_t tailGrid = 0;
+    // queue length of every depth in the last single-chunk frame, with the
+    // chunk size and depth it was measured for: where the fused tail may start
+    uint32_t depthHint[32] = {};
+    uint64_t hintPaths = 0;
+    uint32_t hintMaxDepth = 0;
 
     mrt::SceneView view() const {
         mrt::SceneView v{};

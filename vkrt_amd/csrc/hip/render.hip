@@ -47,7 +47,10 @@ constexpr int kBlock = 256;
 // counters[] layout
 constexpr int kCntQueue0 = 0, kCntQueue1 = 1, kCntShadow = 2, kCntShadowEnv = 3, kCntNrcQuery = 4, kCntDropped = 5,
               kCntNrcRecord = 6, kCntNrcTrain = 7,
-              kCntStats = 8;  // 64-bit stats at counters+8: [0] ext rays, [1] shadow rays, [2] nodes, [3] triangles
+              kCntStats = 8;  // 64-bit stats at counters+8: [0] ext rays, [1] shadow rays, [2] nodes, [3] triangles,
+                              // [4] rays of the fused tail, [5..7] [0], [2], [3] as they stood when the tail began
+// queue length of every depth (the first kDepthHist depths) and the fused tail's claim counter
+constexpr int kCntDepthHist = 24, kDepthHist = 32, kCntTailFetch = kCntDepthHist + kDepthHist;
 // Work claims of the persistent traversal kernels. The batch space of a queue
 // is split into kFetchParts interleaved partitions, each with its own counter
 // on its own 256-byte line: device-scope atomics from the 8 XCDs then spread
@@ -426,14 +429,16 @@ __global__ __launch_bounds__(kBlock) void kGenerate(FrameParams fp, PathView pv,
     q.d[i] = make_float4(d.x, d.y, d.z, kRayTMax);
 }
 
-__global__ void kPrepDepth(uint32_t* counters, int cur, int nxt) {
+__global__ void kPrepDepth(uint32_t* counters, int cur, int nxt, uint32_t depth) {
     const uint32_t t = threadIdx.x;
     if (t == 0) {
         unsigned long long* stats = reinterpret_cast<unsigned long long*>(counters + kCntStats);
         stats[0] += counters[cur];
+        if (depth < (uint32_t)kDepthHist) counters[kCntDepthHist + depth] = counters[cur];
         counters[nxt] = 0u;
         counters[kCntShadow] = 0u;
         counters[kCntShadowEnv] = 0u;
+        counters[kCntTailFetch] = 0u;
     }
     if (t < 3u * kFetchParts) counters[kCntFetchExt + t * kFetchStride] = 0u;
 }
@@ -442,6 +447,13 @@ __global__ void kPrepShadow(uint32_t* counters) {
     stats[1] += counters[kCntShadow] + counters[kCntShadowEnv];
 }
 __global__ void kSetCount(uint32_t* counters, int slot, uint32_t value) { counters[slot] = value; }
+// The ray and traversal totals as they stand before the depth the fused tail takes over.
+__global__ void kTailSnapshot(uint32_t* counters) {
+    unsigned long long* stats = reinterpret_cast<unsigned long long*>(counters + kCntStats);
+    stats[5] = stats[0];
+    stats[6] = stats[2];
+    stats[7] = stats[3];
+}
 
 // Closest-hit traversal, persistent with lane refill. Every loop iteration
 // advances each busy lane by one node or leaf visit; once kRefillLanes lanes of
@@ -669,28 +681,24 @@ __global__ __launch_bounds__(kBlock) void kNrcTargets(const float4* record, cons
 // are read only where used, and shadow rays are queued before BSDF sampling.
 // Materials with sheen take SHEEN variants at two waves per SIMD: the sheen
 // layer does not fit the three-wave budget of the hero kernel.
-template <int MODE, bool NRC = false, bool TEX = false, bool SHEEN = false>
-__global__ __launch_bounds__(kBlock, SHEEN ? 2 : 3) void kShade(FrameParams fp, SceneView sv, PathView pv, QueueView in,
-                                                 const float4* hits, QueueView out, ShadowView sh, ShadowView she,
-                                                 uint32_t* counters, int inSlot, int outSlot, NrcView nv,
-                                                 const uint32_t* __restrict__ order) {
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    const uint32_t n = counters[inSlot];
-    bool cont = false, shadowEmit = false, envEmit = false;
-    F3 nextO = f3(0.0f), nextD = f3(0.0f), shO = f3(0.0f), shD = f3(0.0f), seO = f3(0.0f), seD = f3(0.0f);
+// One path vertex: the ray (ro, rdir) of path q = bits of ro.w, at `depth`,
+// and its hit h. Leaves the continuation ray in nextO / nextD with cont set, or
+// cont false when the path ends here. Shadow and cache records are appended to
+// their queues by whichever lanes of the wave reach the append.
+template <int MODE, bool NRC, bool TEX, bool SHEEN>
+DEV void shadeVertex(const FrameParams& fp, uint32_t depth, const SceneView& sv, const PathView& pv, float4 ro,
+                     float4 rdir, Hit h, const ShadowView& sh, const ShadowView& she, uint32_t* counters,
+                     const NrcView& nv, bool& cont, F3& nextO, F3& nextD) {
+    bool shadowEmit = false, envEmit = false;
+    F3 shO = f3(0.0f), shD = f3(0.0f), seO = f3(0.0f), seD = f3(0.0f);
     float shT = 0.0f;
     F4 shA = f4(0.0f), shB = f4(0.0f), seA = f4(0.0f);
-    uint32_t q = 0;
-    if (i < n) do {
+    const uint32_t q = __float_as_uint(ro.w);
+    do {
         const SceneData& sd = fp.sd;
-        const uint32_t j = order ? order[i] : i;  // shading order (sorted by material class) or queue order
-        float4 ro = in.o[j], rdir = in.d[j];
-        q = __float_as_uint(ro.w);
         F3 o = xyz(ro), d = xyz(rdir);
-        float4 hh = hits[j];
-        Hit h{hh.x, __float_as_uint(hh.y), hh.z, hh.w};
         // depth 0 of a lean frame: the path state is the one kGenerate would have written
-        const bool first = fp.leanStart != 0u && fp.depth == 0u;
+        const bool first = fp.leanStart != 0u && depth == 0u;
         PathStart ps{};
         if (first) ps = pathStart<MODE>(fp, q);
         // first read of the accumulated radiance in this vertex (0 at depth 0 of a lean frame)
@@ -784,7 +792,7 @@ __global__ __launch_bounds__(kBlock, SHEEN ? 2 : 3) void kShade(FrameParams fp, 
         // wavelengths collapsed is traced on without the cache)
         const bool nrcLanes = MODE != kModeHero || heroActive;
         const bool nrcTrain = NRC && q >= fp.nrcRenderPaths;
-        const bool nrcQuery = NRC && !nrcTrain && fp.depth == fp.nrcPrefix && nrcLanes;
+        const bool nrcQuery = NRC && !nrcTrain && depth == fp.nrcPrefix && nrcLanes;
         Rng rng = first ? ps.rng : loadRng(pv.rng[q], fp.rngPolicy);
         F4 tpp = heroActive ? (first ? f4(1.0f) : ld4(pv.tpp[q])) : f4(0.0f);
 
@@ -883,7 +891,7 @@ __global__ __launch_bounds__(kBlock, SHEEN ? 2 : 3) void kShade(FrameParams fp, 
             const bool prevNee = (flags & kFPrevNee) != 0u;
             if (MODE == kModeHero && heroActive) {
                 float w = heroBalance(tpp);
-                if (fp.neeEnabled && prevNee && fp.depth > 0u) {
+                if (fp.neeEnabled && prevNee && depth > 0u) {
                     float lp = areaToSolidAngle(sv.meshes[sf.mesh].lightPdfArea, sf.ng, d, h.t);
                     F4 ptpp = ld4(pv.ptpp[q]), pbtp = ld4(pv.pbtp[q]);
                     w = spectralMis(ptpp * pbtp, ptpp * lp);
@@ -891,7 +899,7 @@ __global__ __launch_bounds__(kBlock, SHEEN ? 2 : 3) void kShade(FrameParams fp, 
                 em = spec4(fp.spec, mat.emission, lam) * w;
             } else {
                 float w = 1.0f;
-                if (fp.neeEnabled && prevNee && fp.depth != 0u && misc.x > 0.0f) {
+                if (fp.neeEnabled && prevNee && depth != 0u && misc.x > 0.0f) {
                     float lp = areaToSolidAngle(sv.meshes[sf.mesh].lightPdfArea, sf.ng, d, h.t);
                     w = lp <= 0.0f ? 1.0f : powerHeuristic(misc.x, lp);
                 }
@@ -994,7 +1002,7 @@ __global__ __launch_bounds__(kBlock, SHEEN ? 2 : 3) void kShade(FrameParams fp, 
         }
 
         // the last vertex of the loop never traces its continuation
-        if (fp.depth + 1u >= sd.rrMaxDepth) break;
+        if (depth + 1u >= sd.rrMaxDepth) break;
 
         // BSDF sampling (src/shaders/bsdf/sample_rgb.slang, sample_spectral.slang)
         if (wo.z <= 0.0f) break;
@@ -1061,7 +1069,7 @@ __global__ __launch_bounds__(kBlock, SHEEN ? 2 : 3) void kShade(FrameParams fp, 
                          MODE == kModeRgb ? 0u : 1u, newFlags, sigma);
 
         // Russian roulette (rgb/transport.slang:56-62, spectral_hero/transport.slang:138-152)
-        if (fp.depth + 1u >= sd.rrMinDepth) {
+        if (depth + 1u >= sd.rrMinDepth) {
             float p;
             if (MODE == kModeRgb)
                 p = clampf(maxc(F3{thr.x, thr.y, thr.z}), kRRMin, kRRMax);
@@ -1094,11 +1102,130 @@ __global__ __launch_bounds__(kBlock, SHEEN ? 2 : 3) void kShade(FrameParams fp, 
         if (newFlags & kFAbsorption) pv.sigma[q] = st4(sigma);
         if (MODE == kModeHero) pv.tpp[q] = st4(tpp);
     } while (false);
+}
 
+template <int MODE, bool NRC = false, bool TEX = false, bool SHEEN = false>
+__global__ __launch_bounds__(kBlock, SHEEN ? 2 : 3) void kShade(FrameParams fp, SceneView sv, PathView pv, QueueView in,
+                                                 const float4* hits, QueueView out, ShadowView sh, ShadowView she,
+                                                 uint32_t* counters, int inSlot, int outSlot, NrcView nv,
+                                                 const uint32_t* __restrict__ order) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t n = counters[inSlot];
+    bool cont = false;
+    F3 nextO = f3(0.0f), nextD = f3(0.0f);
+    uint32_t q = 0;
+    if (i < n) {
+        const uint32_t j = order ? order[i] : i;  // shading order (sorted by material class) or queue order
+        const float4 ro = in.o[j], rdir = in.d[j], hh = hits[j];
+        q = __float_as_uint(ro.w);
+        shadeVertex<MODE, NRC, TEX, SHEEN>(fp, fp.depth, sv, pv, ro, rdir, Hit{hh.x, __float_as_uint(hh.y), hh.z, hh.w},
+                                           sh, she, counters, nv, cont, nextO, nextD);
+    }
     uint32_t slot = waveAppend(cont, &counters[outSlot]);
     if (cont) {
         out.o[slot] = make_float4(nextO.x, nextO.y, nextO.z, __uint_as_float(q));
         out.d[slot] = make_float4(nextD.x, nextD.y, nextD.z, kRayTMax);
+    }
+}
+
+// The last depths of a chunk in one launch, once the queues have become short:
+// every lane takes a path of the queue at depth startDepth and carries it to
+// its end, tracing and shading vertex after vertex (the arithmetic of kExtend
+// and kShade, so the same image), instead of one launch per depth that waits
+// for the slowest ray of that depth. A lane that has its hit waits until
+// kTailShadeLanes lanes of the wave have theirs, or no lane is still tracing,
+// and they shade together; idle lanes claim new paths in batches as in
+// kExtend. Used when no shadow rays, cache, features, material sort, alpha
+// test, material textures or sheen are active (the shading variant without
+// them), at two waves per SIMD: the shading body inside the lane loop needs
+// more registers than the three-wave budget of kShade.
+constexpr uint32_t kTailShadeLanes = 32;
+template <int MODE>
+__global__ __launch_bounds__(kTraceBlock, 2) void kTail(FrameParams fp, SceneView sv, PathView pv, QueueView in,
+                                                        uint32_t* counters, int countSlot, uint32_t startDepth,
+                                                        uint32_t* gstack) {
+    __shared__ uint32_t lds[kLdsStackWords];
+    TraversalStack st;
+    st.lds = lds + threadIdx.x;
+    st.glob = gstack + (size_t)(blockIdx.x * kTraceBlock + threadIdx.x) * kGlobalStackWords;
+    st.dropped = 0;
+    st.sp = 0;
+    const uint32_t n = counters[countSlot];
+    // wavefronts beyond one per 64 queued paths leave at once
+    if ((blockIdx.x * (kTraceBlock / 64u) + (threadIdx.x >> 6)) * 64ull >= n) return;
+    constexpr uint32_t kIdle = 0u, kTrace = 1u, kReady = 2u;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t lanesBelow = (1ull << lane) - 1ull;
+    uint32_t state = kIdle, depth = 0u, traced = 0u;
+    bool exhausted = false;  // wave-uniform: the queue has been claimed to its end
+    float4 ro = make_float4(0.0f, 0.0f, 0.0f, 0.0f), rd = ro;
+    TraversalRay r;
+    ClosestState cs;
+    cs.ref = 0;
+    cs.steps = 0;
+    cs.h.t = 0.0f;
+    cs.h.tri = kNoHit;
+    cs.h.u = cs.h.v = 0.0f;
+    // start the traversal of the lane's ray (ro, rd): a ray that misses the
+    // scene's box (or a scene without geometry) has its miss at once
+    auto beginRay = [&]() {
+        r = makeTraversalRay(xyz(ro), xyz(rd));
+        if (sv.nodeCount > 0u && reachesSceneBox(sv, r, kRayTMin, rd.w)) {
+            closestBegin(cs, st, rd.w);
+            state = kTrace;
+        } else {
+            cs.h.t = rd.w;
+            cs.h.tri = kNoHit;
+            cs.h.u = cs.h.v = 0.0f;
+            state = kReady;
+        }
+    };
+    while (true) {
+        const uint64_t idleMask = __ballot(state == kIdle);
+        const uint32_t idle = (uint32_t)__popcll(idleMask);
+        if (!exhausted && (idle >= kRefillLanes || idle == 64u)) {
+            uint32_t base = 0;
+            if (lane == 0u) base = atomicAdd(&counters[kCntTailFetch], idle);
+            base = (uint32_t)__shfl((int)base, 0);
+            if (base + idle >= n) exhausted = true;
+            if (state == kIdle) {
+                const uint32_t k = base + (uint32_t)__popcll(idleMask & lanesBelow);
+                if (k < n) {
+                    ro = in.o[k];
+                    rd = in.d[k];
+                    depth = startDepth;
+                    beginRay();
+                }
+            }
+        }
+        if (state == kTrace && !closestStep<false, false>(sv, r, kRayTMin, cs, st, 0u)) state = kReady;
+        const uint32_t ready = (uint32_t)__popcll(__ballot(state == kReady));
+        if (ready >= kTailShadeLanes || (ready > 0u && __ballot(state == kTrace) == 0ull)) {
+            if (state == kReady) {
+                bool cont = false;
+                F3 nextO = f3(0.0f), nextD = f3(0.0f);
+                shadeVertex<MODE, false, false, false>(fp, depth, sv, pv, ro, rd, cs.h, ShadowView{}, ShadowView{},
+                                                       counters, NrcView{}, cont, nextO, nextD);
+                if (cont) {
+                    ro = make_float4(nextO.x, nextO.y, nextO.z, ro.w);
+                    rd = make_float4(nextD.x, nextD.y, nextD.z, kRayTMax);
+                    depth++;
+                    traced++;
+                    beginRay();
+                } else {
+                    state = kIdle;
+                }
+            }
+        }
+        if (exhausted && __ballot(state != kIdle) == 0ull) break;
+    }
+    if (st.dropped) atomicAdd(&counters[kCntDropped], st.dropped);
+    // the continuation rays (the queue's own rays were counted when the depth began)
+    for (int off = 32; off > 0; off >>= 1) traced += (uint32_t)__shfl_down((int)traced, off);
+    if (lane == 0u && traced) {
+        unsigned long long* stats = reinterpret_cast<unsigned long long*>(counters + kCntStats);
+        atomicAdd(&stats[0], (unsigned long long)traced);
+        atomicAdd(&stats[4], (unsigned long long)traced);
     }
 }
 
@@ -2005,6 +2132,8 @@ bool ensureTraversalScratch(mrt_context* ctx) {
         // 8 blocks per CU; a multiple of kFetchParts so every claim partition has blocks
         ctx->traceGrid = (uint32_t)std::max(1, ctx->cuCount) * 8u;
         static_assert(8 % kFetchParts == 0, "trace grid must cover every claim partition");
+        // the fused tail: two blocks per CU (two waves per SIMD), within the trace grid's stacks
+        ctx->tailGrid = (uint32_t)std::max(1, ctx->cuCount) * 2u;
     }
     size_t need = (size_t)ctx->traceGrid * kTraceBlock * kGlobalStackWords;
     return ctx->traversalStack.reserve(need);
@@ -2045,7 +2174,7 @@ static PathView pathView(mrt_context* ctx) {
 
 struct StageEvents {
     std::vector<hipEvent_t> pool;
-    std::vector<int> stage;  // per event pair: 0 extend, 1 shade, 2 shadow, 3 cache inference, 4 cache training
+    std::vector<int> stage;  // per event pair: 0 extend, 1 shade, 2 shadow, 3 cache inference, 4 cache training, 5 fused tail
     std::vector<uint32_t> depth;  // per event pair: the path depth it belongs to
     uint32_t depthNow = 0;
     size_t used = 0;
@@ -2116,10 +2245,13 @@ static void launchShade(hipStream_t s, bool tex, const NrcRun* nrc, const FrameP
                            hits, out, shv, shev, cnt, cur, nxt, NrcView{}, order);
 }
 
+// tailDepth: the depth from which the fused tail kernel carries the paths left
+// (0: none). With traversal statistics the depths stay separate launches and
+// only the totals at that depth are recorded.
 template <int MODE>
 static bool launchChunk(mrt_context* ctx, FrameParams fp, StageEvents& ev, bool countTraversal,
                         const NrcRun* nrc = nullptr, bool keepSampleRad = false, bool sortShading = false,
-                        const FeatureView* fv = nullptr) {
+                        const FeatureView* fv = nullptr, uint32_t tailDepth = 0u) {
     hipStream_t s = ctx->stream;
     PathBuffers& b = ctx->paths;
     PathView pv = pathView(ctx);
@@ -2147,7 +2279,15 @@ static bool launchChunk(mrt_context* ctx, FrameParams fp, StageEvents& ev, bool 
         int nxt = 1 - cur;
         fp.depth = depth;
         ev.depthNow = depth;
-        hipLaunchKernelGGL(kPrepDepth, dim3(1), dim3(64), 0, s, cnt, cur, nxt);
+        if (tailDepth != 0u && depth == tailDepth) hipLaunchKernelGGL(kTailSnapshot, dim3(1), dim3(1), 0, s, cnt);
+        hipLaunchKernelGGL(kPrepDepth, dim3(1), dim3(64), 0, s, cnt, cur, nxt, depth);
+        if (tailDepth != 0u && depth == tailDepth && !countTraversal) {
+            bool timed = ev.begin(s, 5);
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(kTail<MODE>), dim3(ctx->tailGrid), dim3(kTraceBlock), 0, s, fp, sv, pv,
+                               qv[cur], cnt, cur, depth, ctx->traversalStack.ptr);
+            if (timed) ev.end(s);
+            break;
+        }
         bool timed = ev.begin(s, 0);
         const uint32_t pathBase = (uint32_t)fp.pathBegin;
         launchExtend(ctx, countTraversal, sv, qv[cur], b.hits.ptr, cnt, cur,
@@ -2469,6 +2609,23 @@ extern "C" int mrt_render_frame(mrt_context* ctx, const SceneData* scene, const 
         fview = FeatureView{b.featState.ptr, b.featAlbedo.ptr, b.featNormal.ptr, ctx->debug0.ptr, ctx->bounce0.ptr};
     }
     const FeatureView* fv = featurePass ? &fview : nullptr;
+    // The fused tail (kTail) takes over from the first depth whose queue held
+    // at most prm->tailRays rays in the previous frame of the same size: a
+    // single-chunk frame whose shading variant is the plain one (no shadow
+    // rays, cache, features, material sort, alpha test, textures or sheen).
+    uint32_t tailDepth = 0u;
+    const uint32_t maxDepth = scene->rrMaxDepth;
+    const SceneView svNow = ctx->view();
+    const bool tailAllowed = prm->tailRays > 0u && totalPaths == chunk && !fp.nrc && !fv && !fs.sortShading &&
+                             !fp.neeEnabled && !fp.envNee && svNow.triAlpha == nullptr && svNow.texels == nullptr &&
+                             !ctx->sheenEnabled;
+    if (tailAllowed && ctx->hintPaths == chunk && ctx->hintMaxDepth == maxDepth) {
+        for (uint32_t d = 1; d < std::min<uint32_t>(maxDepth, kDepthHist); d++)
+            if (ctx->depthHint[d] <= prm->tailRays) {
+                tailDepth = d;
+                break;
+            }
+    }
     for (uint64_t begin = 0; begin < totalPaths; begin += chunk) {
         fp.pathBegin = begin;
         fp.count = (uint32_t)std::min<uint64_t>(chunk, totalPaths - begin);
@@ -2476,11 +2633,11 @@ extern "C" int mrt_render_frame(mrt_context* ctx, const SceneData* scene, const 
         const bool sorted = fs.sortShading;
         bool ok = mode == kModeRgb
                       ? launchChunk<kModeRgb>(ctx, fp, ev, countTraversal, fp.nrc ? &nrcRun : nullptr, false, sorted,
-                                              fv)
+                                              fv, tailDepth)
                   : mode == kModeHero ? launchChunk<kModeHero>(ctx, fp, ev, countTraversal, fp.nrc ? &nrcRun : nullptr,
-                                                               false, sorted, fv)
+                                                               false, sorted, fv, tailDepth)
                                       : launchChunk<kModeSingle>(ctx, fp, ev, countTraversal, nullptr, false, sorted,
-                                                                 fv);
+                                                                 fv, tailDepth);
         if (!ok) return -4;
         chunks++;
     }
@@ -2503,8 +2660,15 @@ extern "C" int mrt_render_frame(mrt_context* ctx, const SceneData* scene, const 
         setError("traversal stack overflow");
         return -5;
     }
+    if (totalPaths == chunk && !fp.nrc) {
+        // queue lengths for the next frame's choice of the tail depth (depths
+        // the fused tail carried were not recorded and read as 0)
+        for (uint32_t d = 0; d < (uint32_t)kDepthHist; d++) ctx->depthHint[d] = hostCounters[kCntDepthHist + d];
+        ctx->hintPaths = chunk;
+        ctx->hintMaxDepth = maxDepth;
+    }
     if (stats) {
-        unsigned long long st64[4];
+        unsigned long long st64[8];
         memcpy(st64, hostCounters + kCntStats, sizeof(st64));
         stats->extensionRays = st64[0];
         stats->shadowRays = st64[1];
@@ -2514,8 +2678,8 @@ extern "C" int mrt_render_frame(mrt_context* ctx, const SceneData* scene, const 
         float ms = 0.0f;
         hipEventElapsedTime(&ms, ctx->evBegin, ctx->evEnd);
         stats->traceMs = ms;
-        float stageMs[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-        uint32_t stageN[5] = {0u, 0u, 0u, 0u, 0u};
+        float stageMs[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        uint32_t stageN[6] = {0u, 0u, 0u, 0u, 0u, 0u};
         for (size_t e = 0; e + 1 < ev.used; e += 2) {
             float t = 0.0f;
             hipEventElapsedTime(&t, ev.pool[e], ev.pool[e + 1]);
@@ -2533,6 +2697,15 @@ extern "C" int mrt_render_frame(mrt_context* ctx, const SceneData* scene, const 
         stats->shadowLaunches = stageN[2];
         stats->nrcInferMs = stageMs[3];
         stats->nrcTrainMs = stageMs[4];
+        stats->raysBeforeTail = tailDepth ? st64[5] : st64[0];
+        stats->nodeVisitsBeforeTail = tailDepth ? st64[6] : st64[2];
+        stats->triangleTestsBeforeTail = tailDepth ? st64[7] : st64[3];
+        if (tailDepth && !countTraversal) {
+            stats->tailStartDepth = tailDepth;
+            stats->tailLaunches = stageN[5];
+            stats->tailMs = stageMs[5];
+            stats->tailRays = st64[0] - st64[5];
+        }
         if (fp.nrc) {
             stats->nrcQueries = std::min(hostCounters[kCntNrcQuery], fs.nrcQueryCapacity);
             stats->nrcRecords = std::min(hostCounters[kCntNrcRecord], fs.nrcRecordCapacity);
@@ -2613,7 +2786,7 @@ extern "C" int mrt_render_backward(mrt_context* ctx, const SceneData* scene, con
         for (uint32_t depth = 0; depth < fp.sd.rrMaxDepth; depth++) {
             int nxt = 1 - cur;
             fp.depth = depth;
-            hipLaunchKernelGGL(kPrepDepth, dim3(1), dim3(64), 0, s, cnt, cur, nxt);
+            hipLaunchKernelGGL(kPrepDepth, dim3(1), dim3(64), 0, s, cnt, cur, nxt, depth);
             // the same alpha decisions as the primal pass: same salt, same path numbering
             launchExtend(ctx, false, sv, qv[cur], b.hits.ptr, cnt, cur, alphaSalt(fp.sd.frameNumber, fp.rngSeed, depth, 0u),
                          (uint32_t)fp.pathBegin);

@@ -41,6 +41,7 @@ VKRT_Result VKRT_create(VKRT** outVkrt) {
     vkrt->rngPolicy = VKRT_RNG_HASH;
     VKRT_defaultNeuralCacheSettings(&vkrt->nrc);
     vkrt->materialSort = 0;
+    vkrt->fusedTailRays = 1u << 21;
     vkrt->renderStatus.renderPhase = VKRT_RENDER_PHASE_INACTIVE;
     vkrt->finalImageDenoiseEnabled = 1u;
     *outVkrt = vkrt;
@@ -228,6 +229,7 @@ static void fillRenderParams(const VKRT* vkrt, mrt_render_params* rp) {
     rp->hasTransmissive = vkrt->packed.hasTransmissive;
     rp->traversalStatistics = vkrt->traversalStatistics;
     rp->materialSort = vkrt->materialSort;
+    rp->tailRays = vkrt->fusedTailRays;
     rp->features = vkrt->denoiserFeatures;
     /* the cache runs in RGB and in hero spectral mode (in CIE XYZ there) */
     const int cacheMode = vkrt->settings.renderMode == VKRT_RENDER_MODE_RGB ||
@@ -276,6 +278,13 @@ VKRT_Result VKRT_trace(VKRT* vkrt) {
     vkrt->stats.shadowLaunches = rs.shadowLaunches;
     vkrt->stats.nodeVisits = rs.nodeVisits;
     vkrt->stats.triangleTests = rs.triangleTests;
+    vkrt->stats.tailStartDepth = rs.tailStartDepth;
+    vkrt->stats.tailLaunches = rs.tailLaunches;
+    vkrt->stats.tailMs = rs.tailMs;
+    vkrt->stats.tailRays = rs.tailRays;
+    vkrt->stats.raysBeforeTail = rs.raysBeforeTail;
+    vkrt->stats.nodeVisitsBeforeTail = rs.nodeVisitsBeforeTail;
+    vkrt->stats.triangleTestsBeforeTail = rs.triangleTestsBeforeTail;
     vkrt->nrcStats.lastLoss = rs.nrcLoss;
     vkrt->nrcStats.queries = rs.nrcQueries;
     vkrt->nrcStats.trainingRecords = rs.nrcRecords;
@@ -448,6 +457,12 @@ VKRT_Result VKRT_deviceSynchronize(VKRT* vkrt) {
 VKRT_Result VKRT_setMaterialSorting(VKRT* vkrt, uint8_t enabled) {
     if (!vkrt) return VKRT_ERROR_INVALID_ARGUMENT;
     vkrt->materialSort = enabled ? 1u : 0u;
+    return VKRT_SUCCESS;
+}
+
+VKRT_Result VKRT_setFusedTail(VKRT* vkrt, uint32_t maxRays) {
+    if (!vkrt) return VKRT_ERROR_INVALID_ARGUMENT;
+    vkrt->fusedTailRays = maxRays;
     return VKRT_SUCCESS;
 }
 

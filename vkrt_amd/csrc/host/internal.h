@@ -133,6 +133,7 @@ struct VKRT {
     VKRT_NeuralCacheStats nrcStats;
     float sceneMin[3], sceneMax[3]; /* box of the uploaded geometry */
     uint8_t materialSort;
+    uint32_t fusedTailRays; /* VKRT_setFusedTail */
     uint8_t denoiserFeatures; /* accumulate the albedo / normal feature images */
 };
 

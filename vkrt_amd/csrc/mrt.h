@@ -107,6 +107,8 @@ typedef struct mrt_render_params {
     float sceneMax[3];
     uint32_t materialSort; /* shade each depth's hits grouped by material class */
     uint32_t features;     /* accumulate the denoiser albedo / normal feature images */
+    uint32_t tailRays;     /* fused tail from the first depth whose queue held at most this many rays
+                              in the previous frame (0: every depth is its own pair of launches) */
 } mrt_render_params;
 
 typedef struct mrt_render_stats {
@@ -129,6 +131,13 @@ typedef struct mrt_render_stats {
     float nrcTrainMs;
     float extendMsDepth0;  /* the part of extendMs / shadeMs spent on the paths' first rays */
     float shadeMsDepth0;
+    uint32_t tailStartDepth;   /* depth from which the fused tail kernel carried the paths (0: none) */
+    uint32_t tailLaunches;
+    float tailMs;
+    uint64_t tailRays;         /* extension rays the fused tail traced */
+    uint64_t raysBeforeTail;   /* extension rays of the depths before tailStartDepth (all without a tail) */
+    uint64_t nodeVisitsBeforeTail;    /* their node visits and triangle tests, with statistics enabled */
+    uint64_t triangleTestsBeforeTail; /* (a statistics frame records them at the depth a tail would start) */
 } mrt_render_stats;
 
 /* One frame: samplesPerPixel paths for every viewport pixel, accumulated into
