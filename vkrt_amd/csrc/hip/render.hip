@@ -1139,7 +1139,7 @@ __global__ __launch_bounds__(kBlock, SHEEN ? 2 : 3) void kShade(FrameParams fp, 
 // test, material textures or sheen are active (the shading variant without
 // them), at two waves per SIMD: the shading body inside the lane loop needs
 // more registers than the three-wave budget of kShade.
-constexpr uint32_t kTailShadeLanes = 32;
+constexpr uint32_t kTailShadeLanes = 48;  // of 32, 48, 56 and 64 lanes, 48 and 56 measured fastest
 template <int MODE>
 __global__ __launch_bounds__(kTraceBlock, 2) void kTail(FrameParams fp, SceneView sv, PathView pv, QueueView in,
                                                         uint32_t* counters, int countSlot, uint32_t startDepth,
