@@ -170,6 +170,7 @@ extern "C" int mrt_upload_geometry(mrt_context* ctx, const mrt_geometry_desc* g)
     ctx->triCount = g->triangleCount;
     ctx->shadeTrisDirty = true;
     ctx->triClassDirty = true;
+    ctx->hintPaths = 0;  // the queue lengths of the old geometry no longer say where the tail may start
     return sync(ctx, "upload geometry");
 }
 
@@ -179,6 +180,7 @@ extern "C" int mrt_upload_meshes(mrt_context* ctx, const MeshInfo* meshInfos, ui
     if (!ctx->meshes.upload(meshInfos, meshCount, ctx->stream)) return -3;
     ctx->shadeTrisDirty = true;
     ctx->triClassDirty = true;
+    ctx->hintPaths = 0;
     return sync(ctx, "upload meshes");
 }
 
