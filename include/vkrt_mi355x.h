@@ -121,7 +121,7 @@ VKRT_Result VKRT_setMaterialSorting(VKRT* vkrt, uint8_t enabled);
  * neural cache, feature images, material sorting, the alpha test, material
  * textures or sheen; other frames keep one launch pair per depth. Results are
  * unchanged (every path keeps its random stream and arithmetic). 0 turns it
- * off; default 2^21 rays. */
+ * off; default 2^20 rays (about two per lane of the traversal kernel's grid). */
 VKRT_Result VKRT_setFusedTail(VKRT* vkrt, uint32_t maxRays);
 
 /* Denoiser feature images. The reference always writes its albedo and normal

@@ -41,7 +41,7 @@ VKRT_Result VKRT_create(VKRT** outVkrt) {
     vkrt->rngPolicy = VKRT_RNG_HASH;
     VKRT_defaultNeuralCacheSettings(&vkrt->nrc);
     vkrt->materialSort = 0;
-    vkrt->fusedTailRays = 1u << 21;
+    vkrt->fusedTailRays = 1u << 20;
     vkrt->renderStatus.renderPhase = VKRT_RENDER_PHASE_INACTIVE;
     vkrt->finalImageDenoiseEnabled = 1u;
     *outVkrt = vkrt;
