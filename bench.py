@@ -67,14 +67,15 @@ WORKLOADS = {
 
 
 def traversal_digest() -> str:
-    """SHA-256 of the code the closest-hit kernel is built from (the traversal
-    and common device headers and kExtend's definition), recorded with the
-    committed traffic counters so that a changed kernel does not report them."""
+    """SHA-256 of the code the closest-hit kernel and the tree it walks are
+    built from (the traversal and common device headers, the BVH builder and
+    kExtend's definition), recorded with the committed traffic counters so that
+    a changed kernel or tree layout does not report them."""
     import hashlib
 
     h = hashlib.sha256()
     hip = os.path.join(ROOT, "vkrt_amd", "csrc", "hip")
-    for name in ("dev_traverse.h", "dev_common.h"):
+    for name in ("dev_traverse.h", "dev_common.h", "bvh.hip"):
         with open(os.path.join(hip, name), "rb") as f:
             h.update(f.read())
     with open(os.path.join(hip, "render.hip")) as f:

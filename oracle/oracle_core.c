@@ -383,6 +383,18 @@ static int alphaAccept(const Ctx* c, uint32_t tri, real u, real v, uint32_t key)
 }
 #define ALPHA_OK(c, i, u, v, key) (!(c)->b->triangleAlpha || alphaAccept((c), (i), (u), (v), (key)))
 
+/* The bound a triangle's t must stay below to become the closest hit: the
+ * closest hit is the smallest t and, among equal t, the lowest triangle index
+ * (what the brute-force loop's strict comparison in index order gives), so a
+ * tree's leaf order cannot change it. */
+static real tieBound(real best, uint32_t i, uint32_t bestTri) {
+#ifdef ORACLE_F64
+    return (bestTri != UINT32_MAX && i < bestTri) ? nextafter(best, INFINITY) : best;
+#else
+    return (bestTri != UINT32_MAX && i < bestTri) ? nextafterf(best, INFINITY) : best;
+#endif
+}
+
 static void traceClosest(const Ctx* c, Ray ray, Hit* h, uint32_t key) {
     h->hit = 0;
     h->mesh = VKRT_INVALID_INDEX;
@@ -417,7 +429,8 @@ static void traceClosest(const Ctx* c, Ray ray, Hit* h, uint32_t key) {
                 for (uint32_t k = 0; k < n->count; k++) {
                     uint32_t i = c->bvh->order[n->first + k];
                     real t, u, v;
-                    if (intersectTri(&tris[(size_t)i * 9u], ray.o, &p, ray.tmin, best, &t, &u, &v) &&
+                    if (intersectTri(&tris[(size_t)i * 9u], ray.o, &p, ray.tmin, tieBound(best, i, bestTri), &t, &u,
+                                     &v) &&
                         ALPHA_OK(c, i, u, v, key)) {
                         best = t;
                         bestTri = i;

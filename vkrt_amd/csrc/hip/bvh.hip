@@ -12,7 +12,7 @@
 //     tree of surface-area-heuristic quality.
 //  5. top-down collapse into 8-wide nodes: a node is opened greedily by
 //     splitting its largest-area child until eight children remain; subtrees
-//     of at most kLeafMaxTris triangles become leaves whose triangles are
+//     of at most kLeafMaxTris triangles (one) become leaves whose triangles are
 //     written contiguously (depth-first order) as 48-byte records; child
 //     boxes are quantised to 8 bits per plane, conservatively
 #include <cstring>

@@ -101,7 +101,11 @@ constexpr int kBvhWidth = 8;
 // bits 27..30 triangle count - 1; internal: node index. kEmptyRef = empty slot.
 constexpr uint32_t kLeafBit = 0x80000000u;
 constexpr uint32_t kEmptyRef = 0xFFFFFFFFu;
-constexpr uint32_t kLeafMaxTris = 4u;
+// Leaves hold one triangle: on the 1080p spectral workload leaves of at most
+// 1, 2, 3 and 4 triangles traced 3.56, 3.48, 3.38 and 3.24 Gbounces/s (the
+// tighter boxes cut triangle tests from 3.9 to 1.3 per ray for 8.4 instead of
+// 7.7 node visits); the format keeps room for up to 16.
+constexpr uint32_t kLeafMaxTris = 1u;
 DEV uint32_t leafFirst(uint32_t ref) { return ref & 0x07FFFFFFu; }
 DEV uint32_t leafCount(uint32_t ref) { return ((ref >> 27) & 0xFu) + 1u; }
 

@@ -362,9 +362,14 @@ DEV bool closestStep(const SceneView& s, const TraversalRay& r, float tmin, Clos
         for (uint32_t i = 0; i < count; i++) {
             if (triTests) (*triTests)++;
             const TriRecord tr = s.tris[first + i];
+            const uint32_t tri = __float_as_uint(tr.v0.w);
+            // the closest hit is the smallest t and, among equal t (two triangles
+            // sharing an edge), the lowest triangle id, whatever order the leaves
+            // are visited in: a lower id may also take the current t itself
+            const float tcap = (cs.h.tri != kNoHit && tri < cs.h.tri)
+                                   ? __uint_as_float(__float_as_uint(cs.h.t) + 1u) : cs.h.t;
             float t, u, v;
-            if (intersectTriangle(tr, r.o, r.pre, tmin, cs.h.t, t, u, v)) {
-                const uint32_t tri = __float_as_uint(tr.v0.w);
+            if (intersectTriangle(tr, r.o, r.pre, tmin, tcap, t, u, v)) {
                 if (ALPHA && s.triAlpha && !alphaAccept<TEXA>(s, tri, u, v, alphaKey)) continue;
                 cs.h.t = t;
                 cs.h.tri = tri;
