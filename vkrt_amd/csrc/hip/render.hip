@@ -222,6 +222,11 @@ DEV LightSurfaceSample sampleLightSurface(const SceneView& sv, const SceneData& 
     return s;
 }
 
+// ((x % W) + W) % W for x in [-W, 2W): the texel columns of the bilinear
+// footprint, x0 = floor(u W - 1/2) in [-1, W - 1] and x1 = x0 + 1 in [0, W]
+// for u in [0, 1], need one wrap at most (no integer division)
+DEV int wrapOnce(int x, int W) { return x < 0 ? x + W : (x >= W ? x - W : x); }
+
 // src/shaders/light/environment.slang:9-27 with the reference's environment
 // sampler (linear filter, repeat U, clamp-to-edge V, level 0)
 DEV F3 environmentRadiance(const SceneView& sv, const SceneData& sd, F3 dirIn) {
@@ -238,8 +243,8 @@ DEV F3 environmentRadiance(const SceneView& sv, const SceneData& sd, F3 dirIn) {
     float tx = fx - x0f, ty = fy - y0f;
     int x0 = (int)x0f, y0 = (int)y0f;
     int x1 = x0 + 1, y1 = y0 + 1;
-    x0 = ((x0 % W) + W) % W;
-    x1 = ((x1 % W) + W) % W;
+    x0 = wrapOnce(x0, W);
+    x1 = wrapOnce(x1, W);
     y0 = min(max(y0, 0), H - 1);
     y1 = min(max(y1, 0), H - 1);
     F3 a00 = xyz(sv.env[(size_t)y0 * W + x0]), a10 = xyz(sv.env[(size_t)y0 * W + x1]);
@@ -1731,8 +1736,8 @@ DEV void environmentGradient(const SceneView& sv, const SceneData& sd, F3 dirIn,
     float tx = fx - x0f, ty = fy - y0f;
     int x0 = (int)x0f, y0 = (int)y0f;
     int x1 = x0 + 1, y1 = y0 + 1;
-    x0 = ((x0 % W) + W) % W;
-    x1 = ((x1 % W) + W) % W;
+    x0 = wrapOnce(x0, W);
+    x1 = wrapOnce(x1, W);
     y0 = min(max(y0, 0), H - 1);
     y1 = min(max(y1, 0), H - 1);
     const float s = sd.environmentLight[3];
