@@ -163,6 +163,13 @@ DEV float etaFor(const BMat& m, uint32_t ff, float l, uint32_t spectral) {
     if (spectral) ior = dispersiveIor(ior, m.abbeNumber, l);
     return ff ? ior : 1.0f / ior;
 }
+// Whether eta depends on the wavelength: dispersiveIor returns max(ior, 1)
+// for every wavelength when this is false, so the hero lanes of such a
+// material share one eta and everything computed from it.
+DEV bool dispersive(const BMat& m) {
+    const float ior = interfaceIor(m);
+    return !(ior <= 1.0f + 1e-4f || m.abbeNumber <= 0.0f);
+}
 DEV float refractionEtaFor(const BMat& m, uint32_t ff, float l, uint32_t spectral) {
     float ior = interfaceIor(m);
     if (spectral) ior = dispersiveIor(ior, m.abbeNumber, l);
@@ -347,29 +354,48 @@ DEV bool transHalf(F3 wo, F3 wi, float etap, F3& wm, float& woD, float& wiD, flo
     return woD > 0.0f && wiD < 0.0f && fabsf(den) > 1e-6f;
 }
 // One transmission lane: value (given the lane's transmission colour) and pdf.
-DEV float evalTransLane(const BMat& m, F3 wo, F3 wi, uint32_t ff, const GGX& g, float coatAtt, float tc, float l,
-                        uint32_t spectral, float& pdf) {
-    pdf = 0.0f;
-    if (m.transmission <= 0.0f || wo.z <= 0.0f || wi.z >= 0.0f) return 0.0f;
+// The transmission lobe of one wavelength lane in two parts, so that lanes
+// with the same eta share the first: its value is coatAtt * tc * x, or
+// coatAtt * tc / x for an index-matched interface, or 0 (kind 0, 1, 2).
+struct TransPart {
+    float x, pdf;
+    uint32_t kind;
+};
+DEV TransPart transPart(const BMat& m, F3 wo, F3 wi, uint32_t ff, const GGX& g, float l, uint32_t spectral) {
+    TransPart t{0.0f, 0.0f, 0u};
+    if (m.transmission <= 0.0f || wo.z <= 0.0f || wi.z >= 0.0f) return t;
     if (dielectricIdentity(m)) {
-        if (dot(wi, -wo) < 0.9999f) return 0.0f;
-        pdf = 1.0f;
-        return coatAtt * tc / fmaxf(fabsf(wi.z), 1e-6f);
+        if (dot(wi, -wo) < 0.9999f) return t;
+        t.pdf = 1.0f;
+        t.x = fmaxf(fabsf(wi.z), 1e-6f);
+        t.kind = 2u;
+        return t;
     }
     float etap = etaFor(m, ff, l, spectral);
     F3 wm;
     float woD, wiD, den;
-    if (!transHalf(wo, wi, etap, wm, woD, wiD, den)) return 0.0f;
+    if (!transHalf(wo, wi, etap, wm, woD, wiD, den)) return t;
     float F = fresnelDielectric(woD, etap);
     float tp = transmissionProb(m, F);
-    if (tp <= 0.0f) return 0.0f;
+    if (tp <= 0.0f) return t;
     float D = ggxD(wm, g), G = ggxG(wo, wi, g);
     float den2 = den * den;
     float ts = 1.0f / fmaxf(etap * etap, 1e-6f);
     float tt = fabsf((wiD * woD) / fmaxf(fabsf(wi.z) * wo.z * den2, 1e-6f));
     float dwm = fabsf(wiD) / fmaxf(den2, 1e-6f);
-    pdf = tp * ggxVisiblePdf(wo, wm, g) * dwm;
-    return coatAtt * tc * ((1.0f - F) * D * G * tt * ts);
+    t.pdf = tp * ggxVisiblePdf(wo, wm, g) * dwm;
+    t.x = (1.0f - F) * D * G * tt * ts;
+    t.kind = 1u;
+    return t;
+}
+DEV float transValue(const TransPart& t, float coatAtt, float tc) {
+    return t.kind == 1u ? coatAtt * tc * t.x : (t.kind == 2u ? coatAtt * tc / t.x : 0.0f);
+}
+DEV float evalTransLane(const BMat& m, F3 wo, F3 wi, uint32_t ff, const GGX& g, float coatAtt, float tc, float l,
+                        uint32_t spectral, float& pdf) {
+    const TransPart t = transPart(m, wo, wi, ff, g, l, spectral);
+    pdf = t.pdf;
+    return transValue(t, coatAtt, tc);
 }
 
 // --------------------------------------------------------- principled --
@@ -670,13 +696,19 @@ DEV F4 evalHero(const Spec& sp, const BState& s, F3 wi, F4 l, F4& tp) {
                 F3 h;
                 float wom;
                 if (ggxHalf(s.wo, wi, h, wom)) {
-                    F4 F = F4{fresnelDielectric(wom, etaFor(m, s.ff, l.x, 1u)), fresnelDielectric(wom, etaFor(m, s.ff, l.y, 1u)),
-                              fresnelDielectric(wom, etaFor(m, s.ff, l.z, 1u)), fresnelDielectric(wom, etaFor(m, s.ff, l.w, 1u))};
+                    F4 F, rp;
+                    if (dispersive(m)) {
+                        F = F4{fresnelDielectric(wom, etaFor(m, s.ff, l.x, 1u)), fresnelDielectric(wom, etaFor(m, s.ff, l.y, 1u)),
+                               fresnelDielectric(wom, etaFor(m, s.ff, l.z, 1u)), fresnelDielectric(wom, etaFor(m, s.ff, l.w, 1u))};
+                        rp = F4{1.0f - transmissionProb(m, F.x), 1.0f - transmissionProb(m, F.y),
+                                1.0f - transmissionProb(m, F.z), 1.0f - transmissionProb(m, F.w)};
+                    } else {  // one eta for every lane
+                        F = f4(fresnelDielectric(wom, etaFor(m, s.ff, l.x, 1u)));
+                        rp = f4(1.0f - transmissionProb(m, F.x));
+                    }
                     float D = ggxD(h, s.g), G = ggxG(s.wo, wi, s.g);
                     float rbp = ggxReflPdf(s.wo, h, s.g);
-                    dp = F4{1.0f - transmissionProb(m, F.x), 1.0f - transmissionProb(m, F.y),
-                            1.0f - transmissionProb(m, F.z), 1.0f - transmissionProb(m, F.w)} *
-                         rbp;
+                    dp = rp * rbp;
                     if (anyPos(dp)) {
                         F4 rc;
                         if (m.transmission > 0.0f && m.metallic <= 0.0f)
@@ -717,10 +749,16 @@ DEV F4 evalHero(const Spec& sp, const BState& s, F3 wi, F4 l, F4& tp) {
     float ca = interior(s) ? 1.0f : (SH ? sheenAtt(m.sheenColor, s.wo, m.sheenRoughness) : 1.0f) * coatAtt(m, s.wo);
     F4 tc = spec4(sp, sat(transmissionColor(m)), l) * ((1.0f - m.metallic) * m.transmission);
     F4 v;
-    v.x = evalTransLane(m, s.wo, wi, s.ff, s.g, ca, tc.x, l.x, 1u, tp.x);
-    v.y = evalTransLane(m, s.wo, wi, s.ff, s.g, ca, tc.y, l.y, 1u, tp.y);
-    v.z = evalTransLane(m, s.wo, wi, s.ff, s.g, ca, tc.z, l.z, 1u, tp.z);
-    v.w = evalTransLane(m, s.wo, wi, s.ff, s.g, ca, tc.w, l.w, 1u, tp.w);
+    if (dispersive(m)) {
+        v.x = evalTransLane(m, s.wo, wi, s.ff, s.g, ca, tc.x, l.x, 1u, tp.x);
+        v.y = evalTransLane(m, s.wo, wi, s.ff, s.g, ca, tc.y, l.y, 1u, tp.y);
+        v.z = evalTransLane(m, s.wo, wi, s.ff, s.g, ca, tc.z, l.z, 1u, tp.z);
+        v.w = evalTransLane(m, s.wo, wi, s.ff, s.g, ca, tc.w, l.w, 1u, tp.w);
+    } else {  // one eta for every lane: one evaluation of everything but the lane's colour
+        const TransPart t = transPart(m, s.wo, wi, s.ff, s.g, l.x, 1u);
+        v = F4{transValue(t, ca, tc.x), transValue(t, ca, tc.y), transValue(t, ca, tc.z), transValue(t, ca, tc.w)};
+        tp = f4(t.pdf);
+    }
     tp = tp * s.w.dielectric;
     return v;
 }
