@@ -236,7 +236,10 @@ VKRT_Result VKRT_exportSceneBuffers(VKRT* vkrt, VKRT_SceneBuffers* out);
  * Outputs per ray: mesh (instance) index, primitive index within the mesh,
  * hit distance and barycentrics (u,v) -- the payload of the reference's
  * closest-hit shader (src/shaders/entry/path/closest_hit.slang:3-5);
- * a miss reports VKRT_INVALID_INDEX. */
+ * a miss reports VKRT_INVALID_INDEX. The closest hit is the smallest distance
+ * and, among triangles at exactly the same distance (a ray through a shared
+ * edge), the one with the lowest global triangle index, whatever the tree's
+ * visiting order; the reference's driver traversal leaves that choice open. */
 VKRT_Result VKRT_traceRays(VKRT* vkrt, const float* rays, size_t count, uint32_t* outMesh, uint32_t* outPrimitive,
                            float* outT, float* outBarycentrics);
 
