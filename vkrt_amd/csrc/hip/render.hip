@@ -84,6 +84,9 @@ struct FrameParams {
     float nrcBoxMin[3], nrcBoxInv[3];
     // kGenerate wrote only the primary rays; kShade builds each path's state at depth 0
     uint32_t leanStart;
+    // lean start without the feature pass: kGenerate stores only the primary rays'
+    // directions; their common origin (the camera's) is recomputed where read
+    uint32_t compactPrimary;
     const float4* accum;  // accumulation image (sample counts for the random streams)
 };
 
@@ -430,8 +433,14 @@ __global__ __launch_bounds__(kBlock) void kGenerate(FrameParams fp, PathView pv,
         if (MODE != kModeHero) pv.misc[i] = make_float4(0.0f, 1.0f, 0.0f, 0.0f);
         // sigma is read only inside an absorbing medium (kFAbsorption), after a transmission wrote it
     }
-    q.o[i] = make_float4(o.x, o.y, o.z, __uint_as_float(i));
+    if (FULL || !fp.compactPrimary) q.o[i] = make_float4(o.x, o.y, o.z, __uint_as_float(i));
     q.d[i] = make_float4(d.x, d.y, d.z, kRayTMax);
+}
+
+// The origin every primary ray has (primaryRay's org), from the same arithmetic.
+DEV F3 cameraOrigin(const SceneData& sd) {
+    F4 org = mulM4(sd.viewInverse, F4{0.0f, 0.0f, 0.0f, 1.0f});
+    return F3{org.x, org.y, org.z};
 }
 
 __global__ void kPrepDepth(uint32_t* counters, int cur, int nxt, uint32_t depth) {
@@ -476,7 +485,8 @@ constexpr uint32_t kNoRay = 0xFFFFFFFFu;
 template <bool COUNT, bool ALPHA, bool TEXA = false>
 __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(TEXA ? 4 : 6))) void kExtend(SceneView sv, const float4* ro, const float4* rd, float4* hits,
                                                       uint32_t* counters, int countSlot, uint32_t* gstack,
-                                                      uint32_t alphaSalt, uint32_t pathBase) {
+                                                      uint32_t alphaSalt, uint32_t pathBase, float4 camO) {
+    // camO.w != 0: the queue holds primary rays without stored origins; camO.xyz is theirs
     uint32_t nodeVisits = 0, triTests = 0;
     __shared__ uint32_t lds[kLdsStackWords];
     TraversalStack st;
@@ -521,7 +531,8 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(TEX
             if (rayId == kNoRay) {
                 const uint32_t k = begin + base + (uint32_t)__popcll(idleMask & lanesBelow);
                 if (base < partSize && k < end) {
-                    float4 a = ro[k], b = rd[k];
+                    float4 a = camO.w != 0.0f ? make_float4(camO.x, camO.y, camO.z, __uint_as_float(k)) : ro[k];
+                    float4 b = rd[k];
                     r = makeTraversalRay(xyz(a), xyz(b));
                     if (reachesSceneBox(sv, r, kRayTMin, b.w)) {
                         closestBegin(cs, st, b.w);
@@ -1121,7 +1132,11 @@ __global__ __launch_bounds__(kBlock, SHEEN ? 2 : 3) void kShade(FrameParams fp, 
     uint32_t q = 0;
     if (i < n) {
         const uint32_t j = order ? order[i] : i;  // shading order (sorted by material class) or queue order
-        const float4 ro = in.o[j], rdir = in.d[j], hh = hits[j];
+        // depth 0 of a compact frame: the ray's origin is the camera's, its path index its queue slot
+        F3 co = f3(0.0f);
+        if (fp.compactPrimary && fp.depth == 0u) co = cameraOrigin(fp.sd);
+        const float4 ro = fp.compactPrimary && fp.depth == 0u ? make_float4(co.x, co.y, co.z, __uint_as_float(j)) : in.o[j];
+        const float4 rdir = in.d[j], hh = hits[j];
         q = __float_as_uint(ro.w);
         shadeVertex<MODE, NRC, TEX, SHEEN>(fp, fp.depth, sv, pv, ro, rdir, Hit{hh.x, __float_as_uint(hh.y), hh.z, hh.w},
                                            sh, she, counters, nv, cont, nextO, nextD);
@@ -2207,26 +2222,27 @@ static void queueViews(mrt_context* ctx, QueueView qv[2]) {
 
 // kExtend in the variant the scene needs: counting statistics or not, alpha test or not.
 static void launchExtend(mrt_context* ctx, bool countTraversal, const SceneView& sv, const QueueView& q, float4* hits,
-                         uint32_t* cnt, int slot, uint32_t salt, uint32_t pathBase) {
+                         uint32_t* cnt, int slot, uint32_t salt, uint32_t pathBase,
+                         float4 camO = make_float4(0.0f, 0.0f, 0.0f, 0.0f)) {
     hipStream_t s = ctx->stream;
     const bool alpha = sv.triAlpha != nullptr;
     const dim3 grid(ctx->traceGrid), block(kTraceBlock);
     uint32_t* stack = ctx->traversalStack.ptr;
     if (countTraversal && alpha)
         hipLaunchKernelGGL(HIP_KERNEL_NAME(kExtend<true, true, true>), grid, block, 0, s, sv, q.o, q.d, hits, cnt, slot, stack,
-                           salt, pathBase);
+                           salt, pathBase, camO);
     else if (countTraversal)
         hipLaunchKernelGGL(HIP_KERNEL_NAME(kExtend<true, false>), grid, block, 0, s, sv, q.o, q.d, hits, cnt, slot, stack,
-                           salt, pathBase);
+                           salt, pathBase, camO);
     else if (alpha && sv.texels)
         hipLaunchKernelGGL(HIP_KERNEL_NAME(kExtend<false, true, true>), grid, block, 0, s, sv, q.o, q.d, hits, cnt,
-                           slot, stack, salt, pathBase);
+                           slot, stack, salt, pathBase, camO);
     else if (alpha)
         hipLaunchKernelGGL(HIP_KERNEL_NAME(kExtend<false, true, false>), grid, block, 0, s, sv, q.o, q.d, hits, cnt,
-                           slot, stack, salt, pathBase);
+                           slot, stack, salt, pathBase, camO);
     else
         hipLaunchKernelGGL(HIP_KERNEL_NAME(kExtend<false, false>), grid, block, 0, s, sv, q.o, q.d, hits, cnt, slot,
-                           stack, salt, pathBase);
+                           stack, salt, pathBase, camO);
 }
 
 // kShade variant for the scene: RGB with the neural cache, textured, sheen.
@@ -2270,6 +2286,7 @@ static bool launchChunk(mrt_context* ctx, FrameParams fp, StageEvents& ev, bool 
     }
     ShadowView shv{b.shadow[0].ptr, b.shadow[1].ptr, b.shadow[2].ptr, b.shadow[3].ptr, cnt + kCntShadow};
     ShadowView shev{b.shadowEnv[0].ptr, b.shadowEnv[1].ptr, b.shadowEnv[2].ptr, nullptr, cnt + kCntShadowEnv};
+    fp.compactPrimary = fp.leanStart && !fv ? 1u : 0u;
     if (fp.leanStart)
         hipLaunchKernelGGL(HIP_KERNEL_NAME(kGenerate<MODE, false>), dim3(blocksFor(fp.count)), dim3(kBlock), 0, s, fp, pv,
                            qv[0]);
@@ -2295,8 +2312,16 @@ static bool launchChunk(mrt_context* ctx, FrameParams fp, StageEvents& ev, bool 
         }
         bool timed = ev.begin(s, 0);
         const uint32_t pathBase = (uint32_t)fp.pathBegin;
+        float4 camO = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (fp.compactPrimary && depth == 0u) {
+            // primaryRay's origin arithmetic, on the host
+            const auto& m = fp.sd.viewInverse;
+            float org[3];
+            for (int r = 0; r < 3; r++) org[r] = m[0][r] * 0.0f + m[1][r] * 0.0f + m[2][r] * 0.0f + m[3][r] * 1.0f;
+            camO = make_float4(org[0], org[1], org[2], 1.0f);
+        }
         launchExtend(ctx, countTraversal, sv, qv[cur], b.hits.ptr, cnt, cur,
-                     alphaSalt(fp.sd.frameNumber, fp.rngSeed, depth, 0u), pathBase);
+                     alphaSalt(fp.sd.frameNumber, fp.rngSeed, depth, 0u), pathBase, camO);
         if (timed) ev.end(s);
         timed = ev.begin(s, 1);
         const uint32_t* order = nullptr;
