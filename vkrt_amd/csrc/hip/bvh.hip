@@ -12,7 +12,8 @@
 //     tree of surface-area-heuristic quality.
 //  5. top-down collapse into 8-wide nodes: a node is opened greedily by
 //     splitting its largest-area child until eight children remain; subtrees
-//     of at most kLeafMaxTris triangles (one) become leaves whose triangles are
+//     of at most leafMax triangles (kLeafMaxTris, or kLeafMaxTrisLarge for
+//     scenes above kLargeScene triangles) become leaves whose triangles are
 //     written contiguously (depth-first order) as 48-byte records; child
 //     boxes are quantised to 8 bits per plane, conservatively
 #include <cstring>
@@ -240,13 +241,14 @@ __global__ __launch_bounds__(kB) void kCollapse(const CollapseItem* in, const ui
                                                uint32_t* outCount, uint32_t* nodeAlloc, uint32_t n,
                                                const int2* children, const uint32_t* counts, const Aabb* boxes,
                                                const uint32_t* sortedTri, const float* tris9, TriRecord* triOut,
-                                               BvhNode* nodes, uint32_t nodeCapacity, uint32_t* overflow) {
+                                               BvhNode* nodes, uint32_t nodeCapacity, uint32_t* overflow,
+                                               uint32_t leafMax) {
     uint32_t w = blockIdx.x * kB + threadIdx.x;
     if (w >= *inCount) return;
     const CollapseItem it = in[w];
     uint32_t slot[kBvhWidth], first[kBvhWidth];
     int cnt = 0;
-    if (it.node < n || counts[it.node] <= kLeafMaxTris) {
+    if (it.node < n || counts[it.node] <= leafMax) {
         slot[cnt] = it.node;  // only the root of a tiny scene
         first[cnt++] = it.first;
     } else {
@@ -261,7 +263,7 @@ __global__ __launch_bounds__(kB) void kCollapse(const CollapseItem* in, const ui
             int best = -1;
             float bestArea = -1.0f;
             for (int k = 0; k < cnt; k++) {
-                if (counts[slot[k]] <= kLeafMaxTris) continue;
+                if (counts[slot[k]] <= leafMax) continue;
                 float a = halfArea(boxes[slot[k]]);
                 if (a > bestArea) {
                     bestArea = a;
@@ -293,9 +295,9 @@ __global__ __launch_bounds__(kB) void kCollapse(const CollapseItem* in, const ui
         const uint32_t nd = slot[k];
         const uint32_t count = counts[nd];
         uint32_t ref;
-        if (count <= kLeafMaxTris) {
+        if (count <= leafMax) {
             // the subtree's triangles, depth first, into [first, first + count)
-            uint32_t stack[kLeafMaxTris];
+            uint32_t stack[kLeafMaxTrisLarge];
             int sp = 0;
             uint32_t written = 0;
             stack[sp++] = nd;
@@ -368,7 +370,7 @@ extern "C" int mrt_build_bvh(mrt_context* ctx, mrt_bvh_stats* stats) {
     }
     hipEventRecord(ctx->evStage[0], s);
     const size_t binNodes = (size_t)2 * n - 1;
-    const size_t wideCap = n;  // every wide node but the root is a distinct binary node with > kLeafMaxTris triangles
+    const size_t wideCap = n;  // every wide node but the root is a distinct binary node with more than one triangle
     DeviceArray<Aabb> triBoxes, boxes;
     DeviceArray<uint32_t> cb, vals, valsSorted, counts, nodeCounts, clusters[2], selected;
     DeviceArray<uint64_t> keys, keysSorted;
@@ -484,7 +486,7 @@ extern "C" int mrt_build_bvh(mrt_context* ctx, mrt_bvh_stats* stats) {
         hipLaunchKernelGGL(kCollapse, dim3(blocks(frontier)), dim3(kB), 0, s, front[fc].ptr, counts.ptr,
                            front[1 - fc].ptr, counts.ptr + 1, counts.ptr + 2, n, children.ptr, nodeCounts.ptr,
                            boxes.ptr, valsSorted.ptr, ctx->worldTris.ptr, ctx->tris.ptr, ctx->nodes.ptr,
-                           (uint32_t)wideCap, counts.ptr + 3);
+                           (uint32_t)wideCap, counts.ptr + 3, n > kLargeScene ? kLeafMaxTrisLarge : kLeafMaxTris);
         // the next frontier becomes the current one
         hipMemcpyAsync(counts.ptr, counts.ptr + 1, sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
         if (!check(hipMemcpyAsync(hostCounts, counts.ptr, sizeof(hostCounts), hipMemcpyDeviceToHost, s), "collapse") ||
