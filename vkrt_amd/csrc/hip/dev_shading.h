@@ -379,9 +379,12 @@ DEV F4 spec4(const Spec& sp, F3 rgb, F4 l) {
 }
 
 // src/shaders/utility/spectral.slang:66-118
+// exp(-t²/2) = exp2(-(t·k)²) with k = sqrt(log2(e)/2) folded into the slopes:
+// one hardware exp2 per lobe instead of expf's extended-precision range reduction
+constexpr float kLobeScale = 0.849321800288019f;
 DEV float lobe(float l, float mu, float a, float b) {
-    float t = (l - mu) * (l < mu ? a : b);
-    return expf(-0.5f * t * t);
+    float t = (l - mu) * (l < mu ? a * kLobeScale : b * kLobeScale);
+    return exp2f(-(t * t));
 }
 DEV F3 cie(float l) {
     float x = 0.362f * lobe(l, 442.0f, 0.0624f, 0.0374f) + 1.056f * lobe(l, 599.8f, 0.0264f, 0.0323f) -
@@ -391,13 +394,14 @@ DEV F3 cie(float l) {
     return F3{fmaxf(x, 0.0f), fmaxf(y, 0.0f), fmaxf(z, 0.0f)};
 }
 constexpr float kCieYIntegral = 106.9461715f;
-DEV F3 toXYZ1(float v, float l, float invPdf) { return cie(l) * ((v * invPdf) / kCieYIntegral); }
+constexpr float kInvCieYIntegral = 1.0f / kCieYIntegral;
+DEV F3 toXYZ1(float v, float l, float invPdf) { return cie(l) * ((v * invPdf) * kInvCieYIntegral); }
 DEV F3 toXYZ4(F4 v, F4 l, F4 ip) {
     F3 r = f3(0.0f);
-    r = r + cie(l.x) * ((v.x * ip.x) / kCieYIntegral);
-    r = r + cie(l.y) * ((v.y * ip.y) / kCieYIntegral);
-    r = r + cie(l.z) * ((v.z * ip.z) / kCieYIntegral);
-    r = r + cie(l.w) * ((v.w * ip.w) / kCieYIntegral);
+    r = r + cie(l.x) * ((v.x * ip.x) * kInvCieYIntegral);
+    r = r + cie(l.y) * ((v.y * ip.y) * kInvCieYIntegral);
+    r = r + cie(l.z) * ((v.z * ip.z) * kInvCieYIntegral);
+    r = r + cie(l.w) * ((v.w * ip.w) * kInvCieYIntegral);
     return r;
 }
 DEV F3 xyzToSrgb(F3 c) {
