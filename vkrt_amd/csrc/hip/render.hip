@@ -1288,15 +1288,17 @@ __global__ __launch_bounds__(kBlock) void kFinalizeReduce(uint32_t numPixels, ui
                                                           PathView pv, float4* frameSum) {
     uint32_t k = blockIdx.x * kBlock + threadIdx.x;
     if (k >= numPixels) return;
-    uint64_t end = pathBegin + count;
-    uint64_t s0 = pathBegin / numPixels;
+    const uint64_t end = pathBegin + count;
+    if (end <= k) return;  // no sample of this pixel in the chunk
+    // samples s with pathBegin <= s * numPixels + k < end, counted before the
+    // loop so that its loads can be issued ahead of the additions
+    const uint64_t sBegin = pathBegin > k ? (pathBegin - k + numPixels - 1u) / numPixels : 0u;
+    const uint64_t sEnd = (end - 1u - k) / numPixels + 1u;
     F4 acc = ld4(frameSum[k]);
+    uint32_t i = (uint32_t)(sBegin * numPixels + k - pathBegin);
 #pragma unroll 4
-    for (uint64_t s = s0;; s++) {
-        uint64_t p = s * numPixels + k;
-        if (p >= end) break;
-        if (p < pathBegin) continue;
-        F3 v = pathRadiance<MODE>(pv, (uint32_t)(p - pathBegin));
+    for (uint64_t s = sBegin; s < sEnd; s++, i += numPixels) {
+        F3 v = pathRadiance<MODE>(pv, i);
         acc = F4{acc.x + v.x, acc.y + v.y, acc.z + v.z, 0.0f};
     }
     frameSum[k] = st4(acc);
